@@ -1,0 +1,1165 @@
+/*
+ * ORACLE — test infrastructure only (see mcac_oracle.h).
+ *
+ * CPU restatement of the reference MC loop for the aggregate-move /
+ * collision-search path, keeping the reference's data-structure semantics
+ * (label-indexed aggregate rows erased on merge, Verlet cells as ordered
+ * sets of labels, multimap candidate order, libstdc++ std::sort pick table,
+ * sequential FP64 sums in list order) so that its results, and its cost
+ * profile, follow the reference.  Build: -O2 -ffp-contract=off.
+ *
+ * Pinning (what checks each part; see tests/test_oracle_*.py):
+ *   RNG stream, random_direction, inverf/inverfc  -> golden vectors written by
+ *       oracle/_ref (reference tools.cpp compiled as-is).
+ *   Verlet query order (a4)                       -> golden vectors from
+ *       oracle/_ref (reference verlet.cpp compiled as-is).
+ *   PhysicalModel derived scalars                 -> known answers of
+ *       pymcac/tests/test_read.py:27-52 (L, lambda, Mu, rpeqmass, gamma_, u_sg).
+ *   swept-sphere primitive (a9)                   -> closed forms (head-on,
+ *       grazing, miss, touching, behind, periodic images, end contact) and a
+ *       brute-force ray march.
+ *   aggregate-pair argmin (a8), ordered scan (a7) -> brute force on tiny inputs.
+ *   merge / update (a10, a11)                     -> closed forms (two touching
+ *       spheres: CoM at midpoint, rmax = 2r, Rg^2 = r^2 + 3/5 r^2, V = 2 v).
+ *   full loop (a1)                                -> invariants (sphere count,
+ *       total volume, merge count) and the product's bit-exact agreement.
+ * The parts of the reference that need inipp.h / <gsl/gsl> / libunwind.h
+ * (physical_model.cpp, the sphere/aggregate classes) cannot be compiled in
+ * this image, so they are pinned by the checks above, not by _ref.
+ */
+#include "mcac_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <list>
+#include <map>
+#include <numeric>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+// constants.hpp:113-128
+const double kPi = std::atan(1.0) * 4;
+const double kVolumeFactor = 4 * kPi / 3;
+const double kSurfaceFactor = 4 * kPi;
+const double kBoltzmann = 1.38066E-23;
+const double kLambdaRef = 66.5E-9;
+const double kTemperatureRef = 293.15;
+const double kSutherland = 110;
+const double kPressureRef = 101300;
+const double kViscosityRef = 18.203E-6;
+const double kContactEpsilon = 1e-28;
+const double kCoordinationEpsilon = 1e-10;
+const double kInf = std::numeric_limits<double>::infinity();
+
+using V3 = std::array<double, 3>;
+inline V3 operator+(const V3 &a, const V3 &b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+inline V3 operator-(const V3 &a, const V3 &b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline V3 operator*(const V3 &a, double f) { return {a[0] * f, a[1] * f, a[2] * f}; }
+inline V3 operator*(double f, const V3 &a) { return {f * a[0], f * a[1], f * a[2]}; }
+
+// std::min / std::max semantics (libstdc++: min returns (b<a)?b:a, max (a<b)?b:a)
+inline double smin(double a, double b) { return (b < a) ? b : a; }
+inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+// physical_model.hpp:99-109
+inline double periodic_distance(double dist, double dim) {
+    double p = dist;
+    double half = 0.5 * dim;
+    while (p < -half) p += dim;
+    while (p >= half) p -= dim;
+    return p;
+}
+// physical_model.hpp:113-122
+inline double periodic_position(double pos, double dim) {
+    double p = pos;
+    while (p < 0) p += dim;
+    while (p >= dim) p -= dim;
+    return p;
+}
+inline V3 periodic_position(const V3 &p, double dim) {
+    return {periodic_position(p[0], dim), periodic_position(p[1], dim), periodic_position(p[2], dim)};
+}
+
+// sphere_distances.cpp:74-81 (distance_2 with periodicity)
+inline double distance_2(const V3 &p1, const V3 &p2, double box) {
+    V3 d = p1 - p2;
+    double dx = periodic_distance(d[0], box);
+    double dy = periodic_distance(d[1], box);
+    double dz = periodic_distance(d[2], box);
+    return dx * dx + dy * dy + dz * dz;
+}
+// sphere_distances.cpp:82-86
+inline double relative_distance_2(const V3 &p1, const V3 &p2) {
+    V3 d = p1 - p2;
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+// sphere_distances.cpp:87-96
+inline bool contact(const V3 &p1, double r1, const V3 &p2, double r2, double box) {
+    double distance = distance_2(p1, p2, box);
+    double s = r1 + r2;
+    double dist_contact = s * s;
+    return (distance - dist_contact <= kContactEpsilon);
+}
+
+// sphere_contact.cpp:47-125 — swept-sphere distance to contact
+double swept_sphere(const V3 &pos1, double r1, V3 pos2, double r2, const V3 &dir, double dist,
+                    double box) {
+    double dist_contact = r1 + r2;
+    double dist_contact_2 = dist_contact * dist_contact;
+    if (distance_2(pos1, pos2, box) <= dist_contact_2) {
+        return 0.;
+    }
+    V3 total = dir * dist;
+    int nper[3] = {0, 0, 0};
+    double zone[3] = {0., 0., 0.};
+    for (int l = 0; l < 3; ++l) {
+        double base = smin(pos1[l], pos1[l] + total[l]) - dist_contact;
+        double end = smax(pos1[l], pos1[l] + total[l]) + dist_contact;
+        zone[l] = end - base;
+        pos2[l] = std::fmod((pos2[l] - base), box);
+        if (pos2[l] < 0) {
+            pos2[l] += box;
+        }
+        pos2[l] += base;
+        nper[l] = static_cast<int>(std::floor(zone[l] / box));
+    }
+    double res = kInf;
+    for (int i = 0; i <= nper[0]; i++) {
+        for (int j = 0; j <= nper[1]; j++) {
+            for (int k = 0; k <= nper[2]; k++) {
+                V3 pos3{pos2[0] + i * box, pos2[1] + j * box, pos2[2] + k * box};
+                V3 diff = pos3 - pos1;
+                if (std::abs(diff[0]) > zone[0]) continue;
+                if (std::abs(diff[1]) > zone[1]) continue;
+                if (std::abs(diff[2]) > zone[2]) continue;
+                double proj = diff[0] * dir[0] + diff[1] * dir[1] + diff[2] * dir[2];
+                if (proj < 0) continue;
+                bool end_contact = relative_distance_2(pos1 + total, pos3) <= dist_contact_2;
+                if ((!end_contact) && dist < proj) continue;
+                V3 cross{diff[1] * dir[2] - diff[2] * dir[1], diff[2] * dir[0] - diff[0] * dir[2],
+                         diff[0] * dir[1] - diff[1] * dir[0]};
+                double c2 = cross[0] * cross[0] + cross[1] * cross[1] + cross[2] * cross[2];
+                if (c2 > dist_contact_2) continue;
+                res = smin(res, proj - std::sqrt(dist_contact_2 - c2));
+            }
+        }
+    }
+    return res;
+}
+
+// glibc TYPE_3 additive feedback generator, private stream (tools.cpp:41-55 uses
+// srand/rand; random_r with a 128-byte state is the same generator).
+struct Rng {
+    struct random_data rd;
+    char state[128];
+    // random_data points into state[]: never copy an Rng, re-seed it in place.
+    explicit Rng(unsigned seed) { reseed(seed); }
+    Rng(const Rng &) = delete;
+    Rng &operator=(const Rng &) = delete;
+    void reseed(unsigned seed) {
+        std::memset(&rd, 0, sizeof(rd));
+        std::memset(state, 0, sizeof(state));
+        initstate_r(seed, state, sizeof(state), &rd);
+        srandom_r(seed, &rd);
+    }
+    int next() {
+        int32_t v;
+        random_r(&rd, &v);
+        return v;
+    }
+    // tools.cpp:51-55
+    double uniform() {
+        double v = next();
+        v = v / RAND_MAX;
+        return v;
+    }
+    // tools.cpp:82-89
+    V3 direction() {
+        double theta = uniform() * 2 * kPi;
+        double phi = std::acos(1 - 2 * uniform());
+        return {std::sin(phi) * std::cos(theta), std::sin(phi) * std::sin(theta), std::cos(phi)};
+    }
+};
+
+// tools.cpp:56-74
+double inverfc(double p) {
+    if (p >= 2.) return -100.;
+    if (p <= 0.0) return 100.;
+    double pp = (p < 1.0) ? p : 2. - p;
+    double t = std::sqrt(-2. * std::log(pp / 2.));
+    double x = -0.70711 * ((2.30753 + t * 0.27061) / (1. + t * (0.99229 + t * 0.04481)) - t);
+    for (int j = 0; j < 2; j++) {
+        double err = std::erfc(x) - pp;
+        x += err / (1.12837916709551257 * std::exp(-(x * x)) - x * err);
+    }
+    return (p < 1.0 ? x : -x);
+}
+// tools.cpp:75-77
+double inverf(double p) { return inverfc(1. - p); }
+
+// ---------------------------------------------------------------- INI (inipp semantics)
+using Ini = std::map<std::string, std::map<std::string, std::string>>;
+
+std::string trim(const std::string &s) {
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace(static_cast<unsigned char>(s[a]))) a++;
+    while (b > a && std::isspace(static_cast<unsigned char>(s[b - 1]))) b--;
+    return s.substr(a, b - a);
+}
+Ini parse_ini(const std::string &text) {
+    Ini ini;
+    std::istringstream is(text);
+    std::string line, section;
+    while (std::getline(is, line)) {
+        line = trim(line);
+        if (line.empty() || line[0] == ';' || line[0] == '#') continue;
+        if (line.front() == '[' && line.back() == ']') {
+            section = trim(line.substr(1, line.size() - 2));
+            continue;
+        }
+        auto eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        ini[section][trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+    }
+    return ini;
+}
+// inipp::extract: leaves dst untouched on missing key or parse failure
+template <class T> void extract(const Ini &ini, const char *sec, const char *key, T &dst) {
+    auto s = ini.find(sec);
+    if (s == ini.end()) return;
+    auto k = s->second.find(key);
+    if (k == s->second.end()) return;
+    std::istringstream is(k->second);
+    T result;
+    char c;
+    if ((is >> std::boolalpha >> result) && !(is >> c)) dst = result;
+}
+void extract(const Ini &ini, const char *sec, const char *key, std::string &dst) {
+    auto s = ini.find(sec);
+    if (s == ini.end()) return;
+    auto k = s->second.find(key);
+    if (k == s->second.end()) return;
+    dst = k->second;
+}
+
+enum { LOGNORMAL = 0, NORMAL = 1 };
+
+// PhysicalModel (physical_model.cpp:32-287, 536-617), the parts this path uses.
+struct Physics {
+    double fractal_dimension = 1.4, fractal_prefactor = 1.8;
+    double flux_surfgrowth = 0., u_sg = 0.;
+    double pressure = kPressureRef, temperature = kTemperatureRef;
+    double gaz_mean_free_path = kLambdaRef, viscosity = kViscosityRef, density = 1800.;
+    double mean_diameter = 30., dispersion_diameter = 1.0;
+    double mean_massic_radius = 0., friction_exponnant = 0.;
+    double time = 0.;
+    double volume_fraction = 1e-3, box_length = 0., box_volume = 0.;
+    double aggregate_concentration = 0., monomer_concentration = 0.;
+    double total_surface_concent = 0., total_volume_concent = 0.;
+    size_t n_verlet_divisions = 10;
+    size_t n_monomeres = 2500;
+    int init_mode = LOGNORMAL;
+    int volsurf_none = 1;
+    int pick_random = 1;
+    size_t n_iter_without_event = 0;
+    double physical_time_limit = -1;
+    int mean_monomere_per_aggregate_limit = -1;
+    size_t number_of_aggregates_limit = 1;
+    int n_iter_without_event_limit = -1;
+    int random_seed = -1;
+    bool with_domain_duplication = true, with_domain_reduction = false, with_nucleation = false;
+    bool with_collisions = true, with_surface_reactions = false, with_flame_coupling = false;
+    bool enforce_volume_fraction = true, with_potentials = false;
+    std::string unsupported;
+
+    void load(const Ini &ini) {
+        std::string s;
+        extract(ini, "monomers", "number", n_monomeres);
+        extract(ini, "monomers", "density", density);
+        extract(ini, "monomers", "dispersion_diameter", dispersion_diameter);
+        extract(ini, "monomers", "mean_diameter", mean_diameter);
+        extract(ini, "monomers", "initialisation_mode", s);
+        if (s == "normal") init_mode = NORMAL;
+        else if (s == "lognormal" || s.empty()) init_mode = LOGNORMAL;
+        else throw std::runtime_error("Monomere initialisation mode unknown: " + s);
+        extract(ini, "environment", "initial_time", time);
+        extract(ini, "environment", "volume_fraction", volume_fraction);
+        extract(ini, "environment", "temperature", temperature);
+        extract(ini, "environment", "pressure", pressure);
+        extract(ini, "environment", "fractal_prefactor", fractal_prefactor);
+        extract(ini, "environment", "fractal_dimension", fractal_dimension);
+        extract(ini, "surface_growth", "with_surface_reactions", with_surface_reactions);
+        extract(ini, "surface_growth", "flux_surfgrowth", flux_surfgrowth);
+        s.clear();
+        extract(ini, "surface_growth", "volsurf_method", s);
+        if (!s.empty() && s != "none") unsupported = "volsurf_method=" + s;
+        extract(ini, "nucleation", "with_nucleation", with_nucleation);
+        extract(ini, "limits", "number_of_aggregates", number_of_aggregates_limit);
+        extract(ini, "limits", "n_iter_without_event", n_iter_without_event_limit);
+        extract(ini, "limits", "physical_time", physical_time_limit);
+        extract(ini, "limits", "mean_monomere_per_aggregate", mean_monomere_per_aggregate_limit);
+        extract(ini, "numerics", "with_domain_duplication", with_domain_duplication);
+        extract(ini, "numerics", "with_domain_reduction", with_domain_reduction);
+        extract(ini, "numerics", "with_collisions", with_collisions);
+        extract(ini, "numerics", "enforce_volume_fraction", enforce_volume_fraction);
+        extract(ini, "numerics", "n_verlet_divisions", n_verlet_divisions);
+        s.clear();
+        extract(ini, "numerics", "pick_method", s);
+        if (!s.empty() && s != "random") unsupported = "pick_method=" + s;
+        extract(ini, "numerics", "random_seed", random_seed);
+        extract(ini, "inter_potential", "with_potentials", with_potentials);
+        extract(ini, "flame_coupling", "with_flame_coupling", with_flame_coupling);
+        if (with_surface_reactions) unsupported = "with_surface_reactions";
+        if (with_nucleation) unsupported = "with_nucleation";
+        if (with_potentials) unsupported = "with_potentials";
+        if (with_flame_coupling) unsupported = "with_flame_coupling";
+        if (with_domain_reduction) unsupported = "with_domain_reduction";
+        if (random_seed < 0) unsupported = "random_seed<0 (clock seeded)";
+        derive();
+    }
+    // physical_model.cpp:228-269
+    void derive() {
+        if (init_mode == NORMAL) {
+            box_length = mean_diameter * 1E-9 *
+                         std::pow(static_cast<double>(n_monomeres) * kPi / 6. / volume_fraction *
+                                      (1. + 3. * std::pow(dispersion_diameter / mean_diameter, 2)),
+                                  1. / 3.);
+            mean_massic_radius =
+                0.5 * 1E-9 *
+                (std::pow(mean_diameter, 4) + 6 * std::pow(mean_diameter, 2) * std::pow(dispersion_diameter, 2) +
+                 3 * std::pow(dispersion_diameter, 4)) /
+                (std::pow(mean_diameter, 3) + 3 * mean_diameter * std::pow(dispersion_diameter, 2));
+        } else {
+            box_length = mean_diameter * 1E-9 *
+                         std::pow(static_cast<double>(n_monomeres) * kPi / 6. / volume_fraction *
+                                      std::exp(9. / 2. * std::pow(std::log(dispersion_diameter), 2)),
+                                  1. / 3.);
+            mean_massic_radius =
+                0.5 * mean_diameter * 1E-9 * std::exp(1.5 * std::pow(std::log(dispersion_diameter), 2));
+        }
+        box_volume = std::pow(box_length, 3);
+        update_temperature(temperature);
+        u_sg = flux_surfgrowth / density;
+        aggregate_concentration = static_cast<double>(n_monomeres) / box_volume;
+        monomer_concentration = aggregate_concentration;
+    }
+    // physical_model.cpp:536-546
+    void update_temperature(double t) {
+        temperature = t;
+        viscosity = kViscosityRef * (kSutherland + kTemperatureRef) / (kSutherland + temperature) *
+                    std::pow(temperature / kTemperatureRef, 1.5);
+        gaz_mean_free_path = kLambdaRef * (kPressureRef / pressure) * (temperature / kTemperatureRef) *
+                             (1. + kSutherland / kTemperatureRef) / (1. + kSutherland / temperature);
+        friction_exponnant = 0.689 * (1. + std::erf(((gaz_mean_free_path / mean_massic_radius) + 4.454) / 10.628));
+    }
+    // physical_model.cpp:550-555
+    double cunningham(double r) const {
+        double a = 1.142, b = 0.558, c = 0.999;
+        return 1.0 + a * gaz_mean_free_path / r + b * gaz_mean_free_path / r * std::exp(-c * r / gaz_mean_free_path);
+    }
+    // physical_model.cpp:591-593
+    double friction_exponent(double r) const {
+        return 0.689 * (1. + std::erf(((gaz_mean_free_path / r) + 4.454) / 10.628));
+    }
+    // physical_model.cpp:594-600
+    double friction_coeff(double av, double sv, double sr) const {
+        double fe = friction_exponent(sr);
+        double cc = cunningham(sr);
+        return (6. * kPi * viscosity * sr / cc) * std::pow(av / sv, fe / fractal_dimension);
+    }
+    // physical_model.cpp:607-617
+    double mobility_diameter(double av, double sv, double sr) const {
+        double fe = friction_exponent(sr);
+        double ar = sr * std::pow(av / sv, fe / fractal_dimension / 2.0);
+        double cc_pp = cunningham(sr);
+        double cc_a = cunningham(ar);
+        return (cc_a / cc_pp) * 2.0 * sr * std::pow(av / sv, fe / fractal_dimension);
+    }
+    // physical_model.cpp:557-578 (random_normal: tools.cpp:78-81)
+    double random_diameter(Rng &rng) const {
+        double d = 0;
+        if (init_mode == NORMAL) {
+            d = mean_diameter + std::sqrt(2.) * dispersion_diameter * inverf(2. * rng.uniform() - 1.0);
+        } else {
+            if (dispersion_diameter < 1.0) throw std::runtime_error("dispersion_diameter cannot be lower than 1");
+            d = mean_diameter * std::pow(dispersion_diameter, std::sqrt(2.) * inverf(2. * rng.uniform() - 1.0));
+        }
+        if (d <= 0) d = mean_diameter;
+        return d * 1E-9;
+    }
+    // physical_model.cpp:489-498
+    void update(size_t n_agg, size_t n_mono, double tv, double ts) {
+        total_volume_concent = tv / box_volume;
+        total_surface_concent = ts / box_volume;
+        aggregate_concentration = static_cast<double>(n_agg) / box_volume;
+        monomer_concentration = static_cast<double>(n_mono) / box_volume;
+        volume_fraction = tv / box_volume;
+    }
+};
+
+struct Agg {
+    double rg = 0, f_agg = 0, lpm = 0, time_step = 0, rmax = 0, volume = 0, surface = 0;
+    V3 pos{0, 0, 0}, rcm{0, 0, 0};
+    double proper_time = 0, dp = 0, dg_over_dp = 0, overlapping = 0, coordination = 0, d_m = 0;
+    double bulk_density = 0;
+    std::vector<size_t> sph;           // myspheres, list order
+    std::vector<double> dcenter;       // distances_center
+    std::array<size_t, 3> vidx{0, 0, 0};
+    int64_t uid = 0;
+};
+
+struct Contact {
+    double distance = kInf;
+    int64_t ms = -1, os = -1;        // moving / other sphere
+    int64_t magg = -1, oagg = -1;    // moving / other aggregate, stable ids
+};
+
+struct Sim {
+    Physics pm;
+    Rng rng{0};
+    // SphereList storage (constants.hpp:34-45)
+    std::vector<double> sx, sy, sz, sr, svol, ssurf, srx, sry, srz;
+    std::vector<long> slabel;
+    std::vector<Agg> aggs;  // label order
+    // Verlet (verlet.hpp:28)
+    size_t n_div = 10;
+    double width = 0;
+    std::vector<std::set<size_t>> grid;
+    double maxradius = 0, avg_npp = 1, max_time_step = 0;
+    std::vector<size_t> index_sorted;
+    std::vector<double> cumulative;
+    // loop state (calcul.cpp:57-61)
+    bool event = true;
+    size_t total_events = 0;
+    int64_t step_count = 0;
+    bool finished_flag = false;
+    bool trace_steps = true, trace_merges = true;
+    std::vector<ora_step_rec> steps;
+    std::vector<ora_merge_rec> merges;
+    // counters of the last query (work accounting)
+    int64_t q_query = 0, q_filtered = 0, q_solved = 0, q_pairs = 0;
+
+    std::set<size_t> &cell(const std::array<size_t, 3> &i) { return grid[(i[0] * n_div + i[1]) * n_div + i[2]]; }
+    const std::set<size_t> &cell(size_t i, size_t j, size_t k) const { return grid[(i * n_div + j) * n_div + k]; }
+    V3 spos(size_t s) const { return {sx[s], sy[s], sz[s]}; }
+    V3 srel(size_t s) const { return {srx[s], sry[s], srz[s]}; }
+    void set_spos(size_t s, const V3 &p) { sx[s] = p[0]; sy[s] = p[1]; sz[s] = p[2]; }
+
+    // aggregat.cpp:699-705
+    std::array<size_t, 3> compute_index_verlet(const Agg &a) const {
+        double step = double(n_div) / pm.box_length;
+        return {size_t(std::floor(a.pos[0] * step)), size_t(std::floor(a.pos[1] * step)),
+                size_t(std::floor(a.pos[2] * step))};
+    }
+    // aggregat.cpp:109-118 + 706-717
+    void set_position(size_t label, const V3 &p, bool in_verlet = true) {
+        Agg &a = aggs[label];
+        a.pos = periodic_position(p, pm.box_length);
+        if (in_verlet) {
+            auto ni = compute_index_verlet(a);
+            if (ni != a.vidx) {
+                cell(a.vidx).erase(label);
+                a.vidx = ni;
+                cell(ni).insert(label);
+            }
+        }
+    }
+
+    // verlet.cpp:52-98
+    std::vector<size_t> verlet_query(const V3 &src, const V3 &vec, double distance) const {
+        double xp{src[0] + distance + smax(vec[0], 0.)};
+        double xm{src[0] - distance + smin(vec[0], 0.)};
+        double yp{src[1] + distance + smax(vec[1], 0.)};
+        double ym{src[1] - distance + smin(vec[1], 0.)};
+        double zp{src[2] + distance + smax(vec[2], 0.)};
+        double zm{src[2] - distance + smin(vec[2], 0.)};
+        int n = static_cast<int>(n_div);
+        int bi1 = static_cast<int>(std::floor(static_cast<double>(n_div) * xm / width));
+        int bi2 = static_cast<int>(std::floor(static_cast<double>(n_div) * xp / width) + 1);
+        int bj1 = static_cast<int>(std::floor(static_cast<double>(n_div) * ym / width));
+        int bj2 = static_cast<int>(std::floor(static_cast<double>(n_div) * yp / width) + 1);
+        int bk1 = static_cast<int>(std::floor(static_cast<double>(n_div) * zm / width));
+        int bk2 = static_cast<int>(std::floor(static_cast<double>(n_div) * zp / width) + 1);
+        if (bi2 - bi1 >= n) { bi1 = 0; bi2 = n - 1; }
+        if (bj2 - bj1 >= n) { bj1 = 0; bj2 = n - 1; }
+        if (bk2 - bk1 >= n) { bk1 = 0; bk2 = n - 1; }
+        std::list<size_t> tmp;
+        for (int i = bi1; i <= bi2; i++) {
+            for (int j = bj1; j <= bj2; j++) {
+                for (int k = bk1; k <= bk2; k++) {
+                    auto ii = static_cast<size_t>(periodic_position(i, static_cast<int>(n_div)));
+                    auto jj = static_cast<size_t>(periodic_position(j, static_cast<int>(n_div)));
+                    auto kk = static_cast<size_t>(periodic_position(k, static_cast<int>(n_div)));
+                    const auto &c = cell(ii, jj, kk);
+                    tmp.insert(tmp.end(), c.begin(), c.end());
+                }
+            }
+        }
+        return std::vector<size_t>(tmp.begin(), tmp.end());
+    }
+
+    // ------------------------------------------------------------ aggregate update
+    // aggregat.cpp:719-764 (volsurf none: the contact map itself is unused)
+    void update_distances_and_overlapping(Agg &a) {
+        a.overlapping = a.coordination = 0.0;
+        int intersections = 0;
+        const size_t n = a.sph.size();
+        a.dcenter.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            size_t si = a.sph[i];
+            for (size_t j = i + 1; j < n; j++) {
+                size_t sj = a.sph[j];
+                double dist = std::sqrt(relative_distance_2(srel(si), srel(sj)));
+                if (dist <= (1. + kCoordinationEpsilon) * (sr[si] + sr[sj])) {
+                    double c_ij = (sr[si] + sr[sj] - dist) / (sr[si] + sr[sj]);
+                    a.overlapping += 2.0 * c_ij;
+                    intersections += 2;
+                }
+            }
+        }
+        if (intersections > 0) {
+            a.overlapping /= static_cast<double>(intersections);
+            a.coordination = static_cast<double>(intersections) / static_cast<double>(n);
+        }
+    }
+    // aggregat.cpp:321-430 (method none)
+    void compute_volume_surface(Agg &a) {
+        a.volume = a.surface = 0.0;
+        for (size_t s : a.sph) {
+            a.volume = a.volume + svol[s];
+            a.surface = a.surface + ssurf[s];
+        }
+        if (a.volume <= 0 || a.surface <= 0) throw std::runtime_error("VolSurfError");
+    }
+    // aggregat.cpp:247-282 (+ 431-483)
+    void update_partial(size_t label, bool in_verlet) {
+        Agg &a = aggs[label];
+        const size_t n = a.sph.size();
+        // compute_mass_center :431-452
+        V3 r{0., 0., 0.};
+        for (size_t i = 0; i < n; i++) {
+            V3 t = srel(a.sph[i]) * svol[a.sph[i]];
+            r[0] += t[0];
+            r[1] += t[1];
+            r[2] += t[2];
+        }
+        r[0] /= a.volume;
+        r[1] /= a.volume;
+        r[2] /= a.volume;
+        a.dcenter.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            V3 diff = srel(a.sph[i]) - r;
+            a.dcenter[i] = std::sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+        }
+        set_position(label, spos(a.sph[0]) + r, in_verlet);
+        a.rcm = r;
+        // compute_max_radius :453-464
+        a.rmax = 0.0;
+        for (size_t i = 0; i < n; i++) a.rmax = smax(a.rmax, sr[a.sph[i]] + a.dcenter[i]);
+        // compute_giration_radius :465-483
+        double arg = 0., brg = 0.;
+        for (size_t i = 0; i < n; i++) {
+            double v = svol[a.sph[i]];
+            arg = arg + v * (a.dcenter[i] * a.dcenter[i]);
+            brg = brg + v * (sr[a.sph[i]] * sr[a.sph[i]]);
+        }
+        a.rg = std::sqrt(std::abs((arg + 3. / 5. * brg) / a.volume));
+        // :254-276
+        a.dp = 0.;
+        double vol_pp = 0.0;
+        for (size_t i = 0; i < n; i++) {
+            vol_pp += svol[a.sph[i]];
+            a.dp += sr[a.sph[i]];
+        }
+        a.dp = 2 * a.dp / static_cast<double>(n);
+        vol_pp = vol_pp / static_cast<double>(n);
+        a.bulk_density = pm.density;  // set_bulk_density without maturity :139-141
+        a.f_agg = pm.friction_coeff(a.volume, vol_pp, 0.5 * a.dp);
+        a.d_m = pm.mobility_diameter(a.volume, vol_pp, 0.5 * a.dp);
+        double masse = a.bulk_density * a.volume;
+        double relax_time = masse / a.f_agg;
+        a.time_step = 3. * relax_time;
+        double diffusivity = kBoltzmann * pm.temperature / a.f_agg;
+        a.lpm = std::sqrt(6. * diffusivity * a.time_step);
+        a.dg_over_dp = 2 * a.rg / a.dp;
+        if (a.rmax > maxradius) maxradius = a.rmax;
+    }
+    // aggregat.cpp:284-288
+    void update(size_t label, bool in_verlet = true) {
+        update_distances_and_overlapping(aggs[label]);
+        compute_volume_surface(aggs[label]);
+        update_partial(label, in_verlet);
+    }
+
+    // ------------------------------------------------------------ init
+    // aggregat_list.cpp:534-548 + aggregat_distance.cpp:45-58
+    bool test_free_space(const V3 &pos, double radius) const {
+        std::vector<size_t> nb = verlet_query(pos, {0, 0, 0}, radius + maxradius);
+        for (size_t suspect : nb) {
+            const Agg &a = aggs[suspect];
+            if (!contact(pos, radius, a.pos, a.rmax, pm.box_length)) continue;
+            for (size_t s : a.sph)
+                if (contact(pos, radius, spos(s), sr[s], pm.box_length)) return false;
+        }
+        return true;
+    }
+    // Same predicate with a private hash grid of the placed spheres; only valid
+    // while every aggregate is a monomer (initial placement).
+    struct HashGrid {
+        size_t n = 1;
+        double h = 1;
+        std::vector<std::vector<size_t>> cells;
+    } hg;
+    bool test_free_space_fast(const V3 &pos, double radius) const {
+        double reach = radius + maxradius;
+        double L = pm.box_length;
+        // reference Verlet window for exact membership (verlet.cpp:57-69)
+        double d = reach;
+        int n = static_cast<int>(n_div);
+        int b1[3], b2[3];
+        for (int l = 0; l < 3; l++) {
+            double p = pos[l] + d + 0.0;
+            double m = pos[l] - d + 0.0;
+            b1[l] = static_cast<int>(std::floor(static_cast<double>(n_div) * m / width));
+            b2[l] = static_cast<int>(std::floor(static_cast<double>(n_div) * p / width) + 1);
+            if (b2[l] - b1[l] >= n) { b1[l] = 0; b2[l] = n - 1; }
+        }
+        int span = static_cast<int>(std::ceil(reach / hg.h)) + 1;
+        int c0[3];
+        for (int l = 0; l < 3; l++) c0[l] = static_cast<int>(std::floor(periodic_position(pos[l], L) / hg.h));
+        int hn = static_cast<int>(hg.n);
+        int lim = std::min(span, (hn - 1) / 2);
+        bool full = span > (hn - 1) / 2;
+        for (int di = -lim; di <= lim + (full ? (hn - 1 - 2 * lim) : 0); di++)
+            for (int dj = -lim; dj <= lim + (full ? (hn - 1 - 2 * lim) : 0); dj++)
+                for (int dk = -lim; dk <= lim + (full ? (hn - 1 - 2 * lim) : 0); dk++) {
+                    size_t ci = static_cast<size_t>(((c0[0] + di) % hn + hn) % hn);
+                    size_t cj = static_cast<size_t>(((c0[1] + dj) % hn + hn) % hn);
+                    size_t ck = static_cast<size_t>(((c0[2] + dk) % hn + hn) % hn);
+                    for (size_t label : hg.cells[(ci * hg.n + cj) * hg.n + ck]) {
+                        const Agg &a = aggs[label];
+                        bool inside = true;
+                        for (int l = 0; l < 3 && inside; l++) {
+                            int c = static_cast<int>(a.vidx[l]);
+                            int off = ((c - b1[l]) % n + n) % n;
+                            inside = off <= b2[l] - b1[l];
+                        }
+                        if (!inside) continue;
+                        if (!contact(pos, radius, a.pos, a.rmax, L)) continue;
+                        for (size_t s : a.sph)
+                            if (contact(pos, radius, spos(s), sr[s], L)) return false;
+                    }
+                }
+        return true;
+    }
+
+    // aggregat_list_storage.cpp:52-88 + aggregat.cpp:162-229
+    int init(bool fast) {
+        const size_t N = pm.n_monomeres;
+        n_div = pm.n_verlet_divisions;
+        width = pm.box_length;
+        grid.assign(n_div * n_div * n_div, {});
+        sx.assign(N, 0.); sy.assign(N, 0.); sz.assign(N, 0.); sr.assign(N, 0.);
+        svol.assign(N, 0.); ssurf.assign(N, 0.); srx.assign(N, 0.); sry.assign(N, 0.); srz.assign(N, 0.);
+        slabel.assign(N, -1);
+        aggs.assign(N, Agg());
+        rng.reseed(static_cast<unsigned>(pm.random_seed));
+        if (fast) {
+            double target = 2e-9 * pm.mean_diameter;  // two mean diameters
+            hg.n = std::min<size_t>(128, std::max<size_t>(3, static_cast<size_t>(pm.box_length / target)));
+            hg.h = pm.box_length / static_cast<double>(hg.n);
+            hg.cells.assign(hg.n * hg.n * hg.n, {});
+        }
+        for (size_t i = 0; i < N; i++) {
+            Agg &a = aggs[i];
+            a.uid = static_cast<int64_t>(i);
+            a.proper_time = pm.time;
+            double diameter = pm.random_diameter(rng);
+            bool placed = false;
+            for (size_t n_try = 0; n_try < N; n_try++) {
+                V3 newpos;
+                newpos[0] = rng.uniform() * pm.box_length;
+                newpos[1] = rng.uniform() * pm.box_length;
+                newpos[2] = rng.uniform() * pm.box_length;
+                bool free = fast ? test_free_space_fast(newpos, diameter * 0.5)
+                                 : test_free_space(newpos, diameter * 0.5);
+                if (free) {
+                    // aggregat.cpp:194-229
+                    a.pos = periodic_position(newpos, pm.box_length);
+                    a.proper_time = pm.time;
+                    a.vidx = compute_index_verlet(a);
+                    cell(a.vidx).insert(i);
+                    slabel[i] = static_cast<long>(i);
+                    set_spos(i, newpos);
+                    sr[i] = diameter * 0.5;
+                    srx[i] = sry[i] = srz[i] = 0.;
+                    svol[i] = kVolumeFactor * std::pow(sr[i], 3);
+                    ssurf[i] = kSurfaceFactor * std::pow(sr[i], 2);
+                    a.sph.assign(1, i);
+                    a.d_m = diameter;
+                    update(i);
+                    if (fast) {
+                        size_t c[3];
+                        for (int l = 0; l < 3; l++)
+                            c[l] = std::min(hg.n - 1, static_cast<size_t>(a.pos[l] / hg.h));
+                        hg.cells[(c[0] * hg.n + c[1]) * hg.n + c[2]].push_back(i);
+                    }
+                    placed = true;
+                    break;
+                }
+            }
+            if (!placed) return 6;  // TooDenseError
+        }
+        refresh();
+        if (pm.enforce_volume_fraction) {
+            double current = get_total_volume();
+            double prescribed = pm.volume_fraction * std::pow(pm.box_length, 3);
+            double correction = std::pow(prescribed / current, 1. / 3.);
+            for (size_t s = 0; s < N; s++) {
+                sr[s] = sr[s] * correction;
+                if (slabel[s] > -1) {
+                    svol[s] = kVolumeFactor * std::pow(sr[s], 3);
+                    ssurf[s] = kSurfaceFactor * std::pow(sr[s], 2);
+                }
+            }
+            for (auto &a : aggs) compute_volume_surface(a);
+        }
+        hg.cells.clear();
+        return 0;
+    }
+
+    // aggregat_list.cpp:28-36
+    double get_total_volume() const {
+        double t = 0.0;
+        for (const auto &a : aggs) t += a.volume;
+        return t;
+    }
+    double get_total_surface() const {
+        double t = 0.0;
+        for (const auto &a : aggs) t += a.surface;
+        return t;
+    }
+    // aggregat_list.cpp:100-108
+    void refresh() {
+        max_time_step = aggs[0].time_step;
+        for (const auto &a : aggs) max_time_step = smax(a.time_step, max_time_step);
+        avg_npp = static_cast<double>(sx.size()) / static_cast<double>(aggs.size());
+    }
+    // aggregat_list.cpp:109-141
+    void sort_time_steps(double factor) {
+        std::vector<double> tp_t(aggs.size());
+        for (size_t i = 0; i < aggs.size(); i++) tp_t[i] = factor / aggs[i].time_step;
+        std::vector<size_t> idx(tp_t.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&tp_t](size_t i1, size_t i2) { return tp_t[i1] < tp_t[i2]; });
+        index_sorted = std::move(idx);
+        cumulative.resize(aggs.size());
+        cumulative[0] = tp_t[index_sorted[0]];
+        for (size_t i = 1; i < aggs.size(); i++) cumulative[i] = cumulative[i - 1] + tp_t[index_sorted[i]];
+    }
+    // aggregat_list.cpp:59-66
+    size_t pick_random() {
+        double val = rng.uniform() * cumulative[aggs.size() - 1];
+        long n = std::lower_bound(cumulative.begin(), cumulative.end(), val) - cumulative.begin();
+        return index_sorted[static_cast<size_t>(n)];
+    }
+
+    // ------------------------------------------------------------ contact search
+    // aggregat_distance.cpp:24-44 + sphere_contact.cpp:126-139
+    Contact agg_distance(size_t l1, size_t l2, const V3 &dir, double dist) {
+        Contact best;
+        const Agg &a = aggs[l1];
+        const Agg &b = aggs[l2];
+        for (size_t i = 0; i < a.sph.size(); i++) {
+            size_t si = a.sph[i];
+            double hd = kInf;
+            int64_t hj = -1;
+            for (size_t j = 0; j < b.sph.size(); j++) {
+                size_t sj = b.sph[j];
+                double d = swept_sphere(spos(si), sr[si], spos(sj), sr[sj], dir, dist, pm.box_length);
+                if (d < hd) {
+                    hd = d;
+                    hj = static_cast<int64_t>(sj);
+                }
+            }
+            if (hd < best.distance) {
+                best.distance = hd;
+                best.os = hj;
+                best.ms = static_cast<int64_t>(si);
+                best.magg = a.uid;
+                best.oagg = b.uid;
+            }
+        }
+        q_pairs += static_cast<int64_t>(a.sph.size() * b.sph.size());
+        return best;
+    }
+    // aggregat_list.cpp:447-532
+    Contact next_contact(size_t source, const V3 &direction, double distance) {
+        // get_neighborhood :485-508
+        double mindist = aggs[source].rmax + maxradius;
+        V3 vec = distance * direction;
+        std::vector<size_t> nb = verlet_query(aggs[source].pos, vec, mindist);
+        bool found = false;
+        for (size_t i = 0; i < nb.size(); i++) {
+            if (nb[i] == source) {
+                nb.erase(nb.begin() + long(i));
+                found = true;
+                break;
+            }
+        }
+        if (!found) throw std::runtime_error("VerletError");
+        q_query = static_cast<int64_t>(nb.size());
+        // filter_neighborhood :510-532
+        std::multimap<double, size_t> sorted;
+        for (size_t other : nb) {
+            double d = swept_sphere(aggs[source].pos, aggs[source].rmax, aggs[other].pos, aggs[other].rmax,
+                                    direction, distance, pm.box_length);
+            if (d < distance) sorted.insert(std::pair<double, size_t>(d, other));
+        }
+        q_filtered = static_cast<int64_t>(sorted.size());
+        q_solved = 0;
+        q_pairs = 0;
+        Contact closest;
+        for (auto &suspect : sorted) {
+            auto [sd, id] = suspect;
+            if (closest.distance <= 0.) break;
+            if (closest.distance < sd) break;
+            Contact pc = agg_distance(source, id, direction, distance);
+            q_solved++;
+            if (pc.distance < closest.distance) closest = pc;
+        }
+        return closest;
+    }
+    // aggregat.cpp:148-161
+    void translate(size_t label, const V3 &v) {
+        Agg &a = aggs[label];
+        set_position(label, a.pos + v);
+        V3 refpos = a.pos - a.rcm;
+        set_spos(a.sph[0], refpos);
+        for (size_t s : a.sph) set_spos(s, refpos + srel(s));
+    }
+
+    // ------------------------------------------------------------ merge
+    size_t label_of_uid(int64_t uid) const {
+        for (size_t l = 0; l < aggs.size(); l++)
+            if (aggs[l].uid == uid) return l;
+        throw std::runtime_error("uid not found");
+    }
+    // aggregat_list.cpp:367-410
+    bool merge(const Contact &ci) {
+        if (ci.ms < 0 || ci.os < 0) return false;
+        size_t ms = static_cast<size_t>(ci.ms), os = static_cast<size_t>(ci.os);
+        if (!contact(spos(ms), sr[ms], spos(os), sr[os], pm.box_length)) return false;
+        size_t keep = static_cast<size_t>(std::min(slabel[ms], slabel[os]));
+        size_t rem = static_cast<size_t>(std::max(slabel[ms], slabel[os]));
+        double newtime = aggs[keep].proper_time + aggs[rem].proper_time - pm.time;
+        // Aggregate::merge aggregat.cpp:486-544
+        {
+            Agg &k = aggs[keep];
+            size_t mysphere, othersphere;
+            if (k.uid == ci.magg) {
+                mysphere = ms;
+                othersphere = os;
+            } else if (k.uid == ci.oagg) {
+                mysphere = os;
+                othersphere = ms;
+            } else {
+                throw std::runtime_error("MergeError");
+            }
+            V3 refpos = spos(k.sph[0]);
+            V3 ref_root_to_contact = srel(mysphere);
+            V3 dc = spos(othersphere) - spos(mysphere);
+            V3 diffcontact{periodic_distance(dc[0], pm.box_length), periodic_distance(dc[1], pm.box_length),
+                           periodic_distance(dc[2], pm.box_length)};
+            V3 other_root_to_contact = srel(othersphere);
+            V3 diffpos = ref_root_to_contact + diffcontact - other_root_to_contact;
+            for (size_t s : aggs[rem].sph) {
+                slabel[s] = static_cast<long>(keep);
+                srx[s] += diffpos[0];
+                sry[s] += diffpos[1];
+                srz[s] += diffpos[2];
+                V3 newpos = srel(s);
+                newpos[0] += refpos[0];
+                newpos[1] += refpos[1];
+                newpos[2] += refpos[2];
+                set_spos(s, newpos);
+            }
+            k.sph.insert(k.sph.end(), aggs[rem].sph.begin(), aggs[rem].sph.end());
+            update(keep);
+        }
+        // AggregatList::remove aggregat_list_storage.cpp:37-44 (~Aggregate unset_verlet first)
+        cell(aggs[rem].vidx).erase(rem);
+        aggs.erase(aggs.begin() + long(rem));
+        for (size_t i = rem; i < aggs.size(); i++) {
+            auto &c = cell(aggs[i].vidx);
+            c.erase(i + 1);
+            for (size_t s : aggs[i].sph) slabel[s]--;
+            c.insert(i);
+        }
+        aggs[keep].proper_time = newtime;
+        return true;
+    }
+
+    // physical_model.cpp:288-337 (STOPCODE / cpu limits are host-environment checks, not modelled)
+    bool finished() const {
+        size_t n = aggs.size();
+        if (n < 1) return true;
+        if (n <= pm.number_of_aggregates_limit) return true;
+        if (pm.n_iter_without_event_limit > 0 &&
+            pm.n_iter_without_event >= static_cast<size_t>(pm.n_iter_without_event_limit))
+            return true;
+        if (pm.physical_time_limit > 0 && pm.time >= pm.physical_time_limit) return true;
+        if (pm.mean_monomere_per_aggregate_limit > 0 && avg_npp >= pm.mean_monomere_per_aggregate_limit)
+            return true;
+        return false;
+    }
+
+    // calcul.cpp:66-281 (one iteration, no surface reactions / nucleation / potentials)
+    int step() {
+        if (event && pm.with_domain_duplication && aggs.size() <= pm.n_monomeres / 8) return 1;
+        double max = max_time_step;
+        if (event) sort_time_steps(max);
+        size_t num = pick_random();
+        double deltatemps = max / cumulative[aggs.size() - 1];
+        double full = aggs[num].lpm;
+        int64_t num_uid = aggs[num].uid;
+        V3 dir = rng.direction();
+        Contact nc;
+        bool contact_flag = false;
+        double move = full;
+        if (pm.with_collisions) {
+            nc = next_contact(num, dir, full);
+            contact_flag = nc.distance <= full;
+            if (contact_flag) move = nc.distance;
+        }
+        translate(num, dir * move);
+        deltatemps = deltatemps * (move / full + static_cast<double>(1 - 1));
+        aggs[num].proper_time += deltatemps;
+        pm.time = pm.time + deltatemps;
+        bool merged = false;
+        if (contact_flag) merged = merge(nc);
+        event = merged;
+        if (event) {
+            pm.n_iter_without_event = 0;
+            total_events++;
+        } else {
+            pm.n_iter_without_event++;
+        }
+        if (event) refresh();
+        if (event) pm.update(aggs.size(), sx.size(), get_total_volume(), get_total_surface());
+        if (trace_steps) {
+            ora_step_rec r{};
+            r.step = step_count;
+            r.agg = num_uid;
+            r.label = static_cast<int64_t>(num);
+            r.dir[0] = dir[0]; r.dir[1] = dir[1]; r.dir[2] = dir[2];
+            r.lpm = full;
+            r.dist = nc.distance;
+            r.moving_sphere = nc.ms;
+            r.other_sphere = nc.os;
+            r.other_agg = nc.oagg;
+            r.contact = contact_flag;
+            r.merged = merged;
+            r.dt = deltatemps;
+            r.time = pm.time;
+            steps.push_back(r);
+        }
+        if (merged && trace_merges) {
+            size_t keep = static_cast<size_t>(std::min(nc.magg, nc.oagg));
+            // keeper is the one with the smaller label, i.e. the smaller stable id
+            size_t kl = label_of_uid(static_cast<int64_t>(keep));
+            const Agg &a = aggs[kl];
+            ora_merge_rec m{};
+            m.step = step_count;
+            m.keeper = a.uid;
+            m.removed = std::max(nc.magg, nc.oagg);
+            m.n = static_cast<int64_t>(a.sph.size());
+            m.volume = a.volume; m.surface = a.surface; m.rg = a.rg; m.rmax = a.rmax; m.dp = a.dp;
+            m.f_agg = a.f_agg; m.lpm = a.lpm; m.time_step = a.time_step; m.d_m = a.d_m;
+            m.proper_time = a.proper_time; m.overlapping = a.overlapping; m.coordination = a.coordination;
+            for (int l = 0; l < 3; l++) { m.com[l] = a.pos[l]; m.rcm[l] = a.rcm[l]; }
+            merges.push_back(m);
+        }
+        step_count++;
+        return 0;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void *ora_create(const char *ini_text, char *err, int errlen) {
+    try {
+        auto *s = new Sim();
+        s->pm.load(parse_ini(ini_text));
+        if (!s->pm.unsupported.empty()) {
+            std::string m = "unsupported option: " + s->pm.unsupported;
+            if (err && errlen > 0) std::snprintf(err, static_cast<size_t>(errlen), "%s", m.c_str());
+            delete s;
+            return nullptr;
+        }
+        return s;
+    } catch (const std::exception &e) {
+        if (err && errlen > 0) std::snprintf(err, static_cast<size_t>(errlen), "%s", e.what());
+        return nullptr;
+    }
+}
+void ora_destroy(void *sim) { delete static_cast<Sim *>(sim); }
+int ora_init(void *sim, int fast) {
+    try {
+        return static_cast<Sim *>(sim)->init(fast != 0);
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+int64_t ora_run(void *sim, int64_t max_steps) {
+    auto *s = static_cast<Sim *>(sim);
+    int64_t done = 0;
+    try {
+        while (done < max_steps) {
+            if (s->finished()) {
+                s->finished_flag = true;
+                break;
+            }
+            int rc = s->step();
+            if (rc != 0) return -rc;
+            done++;
+        }
+        if (s->finished()) s->finished_flag = true;
+    } catch (const std::exception &) {
+        return -1;
+    }
+    return done;
+}
+int ora_finished(void *sim) { return static_cast<Sim *>(sim)->finished() ? 1 : 0; }
+void ora_set_trace(void *sim, int steps, int merges) {
+    auto *s = static_cast<Sim *>(sim);
+    s->trace_steps = steps != 0;
+    s->trace_merges = merges != 0;
+}
+int64_t ora_n_steps(void *sim) { return static_cast<int64_t>(static_cast<Sim *>(sim)->steps.size()); }
+int64_t ora_get_steps(void *sim, ora_step_rec *out, int64_t max) {
+    auto *s = static_cast<Sim *>(sim);
+    int64_t n = std::min<int64_t>(max, static_cast<int64_t>(s->steps.size()));
+    std::memcpy(out, s->steps.data(), static_cast<size_t>(n) * sizeof(ora_step_rec));
+    return n;
+}
+int64_t ora_n_merges(void *sim) { return static_cast<int64_t>(static_cast<Sim *>(sim)->merges.size()); }
+int64_t ora_get_merges(void *sim, ora_merge_rec *out, int64_t max) {
+    auto *s = static_cast<Sim *>(sim);
+    int64_t n = std::min<int64_t>(max, static_cast<int64_t>(s->merges.size()));
+    std::memcpy(out, s->merges.data(), static_cast<size_t>(n) * sizeof(ora_merge_rec));
+    return n;
+}
+double ora_get_scalar(void *sim, const char *name) {
+    auto *s = static_cast<Sim *>(sim);
+    std::string n(name);
+    if (n == "box_length") return s->pm.box_length;
+    if (n == "viscosity") return s->pm.viscosity;
+    if (n == "lambda") return s->pm.gaz_mean_free_path;
+    if (n == "rpeqmass") return s->pm.mean_massic_radius;
+    if (n == "gamma") return s->pm.friction_exponnant;
+    if (n == "u_sg") return s->pm.u_sg;
+    if (n == "time") return s->pm.time;
+    if (n == "maxradius") return s->maxradius;
+    if (n == "n_agg") return static_cast<double>(s->aggs.size());
+    if (n == "n_sph") return static_cast<double>(s->sx.size());
+    if (n == "avg_npp") return s->avg_npp;
+    if (n == "max_time_step") return s->max_time_step;
+    if (n == "volume_fraction") return s->pm.volume_fraction;
+    if (n == "total_volume") return s->get_total_volume();
+    if (n == "steps") return static_cast<double>(s->step_count);
+    return std::nan("");
+}
+int64_t ora_n_spheres(void *sim) { return static_cast<int64_t>(static_cast<Sim *>(sim)->sx.size()); }
+int64_t ora_n_aggs(void *sim) { return static_cast<int64_t>(static_cast<Sim *>(sim)->aggs.size()); }
+void ora_export_spheres(void *sim, double *x, double *y, double *z, double *r, double *vol, double *surf,
+                        double *rx, double *ry, double *rz, int64_t *label) {
+    auto *s = static_cast<Sim *>(sim);
+    size_t n = s->sx.size();
+    for (size_t i = 0; i < n; i++) {
+        x[i] = s->sx[i]; y[i] = s->sy[i]; z[i] = s->sz[i]; r[i] = s->sr[i];
+        vol[i] = s->svol[i]; surf[i] = s->ssurf[i];
+        rx[i] = s->srx[i]; ry[i] = s->sry[i]; rz[i] = s->srz[i];
+        label[i] = s->slabel[i];
+    }
+}
+void ora_export_aggs(void *sim, double *pos, double *rcm, double *rmax, double *volume, double *surface,
+                     double *rg, double *lpm, double *time_step, double *proper_time, int64_t *n,
+                     int64_t *uid, int64_t *offset, int64_t *members) {
+    auto *s = static_cast<Sim *>(sim);
+    int64_t off = 0;
+    for (size_t l = 0; l < s->aggs.size(); l++) {
+        const Agg &a = s->aggs[l];
+        for (int c = 0; c < 3; c++) {
+            pos[3 * l + c] = a.pos[c];
+            rcm[3 * l + c] = a.rcm[c];
+        }
+        rmax[l] = a.rmax; volume[l] = a.volume; surface[l] = a.surface; rg[l] = a.rg;
+        lpm[l] = a.lpm; time_step[l] = a.time_step; proper_time[l] = a.proper_time;
+        n[l] = static_cast<int64_t>(a.sph.size());
+        uid[l] = a.uid;
+        offset[l] = off;
+        for (size_t q : a.sph) members[off++] = static_cast<int64_t>(q);
+    }
+}
+double ora_next_contact(void *sim, int64_t label, const double dir[3], double lpm, int64_t *moving_sphere,
+                        int64_t *other_sphere, int64_t *other_label, int64_t *n_query, int64_t *n_filtered,
+                        int64_t *n_solved, int64_t *n_pairs) {
+    auto *s = static_cast<Sim *>(sim);
+    Contact c = s->next_contact(static_cast<size_t>(label), {dir[0], dir[1], dir[2]}, lpm);
+    *moving_sphere = c.ms;
+    *other_sphere = c.os;
+    *other_label = c.os >= 0 ? s->slabel[static_cast<size_t>(c.os)] : -1;
+    if (n_query) *n_query = s->q_query;
+    if (n_filtered) *n_filtered = s->q_filtered;
+    if (n_solved) *n_solved = s->q_solved;
+    if (n_pairs) *n_pairs = s->q_pairs;
+    return c.distance;
+}
+double ora_swept_sphere(const double p1[3], double r1, const double p2[3], double r2, const double dir[3],
+                        double dist, double box) {
+    return swept_sphere({p1[0], p1[1], p1[2]}, r1, {p2[0], p2[1], p2[2]}, r2, {dir[0], dir[1], dir[2]}, dist,
+                        box);
+}
+int ora_contact(const double p1[3], double r1, const double p2[3], double r2, double box) {
+    return contact({p1[0], p1[1], p1[2]}, r1, {p2[0], p2[1], p2[2]}, r2, box) ? 1 : 0;
+}
+void *ora_rng_create(int seed) { return new Rng(static_cast<unsigned>(seed)); }
+void ora_rng_destroy(void *rng) { delete static_cast<Rng *>(rng); }
+double ora_rng_uniform(void *rng) { return static_cast<Rng *>(rng)->uniform(); }
+void ora_rng_direction(void *rng, double dir[3]) {
+    V3 d = static_cast<Rng *>(rng)->direction();
+    dir[0] = d[0]; dir[1] = d[1]; dir[2] = d[2];
+}
+double ora_inverf(double p) { return inverf(p); }
+double ora_inverfc(double p) { return inverfc(p); }
+int64_t ora_verlet_query(int64_t n_div, double width, const int64_t *cells, int64_t n_labels, const double src[3],
+                         const double vec[3], double dist, int64_t *out, int64_t max) {
+    Sim s;
+    s.n_div = static_cast<size_t>(n_div);
+    s.width = width;
+    s.grid.assign(s.n_div * s.n_div * s.n_div, {});
+    for (int64_t l = 0; l < n_labels; l++) {
+        std::array<size_t, 3> c{static_cast<size_t>(cells[3 * l]), static_cast<size_t>(cells[3 * l + 1]),
+                                static_cast<size_t>(cells[3 * l + 2])};
+        s.cell(c).insert(static_cast<size_t>(l));
+    }
+    auto v = s.verlet_query({src[0], src[1], src[2]}, {vec[0], vec[1], vec[2]}, dist);
+    int64_t n = std::min<int64_t>(max, static_cast<int64_t>(v.size()));
+    for (int64_t i = 0; i < n; i++) out[i] = static_cast<int64_t>(v[static_cast<size_t>(i)]);
+    return static_cast<int64_t>(v.size());
+}
+
+}  // extern "C"
