@@ -1,0 +1,145 @@
+/*
+ * mcac_gpu.h — C-ABI of the MI355X aggregate-move / collision-search path.
+ *
+ * One `mcac_box` holds one periodic aggregation box resident in HBM:
+ * structure-of-arrays sphere and aggregate fields, per-aggregate member
+ * lists, and the scratch of the contact search.  Aggregates are named by a
+ * stable id (their creation index); the reference's label of an aggregate is
+ * its rank among the live ids, so ascending id order is label order and no
+ * relabelling is ever needed.  Spheres are named by their index, which never
+ * changes on this path.
+ *
+ * The entry points replace these reference operators (MCAC, C++):
+ *   mcac_next_contact  AggregatList::distance_to_next_contact
+ *                      (include/aggregats/aggregat_list.hpp:69-71,
+ *                       src/aggregats/aggregat_list.cpp:447-532, with
+ *                       src/verlet/verlet.cpp:52-98 and
+ *                       src/spheres/sphere_contact.cpp:47-139)
+ *   mcac_translate     Aggregate::translate (include/aggregats/aggregat.hpp:108,
+ *                       src/aggregats/aggregat.cpp:148-161)
+ *   mcac_move          the pair of calls at src/calcul.cpp:127-144 fused:
+ *                       contact search, then translate by min(contact, lpm)
+ *   mcac_merge         AggregatList::merge (aggregat_list.hpp:63,
+ *                       aggregat_list.cpp:367-410) with Aggregate::merge and
+ *                       Aggregate::update (aggregat.cpp:284-288, 486-544,
+ *                       719-764, 321-483) minus the host physics
+ *                       (friction, mobility, lpm, time step), which the caller
+ *                       computes from the returned geometry.
+ *   mcac_box_upload /  the SoA storage the reference keeps in
+ *   mcac_box_download  ListStorage (include/list_storage/list_storage.hpp:25-73)
+ *
+ * Every function returns an int status whose values equal the reference's
+ * ErrorCodes (include/constants.hpp:70-83) where one applies, or one of the
+ * MCAC_E_* codes below.  No C++ exception crosses this boundary.
+ */
+#ifndef MCAC_GPU_H
+#define MCAC_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 0..11 mirror the reference ErrorCodes. */
+#define MCAC_NO_ERROR 0
+#define MCAC_UNKNOWN_ERROR 1
+#define MCAC_VERLET_ERROR 3
+#define MCAC_INPUT_ERROR 4
+#define MCAC_VOL_SURF_ERROR 8
+#define MCAC_MERGE_ERROR 9
+#define MCAC_E_DEVICE 32     /* a HIP runtime call failed                     */
+#define MCAC_E_CAPACITY 33   /* a fixed-capacity buffer would overflow         */
+#define MCAC_E_ARGUMENT 34   /* invalid argument (id out of range, dead id...) */
+
+typedef struct mcac_box mcac_box;
+
+typedef struct mcac_box_params {
+    double box_length;          /* periodic box side L [m]                          */
+    int64_t n_div;              /* reference Verlet divisions per axis (candidate   */
+                                /* visit order; physical_model.cpp:162)             */
+    int64_t max_spheres;        /* sphere capacity                                  */
+    int64_t max_aggregates;     /* stable-id capacity                               */
+    int64_t candidate_capacity; /* filtered-candidate buffer per search (0: auto)   */
+    int64_t member_pool;        /* member-list pool entries (0: auto)               */
+    int32_t device;             /* HIP device ordinal                               */
+    int32_t precision;          /* 0: FP64 throughout; 1: FP32 broad phase, FP64    */
+                                /*    final solve (results identical to 0)          */
+} mcac_box_params;
+
+/* Result of a contact search (AggregateContactInfo, tools/contact_info.hpp:73-85). */
+typedef struct mcac_contact {
+    double distance;            /* +inf when no contact is found                    */
+    double move;                /* contact ? distance : lpm                         */
+    int64_t moving_sphere;      /* -1 when none                                     */
+    int64_t other_sphere;       /* -1 when none                                     */
+    int64_t moving_agg;         /* stable id of the searching aggregate             */
+    int64_t other_agg;          /* stable id, -1 when none                          */
+    int32_t contact;            /* distance <= lpm                                  */
+    int32_t status;
+    int64_t n_query;            /* aggregates in the visited Verlet cells (C_q)     */
+    int64_t n_filtered;         /* candidates passing the bounding filter (F)       */
+    int64_t n_pairs;            /* exact sphere-pair tests performed                */
+} mcac_contact;
+
+/* Geometry of the aggregate after a merge (Aggregate::update minus physics). */
+typedef struct mcac_merge_result {
+    int64_t keeper;             /* stable id kept (min label)                       */
+    int64_t removed;            /* stable id removed (max label)                    */
+    int64_t n;                  /* spheres in the merged aggregate                  */
+    int32_t merged;             /* 0 when the contact re-check failed               */
+    int32_t status;
+    double volume, surface, rg, rmax;
+    double radius_sum;          /* sum of sphere radii in list order (dp = 2 s / n) */
+    double com[3];              /* aggregate position (periodic)                    */
+    double rcm[3];              /* centre of mass relative to sphere 0              */
+    double overlapping, coordination;
+} mcac_merge_result;
+
+int mcac_box_create(const mcac_box_params *params, mcac_box **out);
+int mcac_box_destroy(mcac_box *box);
+
+/* Uploads a full state.  Spheres in index order (positions, radius, volume,
+ * surface, position relative to their aggregate's sphere 0).  Aggregates in
+ * stable-id order: position, relative CoM, rmax, volume, surface, sphere count
+ * (0 marks a dead id) and members (concatenated sphere indices, list order). */
+int mcac_box_upload(mcac_box *box, int64_t n_spheres, const double *x, const double *y, const double *z,
+                    const double *r, const double *vol, const double *surf, const double *rx,
+                    const double *ry, const double *rz, int64_t n_aggs, const double *pos3,
+                    const double *rcm3, const double *rmax, const double *volume, const double *surface,
+                    const int64_t *count, const int64_t *members);
+
+int mcac_next_contact(mcac_box *box, int64_t agg, const double dir[3], double lpm, double maxradius,
+                      mcac_contact *out);
+int mcac_translate(mcac_box *box, int64_t agg, const double vec[3]);
+int mcac_move(mcac_box *box, int64_t agg, const double dir[3], double lpm, double maxradius,
+              mcac_contact *out);
+int mcac_merge(mcac_box *box, const mcac_contact *contact, mcac_merge_result *out);
+
+int mcac_box_download_spheres(mcac_box *box, double *x, double *y, double *z, double *rx, double *ry,
+                              double *rz);
+int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rmax, double *volume,
+                           double *surface, double *rg, int64_t *count);
+/* Members of one aggregate in list order; returns the count via *n. */
+int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int64_t *n);
+
+/* Kernel timing on the box's stream (HIP events).  When enabled, every search
+ * and merge records events; mcac_box_timing returns accumulated milliseconds
+ * and launch counts: [search_ms, translate_ms, merge_ms, n_search, n_translate,
+ * n_merge, filter_ms, solve_ms]. */
+int mcac_box_set_timing(mcac_box *box, int enable);
+int mcac_box_timing(mcac_box *box, double out[8]);
+
+/* distance_to_contact(Sphere, Sphere, dir, dist) (src/spheres/sphere_contact.cpp:47-125)
+ * evaluated on the device for n independent inputs: p1, p2, dir are (n,3)
+ * row-major, r1, r2, dist are (n,), box is the period.  out[i] receives the
+ * travel to contact (0 when touching, +inf when none).  Host buffers. */
+int mcac_swept_sphere_batch(int64_t n, const double *p1, const double *r1, const double *p2, const double *r2,
+                            const double *dir, const double *dist, double box, double *out, int device);
+
+/* Library self-description for loaders and tests. */
+const char *mcac_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

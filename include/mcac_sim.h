@@ -1,0 +1,77 @@
+/*
+ * mcac_sim.h — host driver of one MCAC aggregation run on the MI355X path.
+ *
+ * The driver keeps the reference's MC loop on the host CPU (MCAC parameter
+ * file, physical model, glibc random stream, time-step weighted pick and
+ * direction draw, time bookkeeping, advancement.dat) and calls the device
+ * through include/mcac_gpu.h.  It mirrors:
+ *   mcac_sim_create / mcac_sim_init   PhysicalModel(const std::string&) and
+ *                                     AggregatList(PhysicalModel*)
+ *                                     (physical_model.cpp:32-287,
+ *                                      aggregat_list_storage.cpp:52-88)
+ *   mcac_sim_run                      mcac::calcul (src/calcul.cpp:55-290)
+ */
+#ifndef MCAC_SIM_H
+#define MCAC_SIM_H
+#include <stdint.h>
+
+#include "mcac_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mcac_sim mcac_sim;
+
+/* Per-step and per-merge trace records (same layout as the oracle's). */
+typedef struct mcac_step_rec {
+    int64_t step, agg, label;
+    double dir[3];
+    double lpm, dist;
+    int64_t moving_sphere, other_sphere, other_agg;
+    int32_t contact, merged;
+    double dt, time;
+} mcac_step_rec;
+
+typedef struct mcac_merge_rec {
+    int64_t step, keeper, removed, n;
+    double volume, surface, rg, rmax, dp, f_agg, lpm, time_step, d_m, proper_time;
+    double overlapping, coordination;
+    double com[3], rcm[3];
+} mcac_merge_rec;
+
+/* ini_text: content of an MCAC parameter file.  output_dir: directory for
+ * advancement.dat and params.ini (NULL or "" disables file output). */
+int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mcac_sim **out, char *err,
+                    int errlen);
+int mcac_sim_destroy(mcac_sim *sim);
+/* Initial placement (aggregat.cpp:162-229, host) and upload to the device. */
+int mcac_sim_init(mcac_sim *sim);
+/* Runs at most max_steps MC steps; *done receives the number run. */
+int mcac_sim_run(mcac_sim *sim, int64_t max_steps, int64_t *done);
+int mcac_sim_finished(mcac_sim *sim);
+int mcac_sim_set_trace(mcac_sim *sim, int steps, int merges);
+int64_t mcac_sim_n_steps(mcac_sim *sim);
+int64_t mcac_sim_get_steps(mcac_sim *sim, mcac_step_rec *out, int64_t max);
+int64_t mcac_sim_n_merges(mcac_sim *sim);
+int64_t mcac_sim_get_merges(mcac_sim *sim, mcac_merge_rec *out, int64_t max);
+/* Named scalars: box_length, time, maxradius, n_agg, n_sph, avg_npp,
+ * max_time_step, steps, merges, events, total_volume, query, filtered, pairs,
+ * t_host_pick_ms, t_device_ms, init_ms. NaN for unknown names. */
+double mcac_sim_scalar(mcac_sim *sim, const char *name);
+/* Host view of the live aggregates in label order (stable ids). */
+int64_t mcac_sim_n_aggs(mcac_sim *sim);
+int mcac_sim_export_aggs(mcac_sim *sim, int64_t *uid, double *lpm, double *time_step, double *proper_time,
+                         double *volume, double *rg, double *rmax, int64_t *n);
+mcac_box *mcac_sim_box(mcac_sim *sim);
+
+/* Derived scalars of PhysicalModel for a parameter file, without running
+ * (physical_model.cpp:228-269, 536-546): out = [box_length, lambda, viscosity,
+ * rpeqmass, gamma, u_sg, box_volume, aggregate_concentration]. Options this
+ * path does not run are accepted here. */
+int mcac_physics_scalars(const char *ini_text, double out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

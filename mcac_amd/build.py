@@ -1,0 +1,58 @@
+"""Builds the MI355X libraries in-tree.
+
+  mcac_amd/lib/libmcac_gpu.so   HIP kernels + C-ABI (include/mcac_gpu.h), gfx950
+  mcac_amd/lib/libmcac_sim.so   host driver (include/mcac_sim.h), g++, links libmcac_gpu
+  mcac_amd/bin/mcac_run         command-line driver: mcac_run <params.ini> [output_dir]
+
+FP policy on both sides: no contraction, no fast-math (IEEE FP64 as written).
+"""
+import os
+import pathlib
+import shutil
+import subprocess
+
+PKG = pathlib.Path(__file__).resolve().parent
+ROOT = PKG.parent
+LIB = PKG / "lib"
+BIN = PKG / "bin"
+CSRC = PKG / "csrc"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
+ARCH = os.environ.get("MCAC_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
+]
+CXX = os.environ.get("CXX", "g++")
+CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-Wall", "-Wextra"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build(verbose: bool = False) -> None:
+    LIB.mkdir(exist_ok=True)
+    BIN.mkdir(exist_ok=True)
+    gpu = LIB / "libmcac_gpu.so"
+    tmp = LIB / "libmcac_gpu.so.tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), str(CSRC / "mcac_gpu.hip")])
+    os.replace(tmp, gpu)
+    host_src = [str(CSRC / "host" / "physics.cpp"), str(CSRC / "host" / "sim.cpp")]
+    sim = LIB / "libmcac_sim.so"
+    tmp = LIB / "libmcac_sim.so.tmp"
+    _run([CXX, *CXX_FLAGS, "-shared", "-o", str(tmp), *host_src, f"-L{LIB}", "-lmcac_gpu",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, sim)
+    exe = BIN / "mcac_run"
+    _run([CXX, *CXX_FLAGS, "-o", str(exe), str(CSRC / "host" / "main.cpp"), f"-L{LIB}", "-lmcac_sim",
+          "-lmcac_gpu", "-Wl,-rpath,$ORIGIN/../lib"])
+    if verbose:
+        print(f"built {gpu}, {sim}, {exe}")
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,704 @@
+// Host driver of one aggregation run: the reference MC loop (src/calcul.cpp)
+// on the host CPU, the collision path on the device through mcac_gpu.h.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/mcac_sim.h"
+#include "physics.hpp"
+
+namespace mcac {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+inline double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+inline double periodic_position(double p, double L) {
+    while (p < 0) p += L;
+    while (p >= L) p -= L;
+    return p;
+}
+inline double periodic_distance(double d, double L) {
+    double half = 0.5 * L;
+    while (d < -half) d += L;
+    while (d >= half) d -= L;
+    return d;
+}
+inline double distance_2(const Vec3 &a, const Vec3 &b, double L) {
+    double dx = periodic_distance(a[0] - b[0], L);
+    double dy = periodic_distance(a[1] - b[1], L);
+    double dz = periodic_distance(a[2] - b[2], L);
+    return dx * dx + dy * dy + dz * dz;
+}
+inline bool contact(const Vec3 &a, double ra, const Vec3 &b, double rb, double L) {
+    double s = ra + rb;
+    return distance_2(a, b, L) - s * s <= kContactEpsilon;
+}
+
+// Time-step weighted pick table (aggregat_list.cpp:54-66, 109-141).
+// Live aggregates are kept ordered by time step, descending, which is the
+// reference's ascending order of tp_t = max/time_step; the cumulative sums
+// are the same sequential sums of the same values, so they are bit-identical
+// to the reference's.  Only inside a run of equal tp_t values does the
+// libstdc++ std::sort permutation decide which aggregate a position holds:
+// a pick landing in such a run is answered from that permutation, rebuilt
+// from the label-ordered keys exactly as the reference builds it.
+class PickTable {
+  public:
+    void build(const std::vector<int> &ids, const std::vector<double> &ts) {
+        id_ = ids;
+        std::sort(id_.begin(), id_.end(), [&ts](int a, int b) {
+            return ts[static_cast<size_t>(a)] > ts[static_cast<size_t>(b)] ||
+                   (ts[static_cast<size_t>(a)] == ts[static_cast<size_t>(b)] && a < b);
+        });
+        ts_.resize(id_.size());
+        for (size_t i = 0; i < id_.size(); ++i) ts_[i] = ts[static_cast<size_t>(id_[i])];
+        factor_ = ts_.empty() ? 0.0 : ts_.front();
+        cum_.resize(id_.size());
+        recompute(0);
+        ref_valid_ = false;
+    }
+    double factor() const { return factor_; }
+    double total() const { return cum_.back(); }
+    size_t size() const { return id_.size(); }
+
+    // keeper: old -> new time step; removed: leaves the table.
+    void replace(int keep, double keep_old, double keep_new, int rem, double rem_ts) {
+        size_t a = erase(keep_old, keep);
+        size_t b = erase(rem_ts, rem);
+        size_t c = insert(keep_new, keep);
+        size_t p = std::min(a, std::min(b, c));
+        cum_.resize(id_.size());
+        double f = ts_.front();
+        if (f != factor_) {
+            factor_ = f;
+            p = 0;
+        }
+        recompute(p);
+        ref_valid_ = false;
+    }
+
+    // pick_random (aggregat_list.cpp:59-66) for the draw u; live holds the ids in label order.
+    int pick(double u, const std::vector<int> &live, const std::vector<double> &ts, bool *tie_out) {
+        double val = u * cum_.back();
+        size_t n = static_cast<size_t>(std::lower_bound(cum_.begin(), cum_.end(), val) - cum_.begin());
+        double tp = factor_ / ts_[n];
+        bool tie = (n > 0 && factor_ / ts_[n - 1] == tp) || (n + 1 < ts_.size() && factor_ / ts_[n + 1] == tp);
+        if (tie_out) *tie_out = tie;
+        if (!tie) return id_[n];
+        if (!ref_valid_) {
+            std::vector<double> tp_t(live.size());
+            for (size_t i = 0; i < live.size(); ++i) tp_t[i] = factor_ / ts[static_cast<size_t>(live[i])];
+            ref_idx_.resize(live.size());
+            std::iota(ref_idx_.begin(), ref_idx_.end(), 0);
+            std::sort(ref_idx_.begin(), ref_idx_.end(), [&tp_t](size_t i1, size_t i2) { return tp_t[i1] < tp_t[i2]; });
+            ref_valid_ = true;
+        }
+        return live[ref_idx_[n]];
+    }
+
+  private:
+    std::vector<double> ts_;
+    std::vector<int> id_;
+    std::vector<double> cum_;
+    double factor_ = 0;
+    std::vector<size_t> ref_idx_;
+    bool ref_valid_ = false;
+
+    size_t locate(double t, int id) const {
+        // order: ts descending, then id ascending
+        size_t lo = 0, hi = ts_.size();
+        while (lo < hi) {
+            size_t mid = (lo + hi) / 2;
+            bool before = ts_[mid] > t || (ts_[mid] == t && id_[mid] < id);
+            if (before) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    }
+    size_t erase(double t, int id) {
+        size_t p = locate(t, id);
+        if (p >= id_.size() || id_[p] != id) throw std::logic_error("pick table out of sync");
+        ts_.erase(ts_.begin() + static_cast<long>(p));
+        id_.erase(id_.begin() + static_cast<long>(p));
+        return p;
+    }
+    size_t insert(double t, int id) {
+        size_t p = locate(t, id);
+        ts_.insert(ts_.begin() + static_cast<long>(p), t);
+        id_.insert(id_.begin() + static_cast<long>(p), id);
+        return p;
+    }
+    void recompute(size_t from) {
+        const size_t n = ts_.size();
+        if (n == 0) return;
+        if (from == 0) {
+            cum_[0] = factor_ / ts_[0];
+            from = 1;
+        }
+        double acc = cum_[from - 1];
+        for (size_t i = from; i < n; ++i) {
+            acc = acc + factor_ / ts_[i];
+            cum_[i] = acc;
+        }
+    }
+};
+
+}  // namespace
+
+struct Simulation {
+    PhysicalModel pm;
+    std::string ini_text;
+    std::string out_dir;
+    RandomStream rng{0};
+    int device = 0;
+    mcac_box *box = nullptr;
+    // spheres (index order) — host copy of the initial state only
+    size_t N = 0;
+    // aggregates by stable id
+    std::vector<double> ts, lpm, f_agg, d_m, dp, rg, rmax, vol, surf, ptime, dg_over_dp, overlapping, coord;
+    std::vector<Vec3> com, rcm;
+    std::vector<int> nsph;
+    std::vector<int> live;  // live ids, ascending (label order)
+    double maxradius = 0, max_time_step = 0, avg_npp = 1;
+    PickTable pick;
+    bool event = true;
+    bool pm_stale = false;
+    size_t total_events = 0;
+    int64_t step_count = 0, merges = 0;
+    bool trace_steps = false, trace_merges = false;
+    std::vector<mcac_step_rec> steps;
+    std::vector<mcac_merge_rec> merge_recs;
+    double q_query = 0, q_filtered = 0, q_pairs = 0, n_ties = 0;
+    double t_pick = 0, t_device = 0, t_init = 0;
+    bool initialized = false;
+    bool final_written = false;
+
+    ~Simulation() {
+        if (box) mcac_box_destroy(box);
+    }
+
+    // Monomer update() (aggregat.cpp:284-288) for a single sphere of radius r.
+    void monomer_update(size_t i, double r, double v, double s) {
+        vol[i] = 0.0 + v;
+        surf[i] = 0.0 + s;
+        rcm[i] = {0., 0., 0.};
+        rmax[i] = smax(0.0, r + 0.0);
+        double brg = 0. + v * (r * r);
+        double arg = 0. + v * (0.0 * 0.0);
+        rg[i] = std::sqrt(std::abs((arg + 3. / 5. * brg) / vol[i]));
+        AggregatePhysics ph = aggregate_physics(pm, 1, vol[i], 0. + r, rg[i]);
+        dp[i] = ph.dp;
+        f_agg[i] = ph.f_agg;
+        d_m[i] = ph.d_m;
+        ts[i] = ph.time_step;
+        lpm[i] = ph.lpm;
+        dg_over_dp[i] = ph.dg_over_dp;
+        overlapping[i] = 0.0;
+        coord[i] = 0.0;
+        if (rmax[i] > maxradius) maxradius = rmax[i];
+    }
+
+    // AggregatList constructor (aggregat_list_storage.cpp:52-88): random
+    // non-overlapping monomers (aggregat.cpp:162-229), then the volume
+    // fraction rescale.  test_free_space (aggregat_list.cpp:534-548) asks
+    // whether any placed monomer in the reference Verlet window touches the
+    // new sphere; a private hash grid finds the candidates and the window
+    // test keeps exactly the reference's set.
+    int place(std::vector<double> &sx, std::vector<double> &sy, std::vector<double> &sz, std::vector<double> &sr,
+              std::vector<double> &svol, std::vector<double> &ssurf, std::vector<int> &vcell) {
+        const double L = pm.box_length;
+        const int nd = static_cast<int>(pm.n_verlet_divisions);
+        const double vstep = static_cast<double>(nd) / L;
+        const double target = 2e-9 * pm.mean_diameter;
+        const int hn = static_cast<int>(std::min<double>(128, std::max<double>(3, std::floor(L / target))));
+        const double h = L / hn;
+        std::vector<int> head(static_cast<size_t>(hn) * hn * hn, -1), next(N, -1);
+        for (size_t i = 0; i < N; ++i) {
+            ptime[i] = pm.time;
+            double diameter = pm.random_diameter(rng);
+            const double radius = diameter * 0.5;
+            bool placed = false;
+            for (size_t attempt = 0; attempt < N && !placed; ++attempt) {
+                Vec3 p;
+                p[0] = rng.uniform() * L;
+                p[1] = rng.uniform() * L;
+                p[2] = rng.uniform() * L;
+                // reference window of get_neighborhood(pos, radius + maxradius) (verlet.cpp:52-78)
+                const double reach = radius + maxradius;
+                int b1[3], len[3];
+                for (int l = 0; l < 3; ++l) {
+                    double hi = p[l] + reach + 0.0;
+                    double lo = p[l] - reach + 0.0;
+                    int a = static_cast<int>(std::floor(static_cast<double>(nd) * lo / L));
+                    int c = static_cast<int>(std::floor(static_cast<double>(nd) * hi / L) + 1);
+                    if (c - a >= nd) {
+                        a = 0;
+                        c = nd - 1;
+                    }
+                    b1[l] = a;
+                    len[l] = c - a + 1;
+                }
+                int span = static_cast<int>(std::ceil(reach / h)) + 1;
+                int lim = std::min(span, (hn - 1) / 2);
+                int extra = span > (hn - 1) / 2 ? hn - 1 - 2 * lim : 0;
+                int c0[3];
+                for (int l = 0; l < 3; ++l) c0[l] = static_cast<int>(std::floor(periodic_position(p[l], L) / h));
+                bool free = true;
+                for (int di = -lim; di <= lim + extra && free; ++di)
+                    for (int dj = -lim; dj <= lim + extra && free; ++dj)
+                        for (int dk = -lim; dk <= lim + extra && free; ++dk) {
+                            int ci = ((c0[0] + di) % hn + hn) % hn;
+                            int cj = ((c0[1] + dj) % hn + hn) % hn;
+                            int ck = ((c0[2] + dk) % hn + hn) % hn;
+                            for (int j = head[(static_cast<size_t>(ci) * hn + cj) * hn + ck]; j >= 0 && free;
+                                 j = next[static_cast<size_t>(j)]) {
+                                const size_t J = static_cast<size_t>(j);
+                                int vc = vcell[J];
+                                int v[3] = {vc / (nd * nd), (vc / nd) % nd, vc % nd};
+                                bool inside = true;
+                                for (int l = 0; l < 3; ++l)
+                                    inside = inside && (((v[l] - b1[l]) % nd + nd) % nd) < len[l];
+                                if (!inside) continue;
+                                if (!contact(p, radius, com[J], rmax[J], L)) continue;
+                                if (contact(p, radius, {sx[J], sy[J], sz[J]}, sr[J], L)) free = false;
+                            }
+                        }
+                if (!free) continue;
+                // Aggregate::init (aggregat.cpp:194-229)
+                com[i] = {periodic_position(p[0], L), periodic_position(p[1], L), periodic_position(p[2], L)};
+                int vi[3];
+                for (int l = 0; l < 3; ++l) {
+                    int v = static_cast<int>(std::floor(com[i][static_cast<size_t>(l)] * vstep));
+                    vi[l] = v < 0 ? 0 : (v >= nd ? nd - 1 : v);
+                }
+                vcell[i] = (vi[0] * nd + vi[1]) * nd + vi[2];
+                sx[i] = p[0];
+                sy[i] = p[1];
+                sz[i] = p[2];
+                sr[i] = radius;
+                svol[i] = kVolumeFactor * std::pow(sr[i], 3);
+                ssurf[i] = kSurfaceFactor * std::pow(sr[i], 2);
+                nsph[i] = 1;
+                monomer_update(i, sr[i], svol[i], ssurf[i]);
+                int hc[3];
+                for (int l = 0; l < 3; ++l)
+                    hc[l] = std::min(hn - 1, static_cast<int>(com[i][static_cast<size_t>(l)] / h));
+                size_t cell = (static_cast<size_t>(hc[0]) * hn + hc[1]) * hn + hc[2];
+                next[i] = head[cell];
+                head[cell] = static_cast<int>(i);
+                placed = true;
+            }
+            if (!placed) return 6;  // TooDenseError (aggregat.cpp:192)
+        }
+        return 0;
+    }
+
+    int init() {
+        auto t0 = Clock::now();
+        N = pm.n_monomeres;
+        rng.reseed(static_cast<unsigned>(pm.random_seed));
+        for (auto *v : {&ts, &lpm, &f_agg, &d_m, &dp, &rg, &rmax, &vol, &surf, &ptime, &dg_over_dp, &overlapping, &coord})
+            v->assign(N, 0.0);
+        com.assign(N, {0, 0, 0});
+        rcm.assign(N, {0, 0, 0});
+        nsph.assign(N, 0);
+        std::vector<double> sx(N), sy(N), sz(N), sr(N), svol(N), ssurf(N), zero(N, 0.0);
+        std::vector<int> vcell(N, 0);
+        int rc = place(sx, sy, sz, sr, svol, ssurf, vcell);
+        if (rc) return rc;
+        // refresh (aggregat_list.cpp:100-108)
+        max_time_step = ts[0];
+        for (size_t i = 0; i < N; ++i) max_time_step = smax(ts[i], max_time_step);
+        avg_npp = 1.0;
+        if (pm.enforce_volume_fraction) {
+            double current = 0.0;
+            for (size_t i = 0; i < N; ++i) current += vol[i];
+            double prescribed = pm.volume_fraction * std::pow(pm.box_length, 3);
+            double correction = std::pow(prescribed / current, 1. / 3.);
+            for (size_t s = 0; s < N; ++s) {
+                sr[s] = sr[s] * correction;
+                svol[s] = kVolumeFactor * std::pow(sr[s], 3);
+                ssurf[s] = kSurfaceFactor * std::pow(sr[s], 2);
+            }
+            for (size_t i = 0; i < N; ++i) {
+                vol[i] = 0.0 + svol[i];
+                surf[i] = 0.0 + ssurf[i];
+            }
+        }
+        live.resize(N);
+        std::iota(live.begin(), live.end(), 0);
+        // device state
+        mcac_box_params bp{};
+        bp.box_length = pm.box_length;
+        bp.n_div = static_cast<int64_t>(pm.n_verlet_divisions);
+        bp.max_spheres = static_cast<int64_t>(N);
+        bp.max_aggregates = static_cast<int64_t>(N);
+        bp.device = device;
+        rc = mcac_box_create(&bp, &box);
+        if (rc) return rc;
+        std::vector<double> pos3(3 * N), rcm3(3 * N, 0.0);
+        std::vector<int64_t> cnt(N, 1), mem(N);
+        for (size_t i = 0; i < N; ++i) {
+            for (int l = 0; l < 3; ++l) pos3[3 * i + l] = com[i][static_cast<size_t>(l)];
+            mem[i] = static_cast<int64_t>(i);
+        }
+        rc = mcac_box_upload(box, static_cast<int64_t>(N), sx.data(), sy.data(), sz.data(), sr.data(), svol.data(),
+                             ssurf.data(), zero.data(), zero.data(), zero.data(), static_cast<int64_t>(N), pos3.data(),
+                             rcm3.data(), rmax.data(), vol.data(), surf.data(), cnt.data(), mem.data());
+        if (rc) return rc;
+        event = true;
+        initialized = true;
+        t_init = ms_since(t0);
+        if (!out_dir.empty()) {
+            mkdir(out_dir.c_str(), 0755);
+            std::ofstream os(out_dir + "/params.ini");
+            os << IniFile(ini_text).dump();
+        }
+        return 0;
+    }
+
+    int64_t label_of(int id) const {
+        return static_cast<int64_t>(std::lower_bound(live.begin(), live.end(), id) - live.begin());
+    }
+
+    // calcul.cpp:29-43
+    void save_advancement() {
+        if (out_dir.empty()) return;
+        refresh_pm();
+        std::ofstream f(out_dir + "/advancement.dat", std::ios_base::app);
+        f << pm.time;
+        f << " " << pm.aggregate_concentration;
+        f << " " << pm.volume_fraction;
+        f << " " << avg_npp;
+        f << " " << pm.temperature;
+        f << " " << pm.box_volume;
+        f << " " << pm.monomer_concentration;
+        f << " " << pm.u_sg;
+        f << " " << pm.flux_nucleation;
+        f << std::endl;
+    }
+    // PhysicalModel::update at the last event (calcul.cpp:272-277), evaluated
+    // when its outputs are read; the aggregate volumes have not changed since.
+    void refresh_pm() {
+        if (!pm_stale) return;
+        double tv = 0.0, tsf = 0.0;
+        for (int id : live) tv += vol[static_cast<size_t>(id)];
+        for (int id : live) tsf += surf[static_cast<size_t>(id)];
+        pm.update(live.size(), N, tv, tsf);
+        pm_stale = false;
+    }
+
+    bool finished() const {
+        size_t n = live.size();
+        if (n < 1) return true;
+        if (n <= pm.number_of_aggregates_limit) return true;
+        if (pm.n_iter_without_event_limit > 0 &&
+            pm.n_iter_without_event >= static_cast<size_t>(pm.n_iter_without_event_limit))
+            return true;
+        if (pm.physical_time_limit > 0 && pm.time >= pm.physical_time_limit) return true;
+        if (pm.mean_monomere_per_aggregate_limit > 0 && avg_npp >= pm.mean_monomere_per_aggregate_limit) return true;
+        if (!out_dir.empty() && (step_count & 4095) == 0) {
+            struct stat st;
+            if (stat((out_dir + "/STOPCODE").c_str(), &st) == 0) return true;
+        }
+        return false;
+    }
+
+    // One iteration of calcul.cpp:66-281 (collisions on, no surface reactions,
+    // nucleation, potentials or flame coupling).
+    int step() {
+        if (pm.time_to_write(total_events)) save_advancement();
+        if (event && pm.with_domain_duplication && live.size() <= pm.n_monomeres / 8) return MCAC_INPUT_ERROR;
+        auto tp0 = Clock::now();
+        if (event && step_count == 0) pick.build(live, ts);
+        const double max = max_time_step;
+        bool tie = false;
+        const int num = pick.pick(rng.uniform(), live, ts, &tie);
+        n_ties += tie ? 1 : 0;
+        const int64_t num_label = trace_steps ? label_of(num) : -1;
+        double deltatemps = max / pick.total();
+        const size_t U = static_cast<size_t>(num);
+        const double full = lpm[U];
+        const Vec3 dir = rng.direction();
+        t_pick += ms_since(tp0);
+        mcac_contact c{};
+        auto td0 = Clock::now();
+        int rc = mcac_move(box, num, dir.data(), full, maxradius, &c);
+        t_device += ms_since(td0);
+        if (rc) return rc;
+        q_query += static_cast<double>(c.n_query);
+        q_filtered += static_cast<double>(c.n_filtered);
+        q_pairs += static_cast<double>(c.n_pairs);
+        const bool contact_flag = c.contact != 0;
+        const double move = c.move;
+        // calcul.cpp:147-153
+        deltatemps = deltatemps * (move / full + static_cast<double>(1 - 1));
+        ptime[U] += deltatemps;
+        pm.time = pm.time + deltatemps;
+        bool merged = false;
+        if (contact_flag) {
+            mcac_merge_result mr{};
+            auto tm0 = Clock::now();
+            rc = mcac_merge(box, &c, &mr);
+            t_device += ms_since(tm0);
+            if (rc) return rc;
+            if (mr.merged) {
+                merged = true;
+                auto th0 = Clock::now();
+                const size_t K = static_cast<size_t>(mr.keeper), R = static_cast<size_t>(mr.removed);
+                double newtime = ptime[K] + ptime[R] - pm.time;  // aggregat_list.cpp:387
+                double keep_old_ts = ts[K], rem_ts = ts[R];
+                nsph[K] = static_cast<int>(mr.n);
+                nsph[R] = 0;
+                vol[K] = mr.volume;
+                surf[K] = mr.surface;
+                rg[K] = mr.rg;
+                rmax[K] = mr.rmax;
+                com[K] = {mr.com[0], mr.com[1], mr.com[2]};
+                rcm[K] = {mr.rcm[0], mr.rcm[1], mr.rcm[2]};
+                overlapping[K] = mr.overlapping;
+                coord[K] = mr.coordination;
+                AggregatePhysics ph = aggregate_physics(pm, static_cast<size_t>(mr.n), mr.volume, mr.radius_sum, mr.rg);
+                dp[K] = ph.dp;
+                f_agg[K] = ph.f_agg;
+                d_m[K] = ph.d_m;
+                ts[K] = ph.time_step;
+                lpm[K] = ph.lpm;
+                dg_over_dp[K] = ph.dg_over_dp;
+                if (rmax[K] > maxradius) maxradius = rmax[K];  // aggregat.cpp:277-281
+                live.erase(std::lower_bound(live.begin(), live.end(), static_cast<int>(R)));
+                ptime[K] = newtime;
+                pick.replace(static_cast<int>(K), keep_old_ts, ts[K], static_cast<int>(R), rem_ts);
+                ++merges;
+                t_pick += ms_since(th0);
+                if (trace_merges) {
+                    mcac_merge_rec m{};
+                    m.step = step_count;
+                    m.keeper = static_cast<int64_t>(K);
+                    m.removed = static_cast<int64_t>(R);
+                    m.n = mr.n;
+                    m.volume = vol[K];
+                    m.surface = surf[K];
+                    m.rg = rg[K];
+                    m.rmax = rmax[K];
+                    m.dp = dp[K];
+                    m.f_agg = f_agg[K];
+                    m.lpm = lpm[K];
+                    m.time_step = ts[K];
+                    m.d_m = d_m[K];
+                    m.proper_time = ptime[K];
+                    m.overlapping = overlapping[K];
+                    m.coordination = coord[K];
+                    for (int l = 0; l < 3; ++l) {
+                        m.com[l] = com[K][static_cast<size_t>(l)];
+                        m.rcm[l] = rcm[K][static_cast<size_t>(l)];
+                    }
+                    merge_recs.push_back(m);
+                }
+            }
+        }
+        event = merged;
+        if (event) {
+            pm.n_iter_without_event = 0;
+            total_events++;
+            // refresh (aggregat_list.cpp:100-108): the largest time step heads the pick table
+            max_time_step = pick.factor();
+            avg_npp = static_cast<double>(N) / static_cast<double>(live.size());
+            pm_stale = true;
+        } else {
+            pm.n_iter_without_event++;
+        }
+        if (trace_steps) {
+            mcac_step_rec r{};
+            r.step = step_count;
+            r.agg = num;
+            r.label = num_label;
+            r.dir[0] = dir[0];
+            r.dir[1] = dir[1];
+            r.dir[2] = dir[2];
+            r.lpm = full;
+            r.dist = c.distance;
+            r.moving_sphere = c.moving_sphere;
+            r.other_sphere = c.other_sphere;
+            r.other_agg = c.other_agg;
+            r.contact = contact_flag;
+            r.merged = merged;
+            r.dt = deltatemps;
+            r.time = pm.time;
+            steps.push_back(r);
+        }
+        step_count++;
+        return 0;
+    }
+};
+
+}  // namespace mcac
+
+struct mcac_sim {
+    mcac::Simulation s;
+};
+
+extern "C" {
+
+int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mcac_sim **out, char *err, int errlen) {
+    if (!ini_text || !out) return MCAC_E_ARGUMENT;
+    auto *h = new mcac_sim();
+    try {
+        h->s.ini_text = ini_text;
+        h->s.pm.load(mcac::IniFile(ini_text));
+        if (!h->s.pm.unsupported.empty()) throw std::invalid_argument("unsupported option: " + h->s.pm.unsupported);
+    } catch (const std::exception &e) {
+        if (err && errlen > 0) std::snprintf(err, static_cast<size_t>(errlen), "%s", e.what());
+        delete h;
+        return MCAC_INPUT_ERROR;
+    }
+    h->s.out_dir = output_dir ? output_dir : "";
+    h->s.device = device;
+    *out = h;
+    return 0;
+}
+
+int mcac_sim_destroy(mcac_sim *sim) {
+    delete sim;
+    return 0;
+}
+
+int mcac_sim_init(mcac_sim *sim) {
+    if (!sim) return MCAC_E_ARGUMENT;
+    try {
+        return sim->s.init();
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "mcac_sim_init: %s\n", e.what());
+        return MCAC_UNKNOWN_ERROR;
+    }
+}
+
+int mcac_sim_run(mcac_sim *sim, int64_t max_steps, int64_t *done) {
+    if (!sim || !sim->s.initialized) return MCAC_E_ARGUMENT;
+    int64_t n = 0;
+    int rc = 0;
+    try {
+        while (n < max_steps && !sim->s.finished()) {
+            rc = sim->s.step();
+            if (rc) break;
+            ++n;
+        }
+        if (rc == 0 && sim->s.finished() && !sim->s.final_written) {
+            sim->s.save_advancement();  // calcul.cpp:282
+            sim->s.final_written = true;
+        }
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "mcac_sim_run: %s\n", e.what());
+        rc = MCAC_UNKNOWN_ERROR;
+    }
+    if (done) *done = n;
+    return rc;
+}
+
+int mcac_sim_finished(mcac_sim *sim) { return sim && sim->s.finished() ? 1 : 0; }
+
+int mcac_sim_set_trace(mcac_sim *sim, int steps, int merges) {
+    if (!sim) return MCAC_E_ARGUMENT;
+    sim->s.trace_steps = steps != 0;
+    sim->s.trace_merges = merges != 0;
+    return 0;
+}
+
+int64_t mcac_sim_n_steps(mcac_sim *sim) { return sim ? static_cast<int64_t>(sim->s.steps.size()) : 0; }
+int64_t mcac_sim_get_steps(mcac_sim *sim, mcac_step_rec *out, int64_t max) {
+    int64_t n = std::min<int64_t>(max, static_cast<int64_t>(sim->s.steps.size()));
+    std::memcpy(out, sim->s.steps.data(), static_cast<size_t>(n) * sizeof(mcac_step_rec));
+    return n;
+}
+int64_t mcac_sim_n_merges(mcac_sim *sim) { return sim ? static_cast<int64_t>(sim->s.merge_recs.size()) : 0; }
+int64_t mcac_sim_get_merges(mcac_sim *sim, mcac_merge_rec *out, int64_t max) {
+    int64_t n = std::min<int64_t>(max, static_cast<int64_t>(sim->s.merge_recs.size()));
+    std::memcpy(out, sim->s.merge_recs.data(), static_cast<size_t>(n) * sizeof(mcac_merge_rec));
+    return n;
+}
+
+double mcac_sim_scalar(mcac_sim *sim, const char *name) {
+    if (!sim || !name) return std::numeric_limits<double>::quiet_NaN();
+    auto &s = sim->s;
+    std::string n(name);
+    if (n == "box_length") return s.pm.box_length;
+    if (n == "time") return s.pm.time;
+    if (n == "maxradius") return s.maxradius;
+    if (n == "n_agg") return static_cast<double>(s.live.size());
+    if (n == "n_sph") return static_cast<double>(s.N);
+    if (n == "avg_npp") return s.avg_npp;
+    if (n == "max_time_step") return s.max_time_step;
+    if (n == "steps") return static_cast<double>(s.step_count);
+    if (n == "merges") return static_cast<double>(s.merges);
+    if (n == "events") return static_cast<double>(s.total_events);
+    if (n == "query") return s.q_query;
+    if (n == "filtered") return s.q_filtered;
+    if (n == "pairs") return s.q_pairs;
+    if (n == "ties") return s.n_ties;
+    if (n == "t_host_pick_ms") return s.t_pick;
+    if (n == "t_device_ms") return s.t_device;
+    if (n == "init_ms") return s.t_init;
+    if (n == "total_volume") {
+        double t = 0.0;
+        for (int id : s.live) t += s.vol[static_cast<size_t>(id)];
+        return t;
+    }
+    if (n == "volume_fraction") {
+        s.refresh_pm();
+        return s.pm.volume_fraction;
+    }
+    return std::numeric_limits<double>::quiet_NaN();
+}
+
+int64_t mcac_sim_n_aggs(mcac_sim *sim) { return sim ? static_cast<int64_t>(sim->s.live.size()) : 0; }
+
+int mcac_sim_export_aggs(mcac_sim *sim, int64_t *uid, double *lpm, double *time_step, double *proper_time,
+                         double *volume, double *rg, double *rmax, int64_t *n) {
+    if (!sim) return MCAC_E_ARGUMENT;
+    auto &s = sim->s;
+    for (size_t l = 0; l < s.live.size(); ++l) {
+        size_t id = static_cast<size_t>(s.live[l]);
+        if (uid) uid[l] = static_cast<int64_t>(id);
+        if (lpm) lpm[l] = s.lpm[id];
+        if (time_step) time_step[l] = s.ts[id];
+        if (proper_time) proper_time[l] = s.ptime[id];
+        if (volume) volume[l] = s.vol[id];
+        if (rg) rg[l] = s.rg[id];
+        if (rmax) rmax[l] = s.rmax[id];
+        if (n) n[l] = s.nsph[id];
+    }
+    return 0;
+}
+
+mcac_box *mcac_sim_box(mcac_sim *sim) { return sim ? sim->s.box : nullptr; }
+
+int mcac_physics_scalars(const char *ini_text, double out[8]) {
+    if (!ini_text || !out) return MCAC_E_ARGUMENT;
+    try {
+        mcac::PhysicalModel pm;
+        pm.load(mcac::IniFile(ini_text));
+        double v[8] = {pm.box_length, pm.gaz_mean_free_path, pm.viscosity, pm.mean_massic_radius,
+                       pm.friction_exponnant, pm.u_sg, pm.box_volume, pm.aggregate_concentration};
+        for (int i = 0; i < 8; ++i) out[i] = v[i];
+        return 0;
+    } catch (const std::exception &) {
+        return MCAC_INPUT_ERROR;
+    }
+}
+
+}  // extern "C"

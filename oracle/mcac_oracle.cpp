@@ -1128,6 +1128,18 @@ double ora_next_contact(void *sim, int64_t label, const double dir[3], double lp
     if (n_pairs) *n_pairs = s->q_pairs;
     return c.distance;
 }
+int ora_physics(const char *ini_text, double out[8]) {
+    try {
+        Physics pm;
+        pm.load(parse_ini(ini_text));
+        double v[8] = {pm.box_length, pm.gaz_mean_free_path, pm.viscosity, pm.mean_massic_radius,
+                       pm.friction_exponnant, pm.u_sg, pm.box_volume, pm.aggregate_concentration};
+        for (int i = 0; i < 8; ++i) out[i] = v[i];
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
 double ora_swept_sphere(const double p1[3], double r1, const double p2[3], double r2, const double dir[3],
                         double dist, double box) {
     return swept_sphere({p1[0], p1[1], p1[2]}, r1, {p2[0], p2[1], p2[2]}, r2, {dir[0], dir[1], dir[2]}, dist,

@@ -94,6 +94,11 @@ double ora_next_contact(void *sim, int64_t label, const double dir[3], double lp
                         int64_t *n_query, int64_t *n_filtered, int64_t *n_solved,
                         int64_t *n_pairs);
 
+/* PhysicalModel derived scalars for a parameter file, without running:
+ * out = [box_length, lambda, viscosity, rpeqmass, gamma, u_sg, box_volume,
+ * aggregate_concentration]. Returns 0 on success. */
+int ora_physics(const char *ini_text, double out[8]);
+
 /* Primitive: distance_to_contact(Sphere,Sphere,dir,dist) (sphere_contact.cpp:47-125). */
 double ora_swept_sphere(const double p1[3], double r1, const double p2[3], double r2,
                         const double dir[3], double dist, double box);

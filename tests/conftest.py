@@ -1,0 +1,12 @@
+import pathlib
+import sys
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+for p in (str(ROOT), str(ROOT / "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: runs HIP kernels on an MI355X (gfx950)")
+    config.addinivalue_line("markers", "slow: longer parity runs")

@@ -76,6 +76,7 @@ def lib():
         L.ora_inverf.argtypes = [_D]
         L.ora_inverfc.restype = _D
         L.ora_inverfc.argtypes = [_D]
+        L.ora_physics.argtypes = [ctypes.c_char_p, _DP]
         L.ora_verlet_query.restype = _I64
         L.ora_verlet_query.argtypes = [_I64, _D, _I64P, _I64, _DP, _DP, _D, _I64P, _I64]
         _lib = L
@@ -209,3 +210,14 @@ def verlet_query(n_div, width, cells, src, vec, dist):
     n = lib().ora_verlet_query(n_div, width, ip(cells), cells.size // 3, dp(v3(src)), dp(v3(vec)), dist,
                                ip(out), out.size)
     return out[:n]
+
+
+PHYSICS_KEYS = ("box_length", "lambda", "viscosity", "rpeqmass", "gamma", "u_sg", "box_volume",
+                "aggregate_concentration")
+
+
+def physics(ini_text):
+    out = np.zeros(8)
+    rc = lib().ora_physics(ini_text.encode(), dp(out))
+    assert rc == 0
+    return dict(zip(PHYSICS_KEYS, out.tolist()))

@@ -1,0 +1,146 @@
+"""HIP path (through the C-ABI) against the oracle on the same seeded inputs.
+
+Bar: bit-exact for every distance, index, merge geometry and time value."""
+import pathlib
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+CFG = pathlib.Path(__file__).parent / "configs"
+INF = float("inf")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mcac():
+    import mcac_amd
+    return mcac_amd
+
+
+def test_swept_sphere_batch_bit_exact():
+    """a9 on the device vs the oracle on random, touching, grazing and periodic inputs."""
+    say("case: swept-sphere batch, 60000 inputs")
+    rng = np.random.default_rng(3)
+    n = 60000
+    L = 2.4e-6
+    p1 = rng.uniform(-0.2 * L, 1.2 * L, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r1 = rng.uniform(5e-9, 40e-9, n)
+    r2 = rng.uniform(5e-9, 40e-9, n)
+    dist = rng.uniform(1e-8, 1.5 * L, n)
+    t = rng.uniform(0, 1, n)[:, None] * dist[:, None]
+    p2 = p1 + t * d + rng.normal(scale=1, size=(n, 3)) * (r1 + r2)[:, None]
+    p2[::7] = rng.uniform(0, L, (len(p2[::7]), 3))            # far ones
+    p2[1::11] = p1[1::11] + d[1::11] * (r1 + r2)[1::11, None]  # touching
+    p2[2::13] += L * rng.integers(-2, 3, (len(p2[2::13]), 3))  # other periodic images
+    got = mcac().swept_sphere_batch(p1, r1, p2, r2, d, dist, L)
+    ref = np.array([O.swept_sphere(p1[i], r1[i], p2[i], r2[i], d[i], dist[i], L) for i in range(n)])
+    assert np.isfinite(ref).sum() > n // 10
+    mism = np.nonzero(~((got == ref) | (np.isnan(got) & np.isnan(ref))))[0]
+    assert mism.size == 0, f"{mism.size} mismatches, first {mism[:5]}: {got[mism[:5]]} vs {ref[mism[:5]]}"
+
+
+def upload_oracle_state(sim):
+    sph, agg = sim.export()
+    m = mcac()
+    box = m.Box(sim.scalar("box_length"), 10, len(sph["x"]), len(agg["n"]))
+    box.upload(sph, dict(pos=agg["pos"], rcm=agg["rcm"], rmax=agg["rmax"], volume=agg["volume"],
+                         surface=agg["surface"], count=agg["n"], members=agg["members"]))
+    return box, sph, agg
+
+
+@pytest.mark.parametrize("cfg,steps", [("c1_cpu_reference.ini", 4000), ("lognormal_small.ini", 12000)])
+def test_next_contact_bit_exact(cfg, steps):
+    """a4-a9 on an evolved state: distance, sphere pair and other aggregate."""
+    say(f"case: next_contact parity {cfg} after {steps} steps")
+    sim = O.OracleSim((CFG / cfg).read_text(), fast_init=True, trace=False)
+    sim.run(steps)
+    box, sph, agg = upload_oracle_state(sim)
+    maxr = sim.scalar("maxradius")
+    rng = np.random.default_rng(17)
+    hits = 0
+    for q in range(400):
+        label = int(rng.integers(len(agg["n"])))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        lpm = agg["lpm"][label] * (1 + 4 * rng.random())
+        ref = sim.next_contact(label, d, lpm)
+        got = box.next_contact(label, d, lpm, maxr)
+        assert got["distance"] == ref[0], (q, label)
+        assert (got["moving_sphere"], got["other_sphere"], got["other_agg"]) == (ref[1], ref[2], ref[3]), (q, label)
+        assert got["n_query"] == ref[4]["query"]
+        assert got["n_filtered"] == ref[4]["filtered"]
+        hits += ref[0] < INF
+    assert hits > 20
+
+
+def compare_runs(cfg_text, max_steps, fast_init=True):
+    ora = O.OracleSim(cfg_text, fast_init=fast_init)
+    n_ora = ora.run(max_steps)
+    sim = mcac().Simulation(cfg_text, trace_steps=True, trace_merges=True)
+    n_gpu = sim.run(max_steps)
+    assert n_gpu == n_ora
+    a, b = ora.steps(), sim.steps()
+    assert len(a) == len(b)
+    for f in a.dtype.names:
+        eq = (a[f] == b[f]) | (np.isnan(a[f]) & np.isnan(b[f])) if a[f].dtype.kind == "f" else a[f] == b[f]
+        if eq.ndim > 1:
+            eq = eq.all(axis=1)
+        bad = np.nonzero(~eq)[0]
+        assert bad.size == 0, f"step field {f} differs first at step {bad[0]}: {a[bad[0]]} vs {b[bad[0]]}"
+    ma, mb = ora.merges(), sim.merges()
+    assert len(ma) == len(mb)
+    for f in ma.dtype.names:
+        eq = ma[f] == mb[f]
+        if eq.ndim > 1:
+            eq = eq.all(axis=1)
+        bad = np.nonzero(~eq)[0]
+        assert bad.size == 0, f"merge field {f} differs first at merge {bad[0]}: {ma[bad[0]]} vs {mb[bad[0]]}"
+    assert sim.scalar("time") == ora.scalar("time")
+    assert sim.scalar("n_agg") == ora.scalar("n_agg")
+    return ora, sim
+
+
+def test_full_run_c1_bit_exact():
+    """N=1000 monodisperse to one aggregate: every step and merge identical."""
+    say("case: full C1 run, N=1000 -> 1 aggregate")
+    text = (CFG / "c1_cpu_reference.ini").read_text()
+    ora, sim = compare_runs(text, 10 ** 7, fast_init=False)
+    assert sim.finished() and sim.scalar("n_agg") == 1
+
+
+def test_lognormal_run_bit_exact():
+    """N=5000 lognormal to 50 aggregates: every step and merge identical."""
+    say("case: lognormal N=5000 -> 50 aggregates")
+    compare_runs((CFG / "lognormal_small.ini").read_text(), 10 ** 7)
+
+
+def test_c2_prefix_bit_exact():
+    """N=100,000 lognormal (plain GPU configuration): first 4000 steps identical."""
+    say("case: C2 prefix N=1e5, 4000 steps")
+    compare_runs((CFG / "c2_plain_gpu.ini").read_text(), 4000)
+
+
+def test_c3_full_size_invariants():
+    """N=1,000,000: sphere count, volume and merge bookkeeping hold over 3000 steps."""
+    say("case: C3 N=1e6 invariants, 3000 steps")
+    text = (CFG / "c3_n1m.ini").read_text()
+    sim = mcac().Simulation(text, trace_steps=True, trace_merges=True)
+    v0 = sim.scalar("total_volume")
+    n = sim.run(3000)
+    assert n == 3000
+    st, mg = sim.steps(), sim.merges()
+    assert st["merged"].sum() == len(mg) == 1_000_000 - sim.scalar("n_agg")
+    aggs = sim.aggregates()
+    assert aggs["n"].sum() == 1_000_000
+    assert sim.scalar("total_volume") == pytest.approx(v0, rel=1e-12)
+    assert np.all(np.diff(st["time"]) > 0)
+    assert np.all(st["dist"][st["contact"] == 1] <= st["lpm"][st["contact"] == 1])
