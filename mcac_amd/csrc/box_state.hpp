@@ -1,0 +1,107 @@
+// Device-resident state of one aggregation box and the small records that
+// kernels publish to pinned host memory.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/mcac_gpu.h"
+
+namespace mcac_dev {
+
+constexpr int kBlock = 256;            // threads per block (4 waves of 64)
+constexpr int kFilterBlocks = 128;     // contact-search filter grid
+constexpr int kSolveBlocks = 256;      // pair-solve grid (one block per CU)
+constexpr long long kChunk = 4096;     // minimum sphere pairs per solve chunk
+constexpr int kMaxVisit = 4096;        // visited cells handled with an LDS prefix
+constexpr int kMaxPlan = 4096;         // filtered candidates planned in LDS
+constexpr int kOverlapSlots = 16;      // contacts kept per sphere row (multi-kernel merge)
+constexpr int kSmallMerge = 256;       // merged aggregates up to this size use one fused kernel
+
+// Counters and results of the contact search that stay on the device.
+struct SearchState {
+    unsigned int nfilt;        // filtered candidates appended by k_filter
+    unsigned int overflow;     // candidate buffer overflow flag
+    unsigned int done_solve;   // ticket of the participating solve blocks
+    unsigned int dirty_count;  // entries of the dirty list
+    unsigned long long nquery; // aggregates of the visited cells
+    double newcom[3];          // position of the mover after the move
+    double refpos[3];          // new CoM - relative CoM (sphere 0 position)
+    int newcell;
+    int pad;
+};
+
+// Record published to pinned host memory by the last block of a search.
+struct StepRecord {
+    double dist, move;
+    long long ms, os, oagg;
+    int contact, status;
+    unsigned long long nquery, nfilt, npairs;
+    unsigned long long seq;    // written last (system-scope release)
+};
+
+// Record published by the last merge kernel.
+struct MergeRecord {
+    mcac_merge_result res;
+    unsigned long long seq;
+};
+
+// Merge work shared between the kernels of the multi-kernel merge.
+struct MergeState {
+    int ok, keep, rem, n_k, n_r, new_start, start_k, start_r;
+    double diffpos[3], refpos[3], r[3];
+    unsigned long long rmax_bits;
+    unsigned int row_overflow, pad;
+    mcac_merge_result res;
+};
+
+struct DevBox {
+    // spheres, index order
+    double *sx, *sy, *sz, *sr, *svol, *ssurf, *srx, *sry, *srz;
+    // aggregates, stable-id order
+    double *ax, *ay, *az, *arx, *ary, *arz, *armax, *avol, *asurf, *arg;
+    double *csvx, *csvy, *csvz, *csr, *csvr2;  // list-order partial sums the merge continues
+    int *astart, *acount, *acell;              // member segment, size, reference Verlet cell (-1 dead)
+    int *members;
+    int *alive;                                // live ids, ascending (may hold dead ids)
+    // cell list: CSR by reference cell, built from the live ids
+    int *cstart, *citems, *ccursor;
+    unsigned char *dirty;                      // moved, merged or removed since the build
+    int *dirty_list;
+    // search scratch
+    double *cand_db, *cand_exact;
+    int *cand_rank, *cand_id, *order;
+    long long *cand_pair;
+    double *part_d;
+    long long *part_p;
+    // merge scratch
+    double *dc;
+    int *row_cnt, *row_j;
+    double *row_c;
+    SearchState *ss;
+    MergeState *ms;
+};
+
+struct SearchArgs {
+    int src;
+    int n_div;
+    int cap;
+    int n_built;               // items in the CSR cell list
+    long long part_cap;
+    double dir[3];
+    double lpm;
+    double maxradius;
+    double L;
+    int apply;                 // translate by the resulting move
+    int translate_in_finish;   // mover small enough for the finishing block
+    StepRecord *out;           // pinned host record
+    unsigned long long seq;
+};
+
+struct MergeArgs {
+    long long ms, os, magg, oagg;
+    int keep, rem, n_k, n_r, new_start, start_k, start_r, n_div;
+    double L;
+    MergeRecord *out;          // pinned host record
+    unsigned long long seq;
+};
+
+}  // namespace mcac_dev

@@ -1,0 +1,66 @@
+// Device cell list of the contact search (a4/a5): live aggregates bucketed by
+// their reference Verlet cell (aggregat.cpp:699-705) in CSR form.  Moves and
+// merges do not edit the CSR; they mark the aggregate dirty and append it to a
+// dirty list, which the search scans in full.  The host rebuilds the CSR when
+// the dirty list grows past ~sqrt(6 N) entries (rebuild cost ~3 N reads vs
+// ~half a dirty list per step).
+#pragma once
+#include "kernels_common.hpp"
+
+namespace mcac_dev {
+
+__global__ __launch_bounds__(kBlock) void k_cell_clear(DevBox b, int n_cells) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= n_cells; c += gridDim.x * blockDim.x) b.ccursor[c] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_count(DevBox b, int n_alive) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n_alive; t += gridDim.x * blockDim.x) {
+        int id = b.alive[t];
+        int c = b.acell[id];
+        if (c >= 0) atomicAdd(&b.ccursor[c], 1);
+    }
+}
+
+// Exclusive scan of the counts into cstart and ccursor; resets the dirty list.
+__global__ __launch_bounds__(1024) void k_cell_scan(DevBox b, int n_cells) {
+    __shared__ int part[1024];
+    const int per = (n_cells + 1023) / 1024;
+    const int lo = threadIdx.x * per;
+    const int hi = (lo + per < n_cells) ? lo + per : n_cells;
+    int s = 0;
+    for (int i = lo; i < hi; ++i) s += b.ccursor[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < 1024; ++i) {
+            int v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        b.cstart[n_cells] = acc;
+        b.ss->dirty_count = 0;
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int i = lo; i < hi; ++i) {
+        int v = b.ccursor[i];
+        b.cstart[i] = acc;
+        b.ccursor[i] = acc;
+        acc += v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_scatter(DevBox b, int n_alive) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n_alive; t += gridDim.x * blockDim.x) {
+        int id = b.alive[t];
+        int c = b.acell[id];
+        b.dirty[id] = (c >= 0) ? 0 : 1;
+        if (c >= 0) {
+            int pos = atomicAdd(&b.ccursor[c], 1);
+            b.citems[pos] = id;
+        }
+    }
+}
+
+}  // namespace mcac_dev

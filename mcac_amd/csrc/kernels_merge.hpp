@@ -1,0 +1,485 @@
+// Merge of two aggregates and the post-merge update (a10 + a11):
+//   AggregatList::merge     aggregat_list.cpp:367-410
+//   Aggregate::merge        aggregat.cpp:486-544
+//   Aggregate::update       aggregat.cpp:284-288 -> update_distances_and_overlapping
+//                           (:719-764), compute_volume_surface (:321-430, method
+//                           none), compute_mass_center / max_radius /
+//                           giration_radius (:431-483)
+// Every order-dependent FP64 sum runs sequentially in the reference's list
+// order.  The merged list is the keeper's list followed by the removed one's,
+// so the keeper's partial sums are its cached totals and only the removed
+// spheres are added; Σ vol·dc² and the overlap sum need the whole list.
+// Up to kSmallMerge spheres one fused block does everything; larger merges run
+// the multi-kernel sequence below.  Both publish the result to pinned memory.
+#pragma once
+#include "kernels_common.hpp"
+
+namespace mcac_dev {
+
+// aggregat_list.cpp:375-378 and aggregat.cpp:496-535 (single thread).
+__device__ __forceinline__ bool merge_prepare(const DevBox &b, const MergeArgs &a, double diffpos[3],
+                                              double refpos[3]) {
+    const int ms = static_cast<int>(a.ms), os = static_cast<int>(a.os);
+    if (!contact(b.sx[ms], b.sy[ms], b.sz[ms], b.sr[ms], b.sx[os], b.sy[os], b.sz[os], b.sr[os], a.L)) return false;
+    int mysphere = ms, othersphere = os;
+    if (a.keep != a.magg) {
+        mysphere = os;
+        othersphere = ms;
+    }
+    const int s0 = b.members[a.start_k];
+    const double ref[3] = {b.sx[s0], b.sy[s0], b.sz[s0]};
+    const double mpos[3] = {b.sx[mysphere], b.sy[mysphere], b.sz[mysphere]};
+    const double opos[3] = {b.sx[othersphere], b.sy[othersphere], b.sz[othersphere]};
+    const double mrel[3] = {b.srx[mysphere], b.sry[mysphere], b.srz[mysphere]};
+    const double orel[3] = {b.srx[othersphere], b.sry[othersphere], b.srz[othersphere]};
+    for (int l = 0; l < 3; ++l) {
+        double dcontact = periodic_distance(opos[l] - mpos[l], a.L);
+        diffpos[l] = (mrel[l] + dcontact) - orel[l];
+        refpos[l] = ref[l];
+    }
+    return true;
+}
+
+// Relabel/remap one sphere of the removed aggregate (aggregat.cpp:524-535).
+__device__ __forceinline__ void remap_sphere(const DevBox &b, int s, const double diffpos[3], const double refpos[3]) {
+    double rx = b.srx[s] + diffpos[0];
+    double ry = b.sry[s] + diffpos[1];
+    double rz = b.srz[s] + diffpos[2];
+    b.srx[s] = rx;
+    b.sry[s] = ry;
+    b.srz[s] = rz;
+    b.sx[s] = rx + refpos[0];
+    b.sy[s] = ry + refpos[1];
+    b.sz[s] = rz + refpos[2];
+}
+
+// Stores the merged aggregate's geometry and marks keeper/removed (single thread).
+__device__ __forceinline__ void store_merged(const DevBox &b, const MergeArgs &a, double V, double S, double vx, double vy,
+                                             double vz, double sr, double svr2, const double r[3], const double com[3]) {
+    const int k = a.keep;
+    b.avol[k] = V;
+    b.asurf[k] = S;
+    b.csvx[k] = vx;
+    b.csvy[k] = vy;
+    b.csvz[k] = vz;
+    b.csr[k] = sr;
+    b.csvr2[k] = svr2;
+    b.arx[k] = r[0];
+    b.ary[k] = r[1];
+    b.arz[k] = r[2];
+    b.ax[k] = com[0];
+    b.ay[k] = com[1];
+    b.az[k] = com[2];
+    b.acell[k] = ref_cell(com[0], com[1], com[2], a.n_div, a.L);
+    b.astart[k] = a.new_start;
+    b.acount[k] = a.n_k + a.n_r;
+    b.acell[a.rem] = -1;
+    b.acount[a.rem] = 0;
+    mark_dirty(b, k);
+    mark_dirty(b, a.rem);
+}
+
+__device__ __forceinline__ void publish_merge(const MergeArgs &a, const mcac_merge_result &res) {
+    a.out->res = res;
+    __threadfence_system();
+    __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// ------------------------------------------------------------------ fused merge (n <= kSmallMerge)
+__global__ __launch_bounds__(kBlock) void k_merge_small(DevBox b, MergeArgs a) {
+    __shared__ double diffpos[3], refpos[3], r[3];
+    __shared__ int ok;
+    __shared__ double dcs[kSmallMerge], vols[kSmallMerge], rads[kSmallMerge];
+    __shared__ double rlx[kSmallMerge], rly[kSmallMerge], rlz[kSmallMerge];
+    __shared__ int nrow[kSmallMerge];
+    __shared__ double crow[kSmallMerge * 8];
+    __shared__ unsigned int row_over;
+    const int n = a.n_k + a.n_r;
+    if (threadIdx.x == 0) {
+        double dp[3], rp[3];
+        ok = merge_prepare(b, a, dp, rp) ? 1 : 0;
+        for (int l = 0; l < 3; ++l) {
+            diffpos[l] = dp[l];
+            refpos[l] = rp[l];
+        }
+        row_over = 0;
+    }
+    __syncthreads();
+    if (!ok) {
+        if (threadIdx.x == 0) {
+            mcac_merge_result res{};
+            res.keeper = a.keep;
+            res.removed = a.rem;
+            res.merged = 0;
+            publish_merge(a, res);
+        }
+        return;
+    }
+    // member list and removed spheres
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int s;
+        if (t < a.n_k) {
+            s = b.members[a.start_k + t];
+            if (a.new_start != a.start_k) b.members[a.new_start + t] = s;
+        } else {
+            s = b.members[a.start_r + (t - a.n_k)];
+            b.members[a.new_start + t] = s;
+            remap_sphere(b, s, diffpos, refpos);
+        }
+        vols[t] = b.svol[s];
+        rads[t] = b.sr[s];
+        rlx[t] = b.srx[s];
+        rly[t] = b.sry[s];
+        rlz[t] = b.srz[s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int k = a.keep;
+        double V = b.avol[k], S = b.asurf[k], vx = b.csvx[k], vy = b.csvy[k], vz = b.csvz[k];
+        double sr = b.csr[k], svr2 = b.csvr2[k];
+        for (int i = a.n_k; i < n; ++i) {
+            const double v = vols[i];
+            V = V + v;                              // aggregat.cpp:388
+            vx += rlx[i] * v;                       // aggregat.cpp:439
+            vy += rly[i] * v;
+            vz += rlz[i] * v;
+            sr += rads[i];                          // aggregat.cpp:259
+            svr2 = svr2 + v * (rads[i] * rads[i]);  // aggregat.cpp:477
+        }
+        for (int i = a.n_k; i < n; ++i) S = S + b.ssurf[b.members[a.new_start + i]];  // aggregat.cpp:389
+        double rr[3] = {vx / V, vy / V, vz / V};    // aggregat.cpp:442
+        const int s0 = b.members[a.new_start];
+        double com[3] = {periodic_position(b.sx[s0] + rr[0], a.L), periodic_position(b.sy[s0] + rr[1], a.L),
+                         periodic_position(b.sz[s0] + rr[2], a.L)};  // aggregat.cpp:448
+        store_merged(b, a, V, S, vx, vy, vz, sr, svr2, rr, com);
+        for (int l = 0; l < 3; ++l) r[l] = rr[l];
+        mcac_merge_result &res = b.ms->res;
+        res.keeper = k;
+        res.removed = a.rem;
+        res.n = n;
+        res.merged = 1;
+        res.status = (V <= 0 || S <= 0) ? MCAC_VOL_SURF_ERROR : 0;  // aggregat.cpp:427-429
+        res.volume = V;
+        res.surface = S;
+        res.radius_sum = sr;
+        for (int l = 0; l < 3; ++l) {
+            res.com[l] = com[l];
+            res.rcm[l] = rr[l];
+        }
+    }
+    __syncthreads();
+    // distances to the centre of mass (aggregat.cpp:444-447) and contacts per row
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        double d0 = rlx[t] - r[0], d1 = rly[t] - r[1], d2 = rlz[t] - r[2];
+        dcs[t] = __builtin_sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    // aggregat.cpp:736-759: row i in ascending j, so each row is already in reference order
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int cnt = 0;
+        for (int j = i + 1; j < n; ++j) {
+            double d0 = rlx[i] - rlx[j], d1 = rly[i] - rly[j], d2 = rlz[i] - rlz[j];
+            double s2 = rads[i] + rads[j];
+            double q = d0 * d0 + d1 * d1 + d2 * d2;
+            if (q > 1.0001 * (s2 * s2)) continue;
+            double dist = __builtin_sqrt(q);
+            if (dist <= (1. + kCoordinationEpsilon) * s2) {
+                if (cnt < 8) crow[i * 8 + cnt] = (s2 - dist) / s2;
+                else atomicOr(&row_over, 1u);
+                ++cnt;
+            }
+        }
+        nrow[i] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double rmax = 0.0, arg = 0.;
+        for (int i = 0; i < n; ++i) {
+            rmax = smax(rmax, rads[i] + dcs[i]);       // aggregat.cpp:462
+            arg = arg + vols[i] * (dcs[i] * dcs[i]);   // aggregat.cpp:476
+        }
+        const int k = a.keep;
+        const double V = b.avol[k];
+        const double rg = __builtin_sqrt(__builtin_fabs((arg + 3. / 5. * b.csvr2[k]) / V));  // aggregat.cpp:481
+        b.armax[k] = rmax;
+        b.arg[k] = rg;
+        double overlapping = 0.0;
+        long long inter = 0;
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < nrow[i] && c < 8; ++c) {
+                overlapping += 2.0 * crow[i * 8 + c];  // aggregat.cpp:755
+                inter += 2;
+            }
+        double coord = 0.0;
+        if (inter > 0) {
+            overlapping /= static_cast<double>(inter);
+            coord = static_cast<double>(inter) / static_cast<double>(n);
+        }
+        mcac_merge_result res = b.ms->res;
+        res.rmax = rmax;
+        res.rg = rg;
+        res.overlapping = overlapping;
+        res.coordination = coord;
+        if (row_over) res.status = MCAC_E_CAPACITY;
+        publish_merge(a, res);
+    }
+}
+
+// ------------------------------------------------------------------ multi-kernel merge
+__global__ void k_merge_prep(DevBox b, MergeArgs a) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    MergeState *m = b.ms;
+    m->ok = 0;
+    m->rmax_bits = 0ull;
+    m->row_overflow = 0u;
+    m->res = mcac_merge_result{};
+    m->res.keeper = a.keep;
+    m->res.removed = a.rem;
+    double dp[3], rp[3];
+    if (!merge_prepare(b, a, dp, rp)) {
+        publish_merge(a, m->res);
+        return;
+    }
+    for (int l = 0; l < 3; ++l) {
+        m->diffpos[l] = dp[l];
+        m->refpos[l] = rp[l];
+    }
+    m->keep = a.keep;
+    m->rem = a.rem;
+    m->n_k = a.n_k;
+    m->n_r = a.n_r;
+    m->new_start = a.new_start;
+    m->start_k = a.start_k;
+    m->start_r = a.start_r;
+    m->ok = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_lists(DevBox b) {
+    const MergeState *m = b.ms;
+    if (!m->ok) return;
+    const int nk = m->n_k, nr = m->n_r;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nk + nr; t += gridDim.x * blockDim.x) {
+        if (t < nk) {
+            if (m->new_start != m->start_k) b.members[m->new_start + t] = b.members[m->start_k + t];
+        } else {
+            int s = b.members[m->start_r + (t - nk)];
+            b.members[m->new_start + t] = s;
+            remap_sphere(b, s, m->diffpos, m->refpos);
+        }
+    }
+}
+
+// Ordered sums over the removed spheres: one block stages tiles in LDS,
+// thread 0 accumulates them in list order.
+__global__ __launch_bounds__(kBlock) void k_merge_seq(DevBox b, MergeArgs a) {
+    MergeState *m = b.ms;
+    if (!m->ok) return;
+    __shared__ double tv[kBlock], ts[kBlock], tx[kBlock], ty[kBlock], tz[kBlock], tr[kBlock];
+    const int k = m->keep, nk = m->n_k, nr = m->n_r, base = m->new_start + nk;
+    double V = b.avol[k], S = b.asurf[k], vx = b.csvx[k], vy = b.csvy[k], vz = b.csvz[k];
+    double sr = b.csr[k], svr2 = b.csvr2[k];
+    for (int t0 = 0; t0 < nr; t0 += kBlock) {
+        const int cnt = (nr - t0 < kBlock) ? nr - t0 : kBlock;
+        if (threadIdx.x < cnt) {
+            int s = b.members[base + t0 + threadIdx.x];
+            tv[threadIdx.x] = b.svol[s];
+            ts[threadIdx.x] = b.ssurf[s];
+            tx[threadIdx.x] = b.srx[s];
+            ty[threadIdx.x] = b.sry[s];
+            tz[threadIdx.x] = b.srz[s];
+            tr[threadIdx.x] = b.sr[s];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < cnt; ++i) {
+                const double v = tv[i];
+                V = V + v;
+                S = S + ts[i];
+                vx += tx[i] * v;
+                vy += ty[i] * v;
+                vz += tz[i] * v;
+                sr += tr[i];
+                svr2 = svr2 + v * (tr[i] * tr[i]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double r[3] = {vx / V, vy / V, vz / V};
+    const int s0 = b.members[m->new_start];
+    double com[3] = {periodic_position(b.sx[s0] + r[0], a.L), periodic_position(b.sy[s0] + r[1], a.L),
+                     periodic_position(b.sz[s0] + r[2], a.L)};
+    store_merged(b, a, V, S, vx, vy, vz, sr, svr2, r, com);
+    for (int l = 0; l < 3; ++l) m->r[l] = r[l];
+    mcac_merge_result &res = m->res;
+    res.keeper = k;
+    res.removed = a.rem;
+    res.n = nk + nr;
+    res.merged = 1;
+    res.status = (V <= 0 || S <= 0) ? MCAC_VOL_SURF_ERROR : 0;
+    res.volume = V;
+    res.surface = S;
+    res.radius_sum = sr;
+    for (int l = 0; l < 3; ++l) {
+        res.com[l] = com[l];
+        res.rcm[l] = r[l];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_dc(DevBox b) {
+    MergeState *m = b.ms;
+    if (!m->ok) return;
+    const int n = m->n_k + m->n_r, st = m->new_start;
+    const double r0 = m->r[0], r1 = m->r[1], r2 = m->r[2];
+    double mx = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int s = b.members[st + t];
+        double d0 = b.srx[s] - r0, d1 = b.sry[s] - r1, d2 = b.srz[s] - r2;
+        double dc = __builtin_sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        b.dc[t] = dc;
+        mx = smax(mx, b.sr[s] + dc);
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = smax(mx, __shfl_down(mx, off, 64));
+    // non-negative doubles order like their bit patterns
+    if ((threadIdx.x & 63) == 0) atomicMax(&m->rmax_bits, static_cast<unsigned long long>(__double_as_longlong(mx)));
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b) {
+    MergeState *m = b.ms;
+    if (!m->ok) return;
+    __shared__ double tv[kBlock], td[kBlock];
+    const int k = m->keep, n = m->n_k + m->n_r, st = m->new_start;
+    double arg = 0.;
+    for (int t0 = 0; t0 < n; t0 += kBlock) {
+        const int cnt = (n - t0 < kBlock) ? n - t0 : kBlock;
+        if (threadIdx.x < cnt) {
+            int s = b.members[st + t0 + threadIdx.x];
+            tv[threadIdx.x] = b.svol[s];
+            td[threadIdx.x] = b.dc[t0 + threadIdx.x];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < cnt; ++i) arg = arg + tv[i] * (td[i] * td[i]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double rmax = __longlong_as_double(static_cast<long long>(m->rmax_bits));
+    const double rg = __builtin_sqrt(__builtin_fabs((arg + 3. / 5. * b.csvr2[k]) / b.avol[k]));
+    b.armax[k] = rmax;
+    b.arg[k] = rg;
+    m->res.rmax = rmax;
+    m->res.rg = rg;
+}
+
+// Pair pass of the merged aggregate in kBlock x kBlock tiles, j tile in LDS.
+__global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b) {
+    const MergeState *m = b.ms;
+    if (!m->ok) return;
+    const int n = m->n_k + m->n_r, st = m->new_start;
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj < ti) return;
+    __shared__ double jx[kBlock], jy[kBlock], jz[kBlock], jr[kBlock];
+    const int jbase = tj * kBlock;
+    const int jt = jbase + threadIdx.x;
+    if (jt < n) {
+        int s = b.members[st + jt];
+        jx[threadIdx.x] = b.srx[s];
+        jy[threadIdx.x] = b.sry[s];
+        jz[threadIdx.x] = b.srz[s];
+        jr[threadIdx.x] = b.sr[s];
+    }
+    __syncthreads();
+    const int i = ti * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int si = b.members[st + i];
+    const double ix = b.srx[si], iy = b.sry[si], iz = b.srz[si], ir = b.sr[si];
+    const int jend = (jbase + kBlock < n) ? jbase + kBlock : n;
+    for (int j = (jbase > i + 1 ? jbase : i + 1); j < jend; ++j) {
+        const int l = j - jbase;
+        double d0 = ix - jx[l], d1 = iy - jy[l], d2 = iz - jz[l];
+        double s2 = ir + jr[l];
+        double q = d0 * d0 + d1 * d1 + d2 * d2;
+        if (q > 1.0001 * (s2 * s2)) continue;  // well outside the contact tolerance
+        double dist = __builtin_sqrt(q);
+        if (dist <= (1. + kCoordinationEpsilon) * s2) {
+            double c = (s2 - dist) / s2;
+            int slot = atomicAdd(&b.row_cnt[i], 1);
+            if (slot < kOverlapSlots) {
+                b.row_j[static_cast<long long>(i) * kOverlapSlots + slot] = j;
+                b.row_c[static_cast<long long>(i) * kOverlapSlots + slot] = c;
+            } else {
+                atomicOr(&b.ms->row_overflow, 1u);
+            }
+        }
+    }
+}
+
+// Ordered overlap sum in (i, j) order; clears the row counters; publishes.
+__global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
+    MergeState *m = b.ms;
+    if (!m->ok) return;
+    const int n = m->n_k + m->n_r;
+    __shared__ int tcnt[kBlock];
+    double overlapping = 0.0;
+    long long inter = 0;
+    for (int t0 = 0; t0 < n; t0 += kBlock) {
+        const int cnt = (n - t0 < kBlock) ? n - t0 : kBlock;
+        const int i = t0 + threadIdx.x;
+        if (threadIdx.x < cnt) {
+            int rc = b.row_cnt[i];
+            if (rc > kOverlapSlots) rc = kOverlapSlots;
+            const long long base = static_cast<long long>(i) * kOverlapSlots;
+            for (int x = 1; x < rc; ++x) {  // insertion sort of the row by j
+                int kj = b.row_j[base + x];
+                double kc = b.row_c[base + x];
+                int y = x - 1;
+                while (y >= 0 && b.row_j[base + y] > kj) {
+                    b.row_j[base + y + 1] = b.row_j[base + y];
+                    b.row_c[base + y + 1] = b.row_c[base + y];
+                    --y;
+                }
+                b.row_j[base + y + 1] = kj;
+                b.row_c[base + y + 1] = kc;
+            }
+            b.row_cnt[i] = 0;
+            tcnt[threadIdx.x] = rc;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int x = 0; x < cnt; ++x) {
+                const long long base = static_cast<long long>(t0 + x) * kOverlapSlots;
+                for (int y = 0; y < tcnt[x]; ++y) {
+                    overlapping += 2.0 * b.row_c[base + y];
+                    inter += 2;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double coord = 0.0;
+    if (inter > 0) {
+        overlapping /= static_cast<double>(inter);
+        coord = static_cast<double>(inter) / static_cast<double>(n);
+    }
+    m->res.overlapping = overlapping;
+    m->res.coordination = coord;
+    if (m->row_overflow) m->res.status = MCAC_E_CAPACITY;
+    publish_merge(a, m->res);
+}
+
+__global__ void k_gather_members(DevBox b, int agg, int *out, int n) {
+    const int st = b.astart[agg];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) out[t] = b.members[st + t];
+}
+
+// Pool compaction: copy each live aggregate's members to its new start.
+__global__ void k_compact(const int *old_members, int *new_members, const int *old_start, const int *new_start,
+                          const int *count, const int *ids, int n_ids) {
+    for (int a = blockIdx.x; a < n_ids; a += gridDim.x) {
+        const int id = ids[a];
+        const int n = count[id], so = old_start[id], sn = new_start[a];
+        for (int t = threadIdx.x; t < n; t += blockDim.x) new_members[sn + t] = old_members[so + t];
+    }
+}
+
+}  // namespace mcac_dev
