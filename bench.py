@@ -48,6 +48,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-stride", type=int, default=8,
+                    help="HIP events around every n-th contact search of the timed region")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this path runs one aggregation box per GPU "
@@ -64,8 +66,9 @@ def main() -> int:
     if done != args.warmup:
         raise RuntimeError("warm-up ended early")
     box = sim.box()
-    box.set_timing(True)
-    s0 = {k: sim.scalar(k) for k in ("query", "filtered", "pairs", "merges", "t_host_pick_ms", "t_device_ms")}
+    box.set_timing(max(1, args.timing_stride))
+    s0 = {k: sim.scalar(k) for k in ("query", "filtered", "pairs", "merges", "t_host_pick_ms", "t_host_merge_ms",
+                                     "t_device_ms", "pick_rescales", "ties")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done = sim.run(args.steps)
@@ -82,9 +85,11 @@ def main() -> int:
     # Algorithmic bytes per launch: 36 B per aggregate of the visited Verlet cells
     # (CoM x,y,z + rmax 8 B each, 4 B id).
     n_search = max(tm["n_search"], 1.0)
+    if tm["n_search"] < 1:
+        raise RuntimeError("no timed contact search: lower --timing-stride")
     filt_ms = tm["filter_ms"] / n_search
     solve_ms = tm["solve_ms"] / n_search
-    bytes_per_launch = 36.0 * d["query"] / n_search
+    bytes_per_launch = 36.0 * d["query"] / args.steps
     achieved = bytes_per_launch / (filt_ms * 1e-3) / 1e9
     pair_tests = d["query"] + d["pairs"]
     out = {
@@ -108,12 +113,15 @@ def main() -> int:
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": filt_ms},
         "candidate_pair_tests_per_s": pair_tests / elapsed,
         "breakdown": {"init_s": t_init, "host_pick_ms_per_step": d["t_host_pick_ms"] / args.steps,
+                      "host_merge_ms_per_step": d["t_host_merge_ms"] / args.steps,
+                      "pick_rescales": d["pick_rescales"], "pick_ties": d["ties"],
                       "device_call_ms_per_step": d["t_device_ms"] / args.steps,
                       "k_filter_ms": filt_ms, "k_solve_ms": solve_ms,
                       "k_translate_ms": tm["translate_ms"] / max(tm["n_translate"], 1.0),
                       "merge_ms": tm["merge_ms"] / max(tm["n_merge"], 1.0),
-                      "merges": d["merges"], "avg_candidates_visited": d["query"] / n_search,
-                      "avg_candidates_filtered": d["filtered"] / n_search},
+                      "merges": d["merges"], "avg_candidates_visited": d["query"] / args.steps,
+                      "avg_candidates_filtered": d["filtered"] / args.steps,
+                      "timed_searches": tm["n_search"]},
     }
     if not args.no_cpu:
         cb = cpu_baseline(text, args.cpu_steps)

@@ -115,6 +115,19 @@ int mcac_move(mcac_box *box, int64_t agg, const double dir[3], double lpm, doubl
               mcac_contact *out);
 int mcac_merge(mcac_box *box, const mcac_contact *contact, mcac_merge_result *out);
 
+/* Pipelined form of mcac_move (calcul.cpp:119-144 issued ahead of time): the
+ * search and move are enqueued on the box stream and a ticket returned at once,
+ * so the caller can prepare and enqueue the next step while this one runs.  A
+ * step that finds a contact closes a gate: steps enqueued after it do nothing
+ * and report skipped = 1 from mcac_move_result, until mcac_release is called
+ * (after the caller has merged).  At most 8 steps may be in flight. */
+int mcac_move_enqueue(mcac_box *box, int64_t agg, const double dir[3], double lpm, double maxradius,
+                      uint64_t *ticket);
+/* Waits for an enqueued step; *skipped is 1 when it ran behind a closed gate. */
+int mcac_move_result(mcac_box *box, uint64_t ticket, mcac_contact *out, int *skipped);
+/* Reopens the gate closed by a step that found a contact. */
+int mcac_release(mcac_box *box);
+
 int mcac_box_download_spheres(mcac_box *box, double *x, double *y, double *z, double *rx, double *ry,
                               double *rz);
 int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rmax, double *volume,
@@ -122,10 +135,11 @@ int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rm
 /* Members of one aggregate in list order; returns the count via *n. */
 int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int64_t *n);
 
-/* Kernel timing on the box's stream (HIP events).  When enabled, every search
- * and merge records events; mcac_box_timing returns accumulated milliseconds
- * and launch counts: [search_ms, translate_ms, merge_ms, n_search, n_translate,
- * n_merge, filter_ms, solve_ms]. */
+/* Kernel timing on the box's stream (HIP events).  enable = n >= 1 records
+ * events around every n-th search (every merge when n >= 1), 0 turns timing
+ * off; the host never blocks on the events of a search.  mcac_box_timing
+ * returns accumulated milliseconds and launch counts: [search_ms,
+ * translate_ms, merge_ms, n_search, n_translate, n_merge, filter_ms, solve_ms]. */
 int mcac_box_set_timing(mcac_box *box, int enable);
 int mcac_box_timing(mcac_box *box, double out[8]);
 

@@ -56,8 +56,9 @@ int64_t mcac_sim_get_steps(mcac_sim *sim, mcac_step_rec *out, int64_t max);
 int64_t mcac_sim_n_merges(mcac_sim *sim);
 int64_t mcac_sim_get_merges(mcac_sim *sim, mcac_merge_rec *out, int64_t max);
 /* Named scalars: box_length, time, maxradius, n_agg, n_sph, avg_npp,
- * max_time_step, steps, merges, events, total_volume, query, filtered, pairs,
- * t_host_pick_ms, t_device_ms, init_ms. NaN for unknown names. */
+ * max_time_step, steps, merges, events, total_volume, volume_fraction, query,
+ * filtered, pairs, ties, t_host_pick_ms, t_host_merge_ms, t_device_ms, init_ms,
+ * pick_exact_steps, pick_chunks, pick_rescales. NaN for unknown names. */
 double mcac_sim_scalar(mcac_sim *sim, const char *name);
 /* Host view of the live aggregates in label order (stable ids). */
 int64_t mcac_sim_n_aggs(mcac_sim *sim);
@@ -70,6 +71,16 @@ mcac_box *mcac_sim_box(mcac_sim *sim);
  * rpeqmass, gamma, u_sg, box_volume, aggregate_concentration]. Options this
  * path does not run are accepted here. */
 int mcac_physics_scalars(const char *ini_text, double out[8]);
+
+/* Self-check of the pick table (aggregat_list.cpp:54-66, 109-141): random time
+ * steps and merges on n entries; after every merge each cumulative sum and
+ * eight picks are compared with a full re-sort and a plain sequential sum.
+ * mode 0: mostly distinct steps, 1: a dozen distinct values (most picks land
+ * in a run of equal keys), 2: many pairs of equal steps.  Returns the number
+ * of mismatching checks (0 expected).  Host only.  The second call returns the
+ * picks so far that landed in a run of equal keys. */
+int64_t mcac_pick_table_selftest(int64_t n, int64_t events, int64_t chunk, uint64_t seed, int mode);
+int64_t mcac_pick_table_selftest_ties(void);
 
 #ifdef __cplusplus
 }

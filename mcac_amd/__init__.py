@@ -51,14 +51,15 @@ MERGE_RESULT_DTYPE = np.dtype([
 # Every symbol declared in include/mcac_gpu.h and include/mcac_sim.h.
 GPU_SYMBOLS = (
     "mcac_box_create", "mcac_box_destroy", "mcac_box_upload", "mcac_next_contact", "mcac_translate",
-    "mcac_move", "mcac_merge", "mcac_box_download_spheres", "mcac_box_download_aggs", "mcac_box_members",
+    "mcac_move", "mcac_merge", "mcac_move_enqueue", "mcac_move_result", "mcac_release",
+    "mcac_box_download_spheres", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_swept_sphere_batch", "mcac_version",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
     "mcac_sim_set_trace", "mcac_sim_n_steps", "mcac_sim_get_steps", "mcac_sim_n_merges",
     "mcac_sim_get_merges", "mcac_sim_scalar", "mcac_sim_n_aggs", "mcac_sim_export_aggs", "mcac_sim_box",
-    "mcac_physics_scalars",
+    "mcac_physics_scalars", "mcac_pick_table_selftest", "mcac_pick_table_selftest_ties",
 )
 
 _P = ctypes.c_void_p
@@ -101,6 +102,9 @@ def load_gpu():
         L.mcac_move.argtypes = [_P, _I64, _DP, _D, _D, _P]
         L.mcac_translate.argtypes = [_P, _I64, _DP]
         L.mcac_merge.argtypes = [_P, _P, _P]
+        L.mcac_move_enqueue.argtypes = [_P, _I64, _DP, _D, _D, ctypes.POINTER(ctypes.c_uint64)]
+        L.mcac_move_result.argtypes = [_P, ctypes.c_uint64, _P, ctypes.POINTER(_I)]
+        L.mcac_release.argtypes = [_P]
         L.mcac_box_download_spheres.argtypes = [_P] + [_DP] * 6
         L.mcac_box_download_aggs.argtypes = [_P] + [_DP] * 6 + [_I64P]
         L.mcac_box_members.argtypes = [_P, _I64, _I64P, _I64, _I64P]
@@ -139,6 +143,9 @@ def load_sim():
         L.mcac_sim_box.restype = _P
         L.mcac_sim_box.argtypes = [_P]
         L.mcac_physics_scalars.argtypes = [ctypes.c_char_p, _DP]
+        L.mcac_pick_table_selftest.restype = _I64
+        L.mcac_pick_table_selftest.argtypes = [_I64, _I64, _I64, ctypes.c_uint64, _I]
+        L.mcac_pick_table_selftest_ties.restype = _I64
         _sim = L
     return _sim
 
@@ -228,6 +235,24 @@ class Box:
                "mcac_move" if move else "mcac_next_contact")
         return out[0]
 
+    def move_enqueue(self, agg: int, direction, lpm: float, maxradius: float) -> int:
+        """Queues a gated move; returns its ticket (results are read in ticket order)."""
+        t = ctypes.c_uint64()
+        _check(load_gpu().mcac_move_enqueue(self.h, agg, _dp(_v3(direction)), lpm, maxradius, ctypes.byref(t)),
+               "mcac_move_enqueue")
+        return t.value
+
+    def move_result(self, ticket: int):
+        """(contact record, skipped) of a queued move."""
+        out = np.zeros(1, CONTACT_DTYPE)
+        sk = ctypes.c_int()
+        _check(load_gpu().mcac_move_result(self.h, ticket, out.ctypes.data, ctypes.byref(sk)), "mcac_move_result")
+        return out[0], bool(sk.value)
+
+    def release(self):
+        """Reopens the gate closed by a queued move that found a contact."""
+        _check(load_gpu().mcac_release(self.h), "mcac_release")
+
     def translate(self, agg: int, vec):
         _check(load_gpu().mcac_translate(self.h, agg, _dp(_v3(vec))), "mcac_translate")
 
@@ -260,8 +285,9 @@ class Box:
         _check(load_gpu().mcac_box_members(self.h, agg, _ip(out), out.size, _ip(n)), "mcac_box_members")
         return out
 
-    def set_timing(self, enable: bool):
-        _check(load_gpu().mcac_box_set_timing(self.h, int(enable)), "mcac_box_set_timing")
+    def set_timing(self, stride: int):
+        """HIP events around every `stride`-th search (and every merge); 0 turns timing off."""
+        _check(load_gpu().mcac_box_set_timing(self.h, int(stride)), "mcac_box_set_timing")
 
     def timing(self):
         out = np.zeros(8)

@@ -27,6 +27,7 @@ struct SearchState {
     double refpos[3];          // new CoM - relative CoM (sphere 0 position)
     int newcell;
     int pad;
+    unsigned long long gate;   // seq of a gated step that found a contact (0: open)
 };
 
 // Record published to pinned host memory by the last block of a search.
@@ -35,6 +36,7 @@ struct StepRecord {
     long long ms, os, oagg;
     int contact, status;
     unsigned long long nquery, nfilt, npairs;
+    int skipped, dirty_count;  // step skipped behind a closed gate; dirty list length after it
     unsigned long long seq;    // written last (system-scope release)
 };
 
@@ -94,6 +96,7 @@ struct SearchArgs {
     int translate_in_finish;   // mover small enough for the finishing block
     StepRecord *out;           // pinned host record
     unsigned long long seq;
+    int gated;                 // pipelined step: closes the gate on contact, skipped behind it
 };
 
 struct MergeArgs {

@@ -6,17 +6,21 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <limits>
 #include <memory>
 #include <numeric>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../../include/mcac_sim.h"
 #include "physics.hpp"
+#include "pick_table.hpp"
 
 namespace mcac {
 namespace {
@@ -49,113 +53,53 @@ inline bool contact(const Vec3 &a, double ra, const Vec3 &b, double rb, double L
     return distance_2(a, b, L) - s * s <= kContactEpsilon;
 }
 
-// Time-step weighted pick table (aggregat_list.cpp:54-66, 109-141).
-// Live aggregates are kept ordered by time step, descending, which is the
-// reference's ascending order of tp_t = max/time_step; the cumulative sums
-// are the same sequential sums of the same values, so they are bit-identical
-// to the reference's.  Only inside a run of equal tp_t values does the
-// libstdc++ std::sort permutation decide which aggregate a position holds:
-// a pick landing in such a run is answered from that permutation, rebuilt
-// from the label-ordered keys exactly as the reference builds it.
-class PickTable {
+// Live aggregate ids: flags plus a Fenwick tree of counts, so that removal and
+// the label of an id (its rank among live ids, aggregat_list_storage.cpp:37-44)
+// cost O(log N); the label-ordered list is built only where it is read.
+class LiveSet {
   public:
-    void build(const std::vector<int> &ids, const std::vector<double> &ts) {
-        id_ = ids;
-        std::sort(id_.begin(), id_.end(), [&ts](int a, int b) {
-            return ts[static_cast<size_t>(a)] > ts[static_cast<size_t>(b)] ||
-                   (ts[static_cast<size_t>(a)] == ts[static_cast<size_t>(b)] && a < b);
-        });
-        ts_.resize(id_.size());
-        for (size_t i = 0; i < id_.size(); ++i) ts_[i] = ts[static_cast<size_t>(id_[i])];
-        factor_ = ts_.empty() ? 0.0 : ts_.front();
-        cum_.resize(id_.size());
-        recompute(0);
-        ref_valid_ = false;
-    }
-    double factor() const { return factor_; }
-    double total() const { return cum_.back(); }
-    size_t size() const { return id_.size(); }
-
-    // keeper: old -> new time step; removed: leaves the table.
-    void replace(int keep, double keep_old, double keep_new, int rem, double rem_ts) {
-        size_t a = erase(keep_old, keep);
-        size_t b = erase(rem_ts, rem);
-        size_t c = insert(keep_new, keep);
-        size_t p = std::min(a, std::min(b, c));
-        cum_.resize(id_.size());
-        double f = ts_.front();
-        if (f != factor_) {
-            factor_ = f;
-            p = 0;
+    void reset(size_t n) {
+        alive_.assign(n, 1);
+        tree_.assign(n + 1, 0);
+        for (size_t i = 1; i <= n; ++i) {
+            tree_[i] += 1;
+            size_t j = i + (i & (~i + 1));
+            if (j <= n) tree_[j] += tree_[i];
         }
-        recompute(p);
-        ref_valid_ = false;
+        count_ = n;
+        cache_valid_ = false;
     }
-
-    // pick_random (aggregat_list.cpp:59-66) for the draw u; live holds the ids in label order.
-    int pick(double u, const std::vector<int> &live, const std::vector<double> &ts, bool *tie_out) {
-        double val = u * cum_.back();
-        size_t n = static_cast<size_t>(std::lower_bound(cum_.begin(), cum_.end(), val) - cum_.begin());
-        double tp = factor_ / ts_[n];
-        bool tie = (n > 0 && factor_ / ts_[n - 1] == tp) || (n + 1 < ts_.size() && factor_ / ts_[n + 1] == tp);
-        if (tie_out) *tie_out = tie;
-        if (!tie) return id_[n];
-        if (!ref_valid_) {
-            std::vector<double> tp_t(live.size());
-            for (size_t i = 0; i < live.size(); ++i) tp_t[i] = factor_ / ts[static_cast<size_t>(live[i])];
-            ref_idx_.resize(live.size());
-            std::iota(ref_idx_.begin(), ref_idx_.end(), 0);
-            std::sort(ref_idx_.begin(), ref_idx_.end(), [&tp_t](size_t i1, size_t i2) { return tp_t[i1] < tp_t[i2]; });
-            ref_valid_ = true;
+    size_t size() const { return count_; }
+    bool alive(size_t id) const { return alive_[id] != 0; }
+    void erase(size_t id) {
+        if (!alive_[id]) return;
+        alive_[id] = 0;
+        for (size_t i = id + 1; i < tree_.size(); i += i & (~i + 1)) tree_[i] -= 1;
+        --count_;
+        cache_valid_ = false;
+    }
+    int64_t rank(size_t id) const {  // live ids before id
+        int64_t r = 0;
+        for (size_t i = id; i > 0; i -= i & (~i + 1)) r += tree_[i];
+        return r;
+    }
+    const std::vector<int> &ids() {
+        if (!cache_valid_) {
+            cache_.clear();
+            cache_.reserve(count_);
+            for (size_t i = 0; i < alive_.size(); ++i)
+                if (alive_[i]) cache_.push_back(static_cast<int>(i));
+            cache_valid_ = true;
         }
-        return live[ref_idx_[n]];
+        return cache_;
     }
 
   private:
-    std::vector<double> ts_;
-    std::vector<int> id_;
-    std::vector<double> cum_;
-    double factor_ = 0;
-    std::vector<size_t> ref_idx_;
-    bool ref_valid_ = false;
-
-    size_t locate(double t, int id) const {
-        // order: ts descending, then id ascending
-        size_t lo = 0, hi = ts_.size();
-        while (lo < hi) {
-            size_t mid = (lo + hi) / 2;
-            bool before = ts_[mid] > t || (ts_[mid] == t && id_[mid] < id);
-            if (before) lo = mid + 1;
-            else hi = mid;
-        }
-        return lo;
-    }
-    size_t erase(double t, int id) {
-        size_t p = locate(t, id);
-        if (p >= id_.size() || id_[p] != id) throw std::logic_error("pick table out of sync");
-        ts_.erase(ts_.begin() + static_cast<long>(p));
-        id_.erase(id_.begin() + static_cast<long>(p));
-        return p;
-    }
-    size_t insert(double t, int id) {
-        size_t p = locate(t, id);
-        ts_.insert(ts_.begin() + static_cast<long>(p), t);
-        id_.insert(id_.begin() + static_cast<long>(p), id);
-        return p;
-    }
-    void recompute(size_t from) {
-        const size_t n = ts_.size();
-        if (n == 0) return;
-        if (from == 0) {
-            cum_[0] = factor_ / ts_[0];
-            from = 1;
-        }
-        double acc = cum_[from - 1];
-        for (size_t i = from; i < n; ++i) {
-            acc = acc + factor_ / ts_[i];
-            cum_[i] = acc;
-        }
-    }
+    std::vector<char> alive_;
+    std::vector<int64_t> tree_;
+    size_t count_ = 0;
+    std::vector<int> cache_;
+    bool cache_valid_ = false;
 };
 
 }  // namespace
@@ -173,7 +117,7 @@ struct Simulation {
     std::vector<double> ts, lpm, f_agg, d_m, dp, rg, rmax, vol, surf, ptime, dg_over_dp, overlapping, coord;
     std::vector<Vec3> com, rcm;
     std::vector<int> nsph;
-    std::vector<int> live;  // live ids, ascending (label order)
+    LiveSet live;  // live ids (label order = ascending id)
     double maxradius = 0, max_time_step = 0, avg_npp = 1;
     PickTable pick;
     bool event = true;
@@ -184,7 +128,7 @@ struct Simulation {
     std::vector<mcac_step_rec> steps;
     std::vector<mcac_merge_rec> merge_recs;
     double q_query = 0, q_filtered = 0, q_pairs = 0, n_ties = 0;
-    double t_pick = 0, t_device = 0, t_init = 0;
+    double t_pick = 0, t_merge_host = 0, t_device = 0, t_init = 0;
     bool initialized = false;
     bool final_written = false;
 
@@ -340,8 +284,7 @@ struct Simulation {
                 surf[i] = 0.0 + ssurf[i];
             }
         }
-        live.resize(N);
-        std::iota(live.begin(), live.end(), 0);
+        live.reset(N);
         // device state
         mcac_box_params bp{};
         bp.box_length = pm.box_length;
@@ -361,6 +304,7 @@ struct Simulation {
                              ssurf.data(), zero.data(), zero.data(), zero.data(), static_cast<int64_t>(N), pos3.data(),
                              rcm3.data(), rmax.data(), vol.data(), surf.data(), cnt.data(), mem.data());
         if (rc) return rc;
+        pick.build(live.ids(), ts);  // first sort_time_steps (calcul.cpp:99-101)
         event = true;
         initialized = true;
         t_init = ms_since(t0);
@@ -373,7 +317,7 @@ struct Simulation {
     }
 
     int64_t label_of(int id) const {
-        return static_cast<int64_t>(std::lower_bound(live.begin(), live.end(), id) - live.begin());
+        return live.rank(static_cast<size_t>(id));
     }
 
     // calcul.cpp:29-43
@@ -397,8 +341,8 @@ struct Simulation {
     void refresh_pm() {
         if (!pm_stale) return;
         double tv = 0.0, tsf = 0.0;
-        for (int id : live) tv += vol[static_cast<size_t>(id)];
-        for (int id : live) tsf += surf[static_cast<size_t>(id)];
+        for (int id : live.ids()) tv += vol[static_cast<size_t>(id)];
+        for (int id : live.ids()) tsf += surf[static_cast<size_t>(id)];
         pm.update(live.size(), N, tv, tsf);
         pm_stale = false;
     }
@@ -419,28 +363,100 @@ struct Simulation {
         return false;
     }
 
-    // One iteration of calcul.cpp:66-281 (collisions on, no surface reactions,
-    // nucleation, potentials or flame coupling).
-    int step() {
+    // The MC loop of calcul.cpp:66-281 (collisions on; no surface reactions,
+    // nucleation, potentials or flame coupling), pipelined: up to `depth` steps
+    // are picked and enqueued ahead assuming no merge in between; a step that
+    // finds a contact closes the device gate, the steps behind it do nothing,
+    // and they are issued again with the same draws once the merge is done.
+    struct Draw {
+        double u;
+        Vec3 dir;
+    };
+    struct Pending {
+        uint64_t ticket;
+        int num;
+        bool tie;
+        double full;
+        Draw draw;
+    };
+    std::deque<Draw> draws;      // taken from the stream, not yet consumed by a step
+    std::deque<Pending> flight;  // enqueued on the device, in order
+    size_t depth = 3;
+
+    // the k-th draw after the last consumed step (calcul.cpp:102, tools.cpp:83-84)
+    const Draw &draw_at(size_t k) {
+        while (draws.size() <= k) {
+            Draw d;
+            d.u = rng.uniform();
+            d.dir = rng.direction();
+            draws.push_back(d);
+        }
+        return draws[k];
+    }
+
+    bool can_issue(int64_t consumed, int64_t max_steps) const {
+        if (consumed + static_cast<int64_t>(flight.size()) >= max_steps) return false;
+        if (flight.size() >= depth) return false;
+        if (pm.n_iter_without_event_limit > 0 &&
+            pm.n_iter_without_event + flight.size() >= static_cast<size_t>(pm.n_iter_without_event_limit))
+            return flight.empty();
+        if (pm.physical_time_limit > 0) return flight.empty();  // time-driven stop: no look-ahead
+        return true;
+    }
+
+    int issue() {
+        auto t0 = Clock::now();
+        const Draw &d = draw_at(flight.size());
+        size_t tie_pos = 0;
+        int num = pick.pick_fast(d.u, &tie_pos);  // aggregat_list.cpp:59-66
+        bool tie = false;
+        if (num < 0) {
+            tie = true;
+            num = pick.resolve_tie(tie_pos, live.ids(), ts);
+        }
+        const double full = lpm[static_cast<size_t>(num)];
+        t_pick += ms_since(t0);
+        uint64_t ticket = 0;
+        int rc = mcac_move_enqueue(box, num, d.dir.data(), full, maxradius, &ticket);
+        if (rc) return rc;
+        flight.push_back({ticket, num, tie, full, d});
+        return 0;
+    }
+
+    // Device state after a contact: the steps behind it were skipped; forget them
+    // (their draws stay queued) and reopen the gate after the merge.
+    int drain_skipped() {
+        while (!flight.empty()) {
+            mcac_contact c{};
+            int skipped = 0;
+            int rc = mcac_move_result(box, flight.front().ticket, &c, &skipped);
+            if (rc) return rc;
+            if (!skipped) return MCAC_UNKNOWN_ERROR;  // a step ran past a contact
+            flight.pop_front();
+        }
+        return 0;
+    }
+
+    int consume() {
         if (pm.time_to_write(total_events)) save_advancement();
         if (event && pm.with_domain_duplication && live.size() <= pm.n_monomeres / 8) return MCAC_INPUT_ERROR;
-        auto tp0 = Clock::now();
-        if (event && step_count == 0) pick.build(live, ts);
-        const double max = max_time_step;
-        bool tie = false;
-        const int num = pick.pick(rng.uniform(), live, ts, &tie);
-        n_ties += tie ? 1 : 0;
-        const int64_t num_label = trace_steps ? label_of(num) : -1;
-        double deltatemps = max / pick.total();
+        const Pending P = flight.front();
+        flight.pop_front();
+        const int num = P.num;
         const size_t U = static_cast<size_t>(num);
-        const double full = lpm[U];
-        const Vec3 dir = rng.direction();
-        t_pick += ms_since(tp0);
+        const double full = P.full;
+        const Vec3 dir = P.draw.dir;
+        n_ties += P.tie ? 1 : 0;
+        const int64_t num_label = trace_steps ? label_of(num) : -1;
+        double deltatemps = max_time_step / pick.total();  // aggregat_list.cpp:54-58
         mcac_contact c{};
+        int skipped = 0;
         auto td0 = Clock::now();
-        int rc = mcac_move(box, num, dir.data(), full, maxradius, &c);
+        int rc = mcac_move_result(box, P.ticket, &c, &skipped);
         t_device += ms_since(td0);
         if (rc) return rc;
+        if (skipped) return MCAC_UNKNOWN_ERROR;
+        draws.pop_front();
         q_query += static_cast<double>(c.n_query);
         q_filtered += static_cast<double>(c.n_filtered);
         q_pairs += static_cast<double>(c.n_pairs);
@@ -452,6 +468,8 @@ struct Simulation {
         pm.time = pm.time + deltatemps;
         bool merged = false;
         if (contact_flag) {
+            rc = drain_skipped();
+            if (rc) return rc;
             mcac_merge_result mr{};
             auto tm0 = Clock::now();
             rc = mcac_merge(box, &c, &mr);
@@ -481,11 +499,11 @@ struct Simulation {
                 lpm[K] = ph.lpm;
                 dg_over_dp[K] = ph.dg_over_dp;
                 if (rmax[K] > maxradius) maxradius = rmax[K];  // aggregat.cpp:277-281
-                live.erase(std::lower_bound(live.begin(), live.end(), static_cast<int>(R)));
+                live.erase(R);
                 ptime[K] = newtime;
                 pick.replace(static_cast<int>(K), keep_old_ts, ts[K], static_cast<int>(R), rem_ts);
                 ++merges;
-                t_pick += ms_since(th0);
+                t_merge_host += ms_since(th0);
                 if (trace_merges) {
                     mcac_merge_rec m{};
                     m.step = step_count;
@@ -511,6 +529,8 @@ struct Simulation {
                     merge_recs.push_back(m);
                 }
             }
+            rc = mcac_release(box);
+            if (rc) return rc;
         }
         event = merged;
         if (event) {
@@ -543,6 +563,26 @@ struct Simulation {
             steps.push_back(r);
         }
         step_count++;
+        return 0;
+    }
+
+    // Runs at most max_steps steps; returns the number completed in *done.
+    int run(int64_t max_steps, int64_t *done) {
+        int64_t n = 0;
+        while (n < max_steps) {
+            if (flight.empty() && finished()) break;
+            while (!finished() && can_issue(n, max_steps)) {
+                int rc = issue();
+                if (rc) return rc;
+            }
+            if (flight.empty()) break;
+            int rc = consume();
+            if (rc) return rc;
+            ++n;
+            // a stop met with steps still in flight (STOPCODE) lets those
+            // steps, which already ran on the device, complete first
+        }
+        *done = n;
         return 0;
     }
 };
@@ -593,11 +633,7 @@ int mcac_sim_run(mcac_sim *sim, int64_t max_steps, int64_t *done) {
     int64_t n = 0;
     int rc = 0;
     try {
-        while (n < max_steps && !sim->s.finished()) {
-            rc = sim->s.step();
-            if (rc) break;
-            ++n;
-        }
+        rc = sim->s.run(max_steps, &n);
         if (rc == 0 && sim->s.finished() && !sim->s.final_written) {
             sim->s.save_advancement();  // calcul.cpp:282
             sim->s.final_written = true;
@@ -640,6 +676,9 @@ double mcac_sim_scalar(mcac_sim *sim, const char *name) {
     if (n == "time") return s.pm.time;
     if (n == "maxradius") return s.maxradius;
     if (n == "n_agg") return static_cast<double>(s.live.size());
+    if (n == "pick_exact_steps") return static_cast<double>(s.pick.exact_steps());
+    if (n == "pick_chunks") return static_cast<double>(s.pick.chunks());
+    if (n == "pick_rescales") return static_cast<double>(s.pick.rescales());
     if (n == "n_sph") return static_cast<double>(s.N);
     if (n == "avg_npp") return s.avg_npp;
     if (n == "max_time_step") return s.max_time_step;
@@ -651,11 +690,12 @@ double mcac_sim_scalar(mcac_sim *sim, const char *name) {
     if (n == "pairs") return s.q_pairs;
     if (n == "ties") return s.n_ties;
     if (n == "t_host_pick_ms") return s.t_pick;
+    if (n == "t_host_merge_ms") return s.t_merge_host;
     if (n == "t_device_ms") return s.t_device;
     if (n == "init_ms") return s.t_init;
     if (n == "total_volume") {
         double t = 0.0;
-        for (int id : s.live) t += s.vol[static_cast<size_t>(id)];
+        for (int id : s.live.ids()) t += s.vol[static_cast<size_t>(id)];
         return t;
     }
     if (n == "volume_fraction") {
@@ -671,8 +711,9 @@ int mcac_sim_export_aggs(mcac_sim *sim, int64_t *uid, double *lpm, double *time_
                          double *volume, double *rg, double *rmax, int64_t *n) {
     if (!sim) return MCAC_E_ARGUMENT;
     auto &s = sim->s;
-    for (size_t l = 0; l < s.live.size(); ++l) {
-        size_t id = static_cast<size_t>(s.live[l]);
+    const std::vector<int> &ids = s.live.ids();
+    for (size_t l = 0; l < ids.size(); ++l) {
+        size_t id = static_cast<size_t>(ids[l]);
         if (uid) uid[l] = static_cast<int64_t>(id);
         if (lpm) lpm[l] = s.lpm[id];
         if (time_step) time_step[l] = s.ts[id];
@@ -686,6 +727,76 @@ int mcac_sim_export_aggs(mcac_sim *sim, int64_t *uid, double *lpm, double *time_
 }
 
 mcac_box *mcac_sim_box(mcac_sim *sim) { return sim ? sim->s.box : nullptr; }
+
+static int64_t selftest_ties = 0;
+
+int64_t mcac_pick_table_selftest_ties() { return selftest_ties; }
+
+int64_t mcac_pick_table_selftest(int64_t n, int64_t events, int64_t chunk, uint64_t seed, int mode) {
+    // random time steps with repeated values, random merges; every cumulative
+    // sum and every pick compared with a plain re-sort and sequential sum
+    std::mt19937_64 rng(seed);
+    std::lognormal_distribution<double> logn(0.0, 0.5);
+    std::vector<double> ts(static_cast<size_t>(n));
+    for (auto &t : ts) {
+        if (mode == 1) t = 1e-7 * static_cast<double>(1 + rng() % 12);  // a few large runs of equal keys
+        else if (mode == 2) t = 1e-7 * (1.0 + static_cast<double>(rng() % static_cast<uint64_t>(n)) / n);  // pairs
+        else t = (rng() % 7 == 0) ? 1.0 : logn(rng) * 1e-7;  // some exact ties
+    }
+    std::vector<int> live(static_cast<size_t>(n));
+    std::iota(live.begin(), live.end(), 0);
+    mcac::PickTable table(static_cast<size_t>(chunk));
+    table.build(live, ts);
+    int64_t bad = 0;
+    auto check = [&]() {
+        std::vector<int> order(live);
+        std::sort(order.begin(), order.end(), [&ts](int a, int b) {
+            return ts[static_cast<size_t>(a)] > ts[static_cast<size_t>(b)] ||
+                   (ts[static_cast<size_t>(a)] == ts[static_cast<size_t>(b)] && a < b);
+        });
+        const double f = ts[static_cast<size_t>(order[0])];
+        std::vector<double> cum(order.size());
+        for (size_t i = 0; i < order.size(); ++i)
+            cum[i] = (i == 0 ? 0.0 : cum[i - 1]) + f / ts[static_cast<size_t>(order[i])];
+        cum[0] = f / ts[static_cast<size_t>(order[0])];
+        std::vector<double> dts, dcum;
+        std::vector<int> did;
+        table.dump(dts, did, dcum);
+        if (table.factor() != f || table.total() != cum.back() || dcum.size() != cum.size()) return ++bad;
+        for (size_t i = 0; i < cum.size(); ++i)
+            if (dcum[i] != cum[i] || did[i] != order[i]) return ++bad;
+        // reference pick on the label-ordered keys
+        std::vector<double> tp_t(live.size());
+        for (size_t i = 0; i < live.size(); ++i) tp_t[i] = f / ts[static_cast<size_t>(live[i])];
+        std::vector<size_t> idx(live.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&tp_t](size_t i1, size_t i2) { return tp_t[i1] < tp_t[i2]; });
+        for (int q = 0; q < 8; ++q) {
+            double u = static_cast<double>(rng() % 2147483648ull) / 2147483647.0;
+            double val = u * cum.back();
+            size_t k = static_cast<size_t>(std::lower_bound(cum.begin(), cum.end(), val) - cum.begin());
+            bool tie = false;
+            int got = table.pick(u, live, ts, &tie);
+            selftest_ties += tie ? 1 : 0;
+            if (got != live[idx[k]]) return ++bad;
+        }
+        return bad;
+    };
+    check();
+    for (int64_t e = 0; e < events && live.size() > 2; ++e) {
+        size_t a = rng() % live.size(), b = rng() % live.size();
+        if (a == b) continue;
+        int keep = std::min(live[a], live[b]), rem = std::max(live[a], live[b]);
+        double old_k = ts[static_cast<size_t>(keep)], old_r = ts[static_cast<size_t>(rem)];
+        double nt = (rng() % 5 == 0 || mode == 1) ? old_k : old_k * (1.0 + logn(rng) * 0.3);
+        if (rng() % 50 == 0) nt = ts[static_cast<size_t>(live[rng() % live.size()])];  // equal to another key
+        ts[static_cast<size_t>(keep)] = nt;
+        live.erase(std::find(live.begin(), live.end(), rem));
+        table.replace(keep, old_k, nt, rem, old_r);
+        check();
+    }
+    return bad;
+}
 
 int mcac_physics_scalars(const char *ini_text, double out[8]) {
     if (!ini_text || !out) return MCAC_E_ARGUMENT;

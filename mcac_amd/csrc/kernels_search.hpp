@@ -29,7 +29,15 @@ __device__ __forceinline__ void append_candidate(const DevBox &b, const SearchAr
     }
 }
 
+// A gated step enqueued behind a step that found a contact does nothing.
+__device__ __forceinline__ bool behind_gate(const DevBox &b, const SearchArgs &a) {
+    if (!a.gated) return false;
+    const unsigned long long g = b.ss->gate;
+    return g != 0ull && g < a.seq;
+}
+
 __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, SearchArgs a) {
+    if (behind_gate(b, a)) return;
     __shared__ int win[6];
     __shared__ double q[4];
     __shared__ int pre[kMaxVisit + 1];
@@ -126,6 +134,16 @@ __device__ __forceinline__ long long chunk_count(long long pairs, long long chun
 }
 
 __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
+    if (behind_gate(b, a)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.out->skipped = 1;
+            a.out->contact = 0;
+            a.out->status = 0;
+            __threadfence_system();
+            __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        return;
+    }
     __shared__ long long cpre[kMaxPlan + 1];
     __shared__ long long s_chunk;
     __shared__ bool last;
@@ -300,6 +318,9 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
             b.acell[a.src] = s->newcell;
             mark_dirty(b, a.src);
         }
+        if (a.gated && contact) s->gate = a.seq;
+        o->skipped = 0;
+        o->dirty_count = static_cast<int>(s->dirty_count);
         s->nfilt = 0;
         s->nquery = 0;
         s->overflow = 0;
@@ -321,9 +342,12 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     }
 }
 
+__global__ void k_release(DevBox b) { b.ss->gate = 0ull; }
+
 // a13 for a large mover: sphere positions from the refpos of the last search.
-__global__ __launch_bounds__(kBlock) void k_translate(DevBox b, int src) {
+__global__ __launch_bounds__(kBlock) void k_translate(DevBox b, int src, int gated, unsigned long long seq) {
     const SearchState *s = b.ss;
+    if (gated && s->gate != 0ull && s->gate < seq) return;
     const int n = b.acount[src];
     const int st = b.astart[src];
     const double r0 = s->refpos[0], r1 = s->refpos[1], r2 = s->refpos[2];

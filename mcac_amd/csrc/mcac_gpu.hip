@@ -38,17 +38,24 @@ struct mcac_box {
     // host mirrors for allocation, launch sizing and the dirty list
     std::vector<int> count, start, cap;
     std::vector<int> alive_ids;  // ascending, may hold dead ids until rebuilt
-    std::vector<unsigned char> dirty;
-    int64_t dirty_count = 0;
+    int64_t dirty_count = 0;         // upper bound of the device dirty list
     int64_t dead_in_list = 0;
     int64_t pool_used = 0;
     // pinned records (host pointer, device pointer)
-    StepRecord *h_step = nullptr, *d_step = nullptr;
+    static constexpr int kRing = 8;  // steps in flight
+    StepRecord *h_step = nullptr, *d_step = nullptr;  // kRing records
+    int inflight = 0;
+    unsigned long long oldest = 1;   // oldest ticket not yet consumed
+    int slot_apply[kRing] = {0}, slot_translate[kRing] = {0};
+    int64_t slot_agg[kRing] = {0};
     MergeRecord *h_merge = nullptr, *d_merge = nullptr;
-    unsigned long long seq = 0;
+    unsigned long long seq = 0;        // step tickets (searches)
+    unsigned long long merge_seq = 0;  // merge records
     std::vector<void *> allocs;
-    bool timing = false;
+    int timing = 0;                  // 0 off; n >= 1: every n-th search is timed
+    int slot_timed[kRing] = {0};     // 1 timed, 2 timed and consumed, events not read yet
     hipEvent_t ev[4]{};
+    hipEvent_t sev[kRing][4]{};      // per-slot events of pipelined steps
     double t_search = 0, t_translate = 0, t_merge = 0, t_filter = 0, t_solve = 0;
     double n_search = 0, n_translate = 0, n_merge = 0;
     int64_t n_rebuilds = 0;
@@ -129,19 +136,12 @@ int rebuild_cells(mcac_box *b) {
     hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, b->stream, b->d, b->n_cells);
     hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(kBlock), 0, b->stream, b->d, n_alive);
     HIP_TRY(hipGetLastError());
-    for (int id : b->alive_ids) b->dirty[static_cast<size_t>(id)] = 0;
     b->dirty_count = 0;
     b->n_built = n_alive;
     b->n_rebuilds++;
     return 0;
 }
 
-void host_mark_dirty(mcac_box *b, int64_t id) {
-    if (!b->dirty[static_cast<size_t>(id)]) {
-        b->dirty[static_cast<size_t>(id)] = 1;
-        b->dirty_count++;
-    }
-}
 
 int64_t rebuild_threshold(const mcac_box *b) {
     int64_t n = static_cast<int64_t>(b->alive_ids.size()) - b->dead_in_list;
@@ -193,13 +193,19 @@ int compact_pool(mcac_box *b, int64_t need) {
     return 0;
 }
 
-int search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double maxradius, int apply, mcac_contact *out) {
+int harvest_timing(mcac_box *b, int slot);
+
+int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double maxradius, int apply, int gated,
+                   unsigned long long *ticket) {
     if (agg < 0 || agg >= b->n_aggs || b->count[static_cast<size_t>(agg)] <= 0) return MCAC_E_ARGUMENT;
-    if (b->dirty_count >= rebuild_threshold(b)) {
+    if (b->inflight >= mcac_box::kRing) return MCAC_E_CAPACITY;
+    if (b->dirty_count + 2 >= rebuild_threshold(b)) {
         int rc = rebuild_cells(b);
         if (rc) return rc;
     }
     const int n_mov = b->count[static_cast<size_t>(agg)];
+    const unsigned long long seq = ++b->seq;
+    const int slot = static_cast<int>(seq % mcac_box::kRing);
     SearchArgs a{};
     a.src = static_cast<int>(agg);
     a.n_div = static_cast<int>(b->p.n_div);
@@ -214,38 +220,69 @@ int search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double max
     a.L = b->p.box_length;
     a.apply = apply;
     a.translate_in_finish = n_mov <= 4 * kBlock;
-    a.out = b->d_step;
-    a.seq = ++b->seq;
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    a.out = b->d_step + slot;
+    a.seq = seq;
+    a.gated = gated;
+    if (b->slot_timed[slot] == 2) {
+        int rc = harvest_timing(b, slot);
+        if (rc) return rc;
+    }
+    __atomic_store_n(&b->h_step[slot].seq, 0ull, __ATOMIC_RELEASE);
+    hipEvent_t *ev = b->sev[slot];
+    const bool timed = b->timing > 0 && seq % static_cast<unsigned long long>(b->timing) == 0;
+    if (timed) HIP_TRY(hipEventRecord(ev[0], b->stream));
     hipLaunchKernelGGL(k_filter, dim3(kFilterBlocks), dim3(kBlock), 0, b->stream, b->d, a);
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[1], b->stream));
     hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, a);
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[2], b->stream));
     if (apply && !a.translate_in_finish) {
         const int tb = std::min(1024, (n_mov + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, a.src);
+        hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, a.src, gated, seq);
     }
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev[3], b->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[3], b->stream));
+    b->slot_timed[slot] = timed ? 1 : 0;
     HIP_TRY(hipGetLastError());
-    int rc = wait_seq(b, &b->h_step->seq, a.seq);
-    if (rc) return rc;
-    if (apply) host_mark_dirty(b, agg);
-    if (b->timing) {
-        HIP_TRY(hipEventSynchronize(b->ev[3]));
-        float t01 = 0, t12 = 0, t23 = 0;
-        HIP_TRY(hipEventElapsedTime(&t01, b->ev[0], b->ev[1]));
-        HIP_TRY(hipEventElapsedTime(&t12, b->ev[1], b->ev[2]));
-        HIP_TRY(hipEventElapsedTime(&t23, b->ev[2], b->ev[3]));
-        b->t_filter += t01;
-        b->t_solve += t12;
-        b->t_search += t01 + t12;
-        b->n_search += 1;
-        if (apply && !a.translate_in_finish) {
-            b->t_translate += t23;
-            b->n_translate += 1;
-        }
+    if (apply) b->dirty_count += 1;
+    b->slot_apply[slot] = apply;
+    b->slot_agg[slot] = agg;
+    b->slot_translate[slot] = apply && !a.translate_in_finish;
+    b->inflight += 1;
+    *ticket = seq;
+    return 0;
+}
+
+// Adds the event times of a consumed, timed step to the totals.
+int harvest_timing(mcac_box *b, int slot) {
+    hipEvent_t *ev = b->sev[slot];
+    HIP_TRY(hipEventSynchronize(ev[3]));
+    float t01 = 0, t12 = 0, t23 = 0;
+    HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    b->t_filter += t01;
+    b->t_solve += t12;
+    b->t_search += t01 + t12;
+    b->n_search += 1;
+    if (b->slot_translate[slot]) {
+        b->t_translate += t23;
+        b->n_translate += 1;
     }
-    const StepRecord &s = *b->h_step;
+    b->slot_timed[slot] = 0;
+    return 0;
+}
+
+int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contact *out, int *skipped) {
+    if (ticket != b->oldest || b->inflight <= 0) return MCAC_E_ARGUMENT;  // results are read in order
+    const int slot = static_cast<int>(ticket % mcac_box::kRing);
+    int rc = wait_seq(b, &b->h_step[slot].seq, ticket);
+    if (rc) return rc;
+    b->inflight -= 1;
+    b->oldest = ticket + 1;
+    const StepRecord &s = b->h_step[slot];
+    // events are read when the slot is reused or the totals are asked for,
+    // so that the host never blocks on them here
+    if (b->slot_timed[slot] == 1) b->slot_timed[slot] = s.skipped ? 0 : 2;
+    if (skipped) *skipped = s.skipped;
     if (out) {
         out->distance = s.dist;
         out->move = s.move;
@@ -259,7 +296,16 @@ int search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double max
         out->n_filtered = static_cast<int64_t>(s.nfilt);
         out->n_pairs = static_cast<int64_t>(s.npairs);
     }
-    return s.status;
+    if (!s.skipped && s.dirty_count >= 0) b->dirty_count = std::max<int64_t>(s.dirty_count, 0) + b->inflight;
+    return s.skipped ? 0 : s.status;
+}
+
+int search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double maxradius, int apply, mcac_contact *out) {
+    if (b->inflight != 0) return MCAC_E_ARGUMENT;  // pipelined steps pending
+    unsigned long long ticket = 0;
+    int rc = enqueue_search(b, agg, dir, lpm, maxradius, apply, 0, &ticket);
+    if (rc) return rc;
+    return wait_search(b, ticket, agg, out, nullptr);
 }
 
 }  // namespace
@@ -341,17 +387,20 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.row_cnt, Ns); ALLOC(d.row_j, Ns * kOverlapSlots); ALLOC(d.row_c, Ns * kOverlapSlots);
     ALLOC(d.ss, 1); ALLOC(d.ms, 1);
 #undef ALLOC
-    if (hipHostMalloc(reinterpret_cast<void **>(&b->h_step), sizeof(StepRecord), hipHostMallocCoherent) != hipSuccess ||
+    if (hipHostMalloc(reinterpret_cast<void **>(&b->h_step), mcac_box::kRing * sizeof(StepRecord),
+                      hipHostMallocCoherent) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void **>(&b->h_merge), sizeof(MergeRecord), hipHostMallocCoherent) != hipSuccess)
         return fail(MCAC_E_DEVICE);
-    std::memset(b->h_step, 0, sizeof(StepRecord));
+    std::memset(b->h_step, 0, mcac_box::kRing * sizeof(StepRecord));
     std::memset(b->h_merge, 0, sizeof(MergeRecord));
     if (hipHostGetDevicePointer(reinterpret_cast<void **>(&b->d_step), b->h_step, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void **>(&b->d_merge), b->h_merge, 0) != hipSuccess)
         return fail(MCAC_E_DEVICE);
     for (auto &e : b->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(MCAC_E_DEVICE);
-    b->dirty.assign(static_cast<size_t>(Na), 0);
+    for (auto &row : b->sev)
+        for (auto &e : row)
+            if (hipEventCreate(&e) != hipSuccess) return fail(MCAC_E_DEVICE);
     *out = b;
     return 0;
 }
@@ -364,6 +413,9 @@ int mcac_box_destroy(mcac_box *b) {
     if (b->h_merge) (void)hipHostFree(b->h_merge);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &row : b->sev)
+        for (auto &e : row)
+            if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return 0;
@@ -457,7 +509,8 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
     b->alive_ids = alive;
     b->dead_in_list = 0;
     b->pool_used = static_cast<int64_t>(pool.size());
-    std::fill(b->dirty.begin(), b->dirty.end(), 0);
+    b->inflight = 0;
+    b->oldest = b->seq + 1;
     rc = rebuild_cells(b);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -476,7 +529,8 @@ int mcac_move(mcac_box *b, int64_t agg, const double dir[3], double lpm, double 
 }
 
 int mcac_translate(mcac_box *b, int64_t agg, const double vec[3]) {
-    if (!b || !vec || agg < 0 || agg >= b->n_aggs || b->count[static_cast<size_t>(agg)] <= 0) return MCAC_E_ARGUMENT;
+    if (!b || !vec || agg < 0 || agg >= b->n_aggs || b->count[static_cast<size_t>(agg)] <= 0 || b->inflight)
+        return MCAC_E_ARGUMENT;
     if (b->dirty_count >= rebuild_threshold(b)) {
         int rc = rebuild_cells(b);
         if (rc) return rc;
@@ -485,10 +539,10 @@ int mcac_translate(mcac_box *b, int64_t agg, const double vec[3]) {
                        static_cast<int>(b->p.n_div), b->p.box_length);
     const int n = b->count[static_cast<size_t>(agg)];
     const int tb = std::min(1024, std::max(1, (n + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, static_cast<int>(agg));
+    hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, static_cast<int>(agg), 0, 0ull);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(b->stream));
-    host_mark_dirty(b, agg);
+    b->dirty_count += 1;
     return 0;
 }
 
@@ -535,7 +589,7 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     a.n_div = static_cast<int>(b->p.n_div);
     a.L = b->p.box_length;
     a.out = b->d_merge;
-    a.seq = ++b->seq;
+    a.seq = ++b->merge_seq;
     const int n = nk + nr;
     if (b->timing) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (n <= kSmallMerge) {
@@ -576,9 +630,28 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     b->count[K] = n;
     b->count[R] = 0;
     b->dead_in_list += 1;
-    host_mark_dirty(b, keep);
-    host_mark_dirty(b, rem);
+    b->dirty_count += 2;
     return out->status;
+}
+
+int mcac_move_enqueue(mcac_box *b, int64_t agg, const double dir[3], double lpm, double maxradius, uint64_t *ticket) {
+    if (!b || !dir || !ticket) return MCAC_E_ARGUMENT;
+    unsigned long long t = 0;
+    int rc = enqueue_search(b, agg, dir, lpm, maxradius, 1, 1, &t);
+    *ticket = t;
+    return rc;
+}
+
+int mcac_move_result(mcac_box *b, uint64_t ticket, mcac_contact *out, int *skipped) {
+    if (!b) return MCAC_E_ARGUMENT;
+    return wait_search(b, ticket, b->slot_agg[ticket % mcac_box::kRing], out, skipped);
+}
+
+int mcac_release(mcac_box *b) {
+    if (!b) return MCAC_E_ARGUMENT;
+    hipLaunchKernelGGL(k_release, dim3(1), dim3(1), 0, b->stream, b->d);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 int mcac_box_download_spheres(mcac_box *b, double *x, double *y, double *z, double *rx, double *ry, double *rz) {
@@ -648,7 +721,8 @@ int mcac_box_members(mcac_box *b, int64_t agg, int64_t *out, int64_t max, int64_
 
 int mcac_box_set_timing(mcac_box *b, int enable) {
     if (!b) return MCAC_E_ARGUMENT;
-    b->timing = enable != 0;
+    b->timing = enable < 0 ? 0 : enable;
+    for (int k = 0; k < mcac_box::kRing; ++k) b->slot_timed[k] = 0;
     b->t_search = b->t_translate = b->t_merge = b->t_filter = b->t_solve = 0;
     b->n_search = b->n_translate = b->n_merge = 0;
     return 0;
@@ -656,6 +730,11 @@ int mcac_box_set_timing(mcac_box *b, int enable) {
 
 int mcac_box_timing(mcac_box *b, double out[8]) {
     if (!b || !out) return MCAC_E_ARGUMENT;
+    for (int k = 0; k < mcac_box::kRing; ++k)
+        if (b->slot_timed[k] == 2) {
+            int rc = harvest_timing(b, k);
+            if (rc) return rc;
+        }
     out[0] = b->t_search;
     out[1] = b->t_translate;
     out[2] = b->t_merge;

@@ -144,3 +144,83 @@ def test_c3_full_size_invariants():
     assert sim.scalar("total_volume") == pytest.approx(v0, rel=1e-12)
     assert np.all(np.diff(st["time"]) > 0)
     assert np.all(st["dist"][st["contact"] == 1] <= st["lpm"][st["contact"] == 1])
+
+
+def test_queued_moves_match_synchronous_moves():
+    """Gated queued moves (three in flight) give the records and positions of one-at-a-time moves."""
+    say("case: queued moves vs synchronous moves, lognormal_small after 3000 steps")
+    sim = O.OracleSim((CFG / "lognormal_small.ini").read_text(), fast_init=True, trace=False)
+    sim.run(3000)
+    a, sph, agg = upload_oracle_state(sim)
+    b, _, _ = upload_oracle_state(sim)
+    maxr = sim.scalar("maxradius")
+    rng = np.random.default_rng(5)
+    live = list(range(len(agg["n"])))
+    draws = []
+
+    def draw(k):
+        while len(draws) <= k:
+            d = rng.normal(size=3)
+            draws.append((int(rng.integers(1 << 30)), d / np.linalg.norm(d), 1 + 6 * rng.random()))
+        return draws[k]
+
+    def plan(k):
+        i, d, f = draw(k)
+        agg_id = live[i % len(live)]
+        return agg_id, d, agg["lpm"][agg_id] * f
+
+    contacts = 0
+    flight = []
+    for step in range(600):
+        while len(flight) < 3:
+            aid, d, lpm = plan(len(flight))
+            flight.append((b.move_enqueue(aid, d, lpm, maxr), aid, d, lpm))
+        ticket, aid, d, lpm = flight.pop(0)
+        got, skipped = b.move_result(ticket)
+        assert not skipped
+        ref = a.next_contact(aid, d, lpm, maxr, move=True)
+        for f in ("distance", "move", "moving_sphere", "other_sphere", "other_agg", "contact"):
+            assert got[f] == ref[f], (step, f)
+        draws.pop(0)
+        if got["contact"]:
+            contacts += 1
+            for t, *_ in flight:
+                assert b.move_result(t)[1], "a move ran past a contact"
+            flight.clear()
+            ra, rb = a.merge(ref), b.merge(got)
+            assert ra.tobytes() == rb.tobytes()
+            b.release()
+            if ra["merged"]:
+                live.remove(int(ra["removed"]))
+    while flight:  # moves still queued ran on the device: replay them on the other box
+        ticket, aid, d, lpm = flight.pop(0)
+        got, skipped = b.move_result(ticket)
+        assert not skipped
+        ref = a.next_contact(aid, d, lpm, maxr, move=True)
+        assert got["distance"] == ref["distance"] and got["move"] == ref["move"]
+        if got["contact"]:
+            for t, *_ in flight:
+                assert b.move_result(t)[1]
+            flight.clear()
+            assert a.merge(ref).tobytes() == b.merge(got).tobytes()
+            b.release()
+    assert contacts > 5
+    sa, sb = a.spheres(), b.spheres()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_run_in_pieces_bit_exact():
+    """Runs split over many mcac_sim_run calls of odd sizes equal one oracle run."""
+    say("case: lognormal_small in pieces")
+    text = (CFG / "lognormal_small.ini").read_text()
+    ora = O.OracleSim(text, fast_init=True)
+    ora.run(5000)
+    sim = mcac().Simulation(text, trace_steps=True, trace_merges=True)
+    done = 0
+    for n in (1, 2, 3, 7, 333, 1000, 1, 1654, 1999):
+        done += sim.run(n)
+    assert done == 5000
+    a, b = ora.steps(), sim.steps()
+    assert a.tobytes() == b.tobytes()
+    assert ora.merges().tobytes() == sim.merges().tobytes()
