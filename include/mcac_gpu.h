@@ -123,7 +123,10 @@ int mcac_merge(mcac_box *box, const mcac_contact *contact, mcac_merge_result *ou
  * (after the caller has merged).  At most 8 steps may be in flight. */
 int mcac_move_enqueue(mcac_box *box, int64_t agg, const double dir[3], double lpm, double maxradius,
                       uint64_t *ticket);
-/* Waits for an enqueued step; *skipped is 1 when it ran behind a closed gate. */
+/* Waits for an enqueued step, in ticket order.  *skipped is 0 when the step
+ * ran, 1 when it did nothing behind a closed gate (issue it again), and 2 when
+ * it ran without a contact but needed a second launch, during which the steps
+ * enqueued after it were skipped (they report 1; the gate is open again). */
 int mcac_move_result(mcac_box *box, uint64_t ticket, mcac_contact *out, int *skipped);
 /* Reopens the gate closed by a step that found a contact. */
 int mcac_release(mcac_box *box);
@@ -142,6 +145,11 @@ int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int6
  * translate_ms, merge_ms, n_search, n_translate, n_merge, filter_ms, solve_ms]. */
 int mcac_box_set_timing(mcac_box *box, int enable);
 int mcac_box_timing(mcac_box *box, double out[8]);
+/* Launch statistics: [searches in one launch (k_step), searches in two
+ * launches (k_filter + k_solve), one-launch searches finished by a second
+ * launch, cell-list rebuilds, member-pool compactions, member-pool slots in
+ * use].  No reference counterpart (diagnostics). */
+int mcac_box_stats(mcac_box *box, int64_t out[6]);
 
 /* distance_to_contact(Sphere, Sphere, dir, dist) (src/spheres/sphere_contact.cpp:47-125)
  * evaluated on the device for n independent inputs: p1, p2, dir are (n,3)

@@ -53,7 +53,7 @@ GPU_SYMBOLS = (
     "mcac_box_create", "mcac_box_destroy", "mcac_box_upload", "mcac_next_contact", "mcac_translate",
     "mcac_move", "mcac_merge", "mcac_move_enqueue", "mcac_move_result", "mcac_release",
     "mcac_box_download_spheres", "mcac_box_download_aggs", "mcac_box_members",
-    "mcac_box_set_timing", "mcac_box_timing", "mcac_swept_sphere_batch", "mcac_version",
+    "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_swept_sphere_batch", "mcac_version",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
@@ -110,6 +110,7 @@ def load_gpu():
         L.mcac_box_members.argtypes = [_P, _I64, _I64P, _I64, _I64P]
         L.mcac_box_set_timing.argtypes = [_P, _I]
         L.mcac_box_timing.argtypes = [_P, _DP]
+        L.mcac_box_stats.argtypes = [_P, _I64P]
         L.mcac_swept_sphere_batch.argtypes = [_I64, _DP, _DP, _DP, _DP, _DP, _DP, _D, _DP, _I]
         L.mcac_version.restype = ctypes.c_char_p
         _gpu = L
@@ -243,11 +244,12 @@ class Box:
         return t.value
 
     def move_result(self, ticket: int):
-        """(contact record, skipped) of a queued move."""
+        """(contact record, skipped) of a queued move: skipped 0 ran, 1 did nothing
+        behind a closed gate, 2 ran while the moves queued after it were skipped."""
         out = np.zeros(1, CONTACT_DTYPE)
         sk = ctypes.c_int()
         _check(load_gpu().mcac_move_result(self.h, ticket, out.ctypes.data, ctypes.byref(sk)), "mcac_move_result")
-        return out[0], bool(sk.value)
+        return out[0], int(sk.value)
 
     def release(self):
         """Reopens the gate closed by a queued move that found a contact."""
@@ -294,6 +296,13 @@ class Box:
         _check(load_gpu().mcac_box_timing(self.h, _dp(out)), "mcac_box_timing")
         keys = ("search_ms", "translate_ms", "merge_ms", "n_search", "n_translate", "n_merge", "filter_ms", "solve_ms")
         return dict(zip(keys, out.tolist()))
+
+
+    def stats(self):
+        out = np.zeros(6, np.int64)
+        _check(load_gpu().mcac_box_stats(self.h, _ip(out)), "mcac_box_stats")
+        keys = ("one_launch", "two_launch", "handover", "cell_rebuilds", "pool_compactions", "pool_used")
+        return dict(zip(keys, (int(v) for v in out)))
 
 
 class _BorrowedBox(Box):

@@ -15,6 +15,10 @@ constexpr int kMaxVisit = 4096;        // visited cells handled with an LDS pref
 constexpr int kMaxPlan = 4096;         // filtered candidates planned in LDS
 constexpr int kOverlapSlots = 16;      // contacts kept per sphere row (multi-kernel merge)
 constexpr int kSmallMerge = 256;       // merged aggregates up to this size use one fused kernel
+constexpr int kStepCands = 32;         // k_step solves up to this many survivors itself
+constexpr long long kStepPairs = 16384;  // ... holding up to this many sphere pairs
+constexpr int kStepMover = 1024;       // movers up to this size may take k_step
+constexpr int kPendingSolve = -1;      // StepRecord.status: pairs left for k_solve
 
 // Counters and results of the contact search that stay on the device.
 struct SearchState {
@@ -28,6 +32,7 @@ struct SearchState {
     int newcell;
     int pad;
     unsigned long long gate;   // seq of a gated step that found a contact (0: open)
+    unsigned long long tick;   // k_step blocks: finished << 56 | appended << 32 | visited
 };
 
 // Record published to pinned host memory by the last block of a search.
@@ -66,6 +71,8 @@ struct DevBox {
     int *alive;                                // live ids, ascending (may hold dead ids)
     // cell list: CSR by reference cell, built from the live ids
     int *cstart, *citems, *ccursor;
+    double *cpx, *cpy, *cpz, *cpr;             // CoM and rmax in CSR order (rmax < 0: entry dropped)
+    int *cslot;                                // CSR position of each aggregate (-1: none)
     unsigned char *dirty;                      // moved, merged or removed since the build
     int *dirty_list;
     // search scratch
@@ -84,6 +91,12 @@ struct DevBox {
 
 struct SearchArgs {
     int src;
+    int n_mov, start;          // mover size and member segment (host mirror)
+    int known;                 // host supplies the mover's CoM, rmax, relative CoM and window
+    double sx, sy, sz, srmax, srx, sry, srz;
+    int b1[3], len[3];
+    int dirty_n;               // dirty-list length when this step runs
+    int dirty_slot;            // dirty-list index this step's move appends, -1 when already listed
     int n_div;
     int cap;
     int n_built;               // items in the CSR cell list
@@ -97,6 +110,7 @@ struct SearchArgs {
     StepRecord *out;           // pinned host record
     unsigned long long seq;
     int gated;                 // pipelined step: closes the gate on contact, skipped behind it
+    int reopen;                // second-launch solve of a gated step: reopens the gate without contact
 };
 
 struct MergeArgs {

@@ -17,11 +17,11 @@ constexpr double kCoordinationEpsilon = 1e-10; // constants.hpp:128
 // for finite data and only guarantees termination on corrupted input.
 constexpr int kWrapCap = 64;
 
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
 // physical_model.hpp:99-109
-__device__ __forceinline__ double periodic_distance(double d, double L) {
+__host__ __device__ __forceinline__ double periodic_distance(double d, double L) {
     double p = d;
     double half = 0.5 * L;
     for (int it = 0; it < kWrapCap && p < -half; ++it) p += L;
@@ -29,7 +29,7 @@ __device__ __forceinline__ double periodic_distance(double d, double L) {
     return p;
 }
 // physical_model.hpp:113-122
-__device__ __forceinline__ double periodic_position(double x, double L) {
+__host__ __device__ __forceinline__ double periodic_position(double x, double L) {
     double p = x;
     for (int it = 0; it < kWrapCap && p < 0; ++it) p += L;
     for (int it = 0; it < kWrapCap && p >= L; ++it) p -= L;

@@ -188,28 +188,30 @@ void PickTable::materialize(Chunk &c, double A) {
 // Offsets of the chunk for the true start S.
 void PickTable::settle(Chunk &c, double S) {
     const size_t m = c.ts.size();
-    c.seg.clear();
-    c.segd.clear();
-    c.segabs.clear();
     if (m == 0) {
+        c.fast = true;
+        c.fast_d = 0.0;
         c.true_end = S;
         return;
     }
-    if (!same_binade(S, c.A) && S != c.A) {
+    if (S != c.A && !same_binade(S, c.A)) {
         // the stored start is in another binade: re-sum from the true start
         materialize(c, S);
     }
     const double d0 = S - c.A;
     const double L0 = c.L[0], L1 = c.L[m - 1];
-    if (S == c.A || (c.flags.empty() && same_binade(S, c.A) && binade(L0) == binade(L1) &&
-                     same_binade(L0 + d0, L0) && same_binade(L1 + d0, L1))) {
+    if (S == c.A || (c.flags.empty() && binade(L0) == binade(L1) && same_binade(L0 + d0, L0) &&
+                     same_binade(L1 + d0, L1))) {
         // no tie and no binade change for L or L + d: the whole chunk is L + d
-        c.seg.push_back(0);
-        c.segd.push_back(d0);
-        c.segabs.push_back(0);
+        c.fast = true;
+        c.fast_d = d0;
         c.true_end = L1 + d0;
         return;
     }
+    c.fast = false;
+    c.seg.clear();
+    c.segd.clear();
+    c.segabs.clear();
     // bound on |d| over the chunk: the start offset plus a grid unit per exact step
     const double ulp_end = std::ldexp(1.0, binade(std::max(std::fabs(L1), std::fabs(S)) * 2.0) - 52);
     const double D = std::fabs(d0) + 64.0 * ulp_end;
@@ -286,9 +288,8 @@ void PickTable::settle(Chunk &c, double S) {
     // many exact positions, or the offset left its bound: re-sum from the true start
     if (!ok || c.seg.size() > 24) {
         materialize(c, S);
-        c.seg.assign(1, 0);
-        c.segd.assign(1, 0.0);
-        c.segabs.assign(1, 0);
+        c.fast = true;
+        c.fast_d = 0.0;
         c.true_end = c.L[m - 1];
         return;
     }
@@ -306,6 +307,7 @@ void PickTable::walk(size_t from) {
 }
 
 double PickTable::true_at(const Chunk &c, size_t i) const {
+    if (c.fast) return c.L[i] + c.fast_d;
     size_t s = static_cast<size_t>(std::upper_bound(c.seg.begin(), c.seg.end(), static_cast<int>(i)) - c.seg.begin()) - 1;
     return c.segabs[s] ? c.segd[s] : c.L[i] + c.segd[s];
 }

@@ -56,6 +56,8 @@ class PickTable {
         std::vector<int> seg;      // segment starts; true(i) = L[i] + segd[segment of i],
         std::vector<double> segd;  // or segd itself for an exactly stepped position
         std::vector<char> segabs;
+        bool fast = true;          // true(i) = L[i] + fast_d over the whole chunk
+        double fast_d = 0.0;
         double true_end = 0.0;
         bool dirty = true;         // entries changed: re-sum from the exact start
     };

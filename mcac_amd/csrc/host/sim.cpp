@@ -437,11 +437,27 @@ struct Simulation {
         return 0;
     }
 
-    int consume() {
+    // Reads the oldest queued step.  *done is false when the device skipped it
+    // (queued behind a step that closed the gate): then every step still in
+    // flight was skipped too, and their draws are issued again.
+    int consume(bool *done) {
+        *done = false;
+        const Pending P = flight.front();
+        mcac_contact c{};
+        int skipped = 0;
+        auto td0 = Clock::now();
+        int rc = mcac_move_result(box, P.ticket, &c, &skipped);
+        t_device += ms_since(td0);
+        if (rc) return rc;
+        flight.pop_front();
+        if (skipped == 1) return drain_skipped();
+        if (skipped == 2) {
+            rc = drain_skipped();  // the steps behind it did nothing; issue them again
+            if (rc) return rc;
+        }
         if (pm.time_to_write(total_events)) save_advancement();
         if (event && pm.with_domain_duplication && live.size() <= pm.n_monomeres / 8) return MCAC_INPUT_ERROR;
-        const Pending P = flight.front();
-        flight.pop_front();
+        *done = true;
         const int num = P.num;
         const size_t U = static_cast<size_t>(num);
         const double full = P.full;
@@ -449,13 +465,6 @@ struct Simulation {
         n_ties += P.tie ? 1 : 0;
         const int64_t num_label = trace_steps ? label_of(num) : -1;
         double deltatemps = max_time_step / pick.total();  // aggregat_list.cpp:54-58
-        mcac_contact c{};
-        int skipped = 0;
-        auto td0 = Clock::now();
-        int rc = mcac_move_result(box, P.ticket, &c, &skipped);
-        t_device += ms_since(td0);
-        if (rc) return rc;
-        if (skipped) return MCAC_UNKNOWN_ERROR;
         draws.pop_front();
         q_query += static_cast<double>(c.n_query);
         q_filtered += static_cast<double>(c.n_filtered);
@@ -576,9 +585,10 @@ struct Simulation {
                 if (rc) return rc;
             }
             if (flight.empty()) break;
-            int rc = consume();
+            bool stepped = false;
+            int rc = consume(&stepped);
             if (rc) return rc;
-            ++n;
+            if (stepped) ++n;
             // a stop met with steps still in flight (STOPCODE) lets those
             // steps, which already ran on the device, complete first
         }

@@ -1,7 +1,8 @@
 // Device cell list of the contact search (a4/a5): live aggregates bucketed by
 // their reference Verlet cell (aggregat.cpp:699-705) in CSR form.  Moves and
-// merges do not edit the CSR; they mark the aggregate dirty and append it to a
-// dirty list, which the search scans in full.  The host rebuilds the CSR when
+// merges do not edit the CSR; they drop the aggregate's packed entry (CoM and
+// rmax copied in CSR order) and append it to a dirty list, which the search
+// scans in full.  The host rebuilds the CSR when
 // the dirty list grows past ~sqrt(6 N) entries (rebuild cost ~3 N reads vs
 // ~half a dirty list per step).
 #pragma once
@@ -56,10 +57,16 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(DevBox b, int n_alive) 
         int id = b.alive[t];
         int c = b.acell[id];
         b.dirty[id] = (c >= 0) ? 0 : 1;
+        int pos = -1;
         if (c >= 0) {
-            int pos = atomicAdd(&b.ccursor[c], 1);
+            pos = atomicAdd(&b.ccursor[c], 1);
             b.citems[pos] = id;
+            b.cpx[pos] = b.ax[id];
+            b.cpy[pos] = b.ay[id];
+            b.cpz[pos] = b.az[id];
+            b.cpr[pos] = b.armax[id];
         }
+        b.cslot[id] = pos;
     }
 }
 

@@ -46,28 +46,35 @@ template <class T> __device__ __forceinline__ T block_sum(T v) {
     return t;
 }
 
-// In-place exclusive prefix sum of a[0..n) in LDS (n <= kBlock * chunk); writes
-// the total to a[n].  All threads of the block must call it.
+// Inclusive prefix sum over the 64 lanes of a wave.
+template <class T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// In-place exclusive prefix sum of a[0..n) in LDS; writes the total to a[n].
+// Each thread sums a contiguous run, the runs are scanned across the block
+// with wave shuffles.  All threads of the block must call it.
 template <class T> __device__ __forceinline__ void block_exclusive_scan(T *a, int n) {
-    __shared__ T part[kBlock];
+    __shared__ T wsum[kBlock / 64];
     const int per = (n + kBlock - 1) / kBlock;
     const int lo = threadIdx.x * per;
     const int hi = (lo + per < n) ? lo + per : n;
     T s = 0;
     for (int i = lo; i < hi; ++i) s += a[i];
-    part[threadIdx.x] = s;
+    const T incl = wave_inclusive_scan(s);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) wsum[w] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        T acc = 0;
-        for (int i = 0; i < kBlock; ++i) {
-            T v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
-        a[n] = acc;
-    }
-    __syncthreads();
-    T acc = part[threadIdx.x];
+    T off = 0;
+    for (int k = 0; k < w; ++k) off += wsum[k];
+    T acc = off + incl - s;
+    if (threadIdx.x == kBlock - 1) a[n] = off + incl;
     for (int i = lo; i < hi; ++i) {
         T v = a[i];
         a[i] = acc;
@@ -77,7 +84,7 @@ template <class T> __device__ __forceinline__ void block_exclusive_scan(T *a, in
 }
 
 // aggregat.cpp:699-705 — reference Verlet cell of an aggregate position.
-__device__ __forceinline__ int ref_cell(double x, double y, double z, int n, double L) {
+__host__ __device__ __forceinline__ int ref_cell(double x, double y, double z, int n, double L) {
     const double step = static_cast<double>(n) / L;
     const double p[3] = {x, y, z};
     int c[3];
@@ -90,7 +97,7 @@ __device__ __forceinline__ int ref_cell(double x, double y, double z, int n, dou
 }
 
 // verlet.cpp:57-78 — first visited cell and number of cells per axis.
-__device__ __forceinline__ void verlet_window(const double src[3], const double vec[3], double mindist, int n,
+__host__ __device__ __forceinline__ void verlet_window(const double src[3], const double vec[3], double mindist, int n,
                                               double W, int b1[3], int len[3]) {
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
@@ -115,9 +122,12 @@ __device__ __forceinline__ int window_offset(int c, int b1, int n) {
     return o;
 }
 
-// Marks an aggregate as moved since the cell-list build (single thread).
+// Marks an aggregate as moved since the cell-list build (single thread): its
+// packed cell entry is dropped and it joins the dirty list.
 __device__ __forceinline__ void mark_dirty(const DevBox &b, int id) {
     if (!b.dirty[id]) {
+        const int cs = b.cslot[id];
+        if (cs >= 0) b.cpr[cs] = -1.0;
         b.dirty[id] = 1;
         unsigned int k = b.ss->dirty_count;
         b.dirty_list[k] = id;

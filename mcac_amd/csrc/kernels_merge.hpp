@@ -77,6 +77,7 @@ __device__ __forceinline__ void store_merged(const DevBox &b, const MergeArgs &a
     b.acount[a.rem] = 0;
     mark_dirty(b, k);
     mark_dirty(b, a.rem);
+    b.cslot[a.rem] = -1;  // removed for good: no entry of a later build is its
 }
 
 __device__ __forceinline__ void publish_merge(const MergeArgs &a, const mcac_merge_result &res) {

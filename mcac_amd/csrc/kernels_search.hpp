@@ -12,6 +12,7 @@
 //            runs the reference early-exit scan (aggregat_list.cpp:459-482),
 //            moves the aggregate (aggregat.cpp:148-161) when it is small, and
 //            publishes the result to pinned host memory.
+// k_step     both in one launch for small movers with few survivors.
 // k_translate moves the spheres of a large mover.
 #pragma once
 #include "kernels_common.hpp"
@@ -29,6 +30,10 @@ __device__ __forceinline__ void append_candidate(const DevBox &b, const SearchAr
     }
 }
 
+// Per-thread counts of the filter: appended candidates << 32 | aggregates in the window.
+constexpr unsigned long long kAppendOne = 1ull << 32;
+constexpr unsigned long long kTicketOne = 1ull << 56;
+
 // A gated step enqueued behind a step that found a contact does nothing.
 __device__ __forceinline__ bool behind_gate(const DevBox &b, const SearchArgs &a) {
     if (!a.gated) return false;
@@ -36,101 +41,290 @@ __device__ __forceinline__ bool behind_gate(const DevBox &b, const SearchArgs &a
     return g != 0ull && g < a.seq;
 }
 
-__global__ __launch_bounds__(kBlock) void k_filter(DevBox b, SearchArgs a) {
-    if (behind_gate(b, a)) return;
+// Filter work of one block: the aggregates of the reference's visited Verlet
+// cells and the dirty list, bounding sweep, candidate append (shared by
+// k_filter and k_step).
+__device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, const SearchArgs &a) {
     __shared__ int win[6];
     __shared__ double q[4];
     __shared__ int pre[kMaxVisit + 1];
+    __shared__ int cbeg[kMaxVisit];
     __shared__ int nvisit;
     const int n = a.n_div;
-    if (threadIdx.x == 0) {
-        double src[3] = {b.ax[a.src], b.ay[a.src], b.az[a.src]};
-        double vec[3] = {a.lpm * a.dir[0], a.lpm * a.dir[1], a.lpm * a.dir[2]};
-        double mindist = b.armax[a.src] + a.maxradius;  // aggregat_list.cpp:492
-        int b1[3], len[3];
-        verlet_window(src, vec, mindist, n, a.L, b1, len);
-        for (int l = 0; l < 3; ++l) {
-            win[l] = b1[l];
-            win[3 + l] = len[l];
-            q[l] = src[l];
+    int b1i, b1j, b1k, li, lj, lk, V;
+    double sx, sy, sz, sr;
+    if (a.known) {
+        // window and mover data from the host mirror (verlet.cpp:57-78)
+        b1i = a.b1[0];
+        b1j = a.b1[1];
+        b1k = a.b1[2];
+        li = a.len[0];
+        lj = a.len[1];
+        lk = a.len[2];
+        sx = a.sx;
+        sy = a.sy;
+        sz = a.sz;
+        sr = a.srmax;
+        V = li * lj * lk;
+    } else {
+        if (threadIdx.x == 0) {
+            double src[3] = {b.ax[a.src], b.ay[a.src], b.az[a.src]};
+            double vec[3] = {a.lpm * a.dir[0], a.lpm * a.dir[1], a.lpm * a.dir[2]};
+            double mindist = b.armax[a.src] + a.maxradius;  // aggregat_list.cpp:492
+            int b1[3], len[3];
+            verlet_window(src, vec, mindist, n, a.L, b1, len);
+            for (int l = 0; l < 3; ++l) {
+                win[l] = b1[l];
+                win[3 + l] = len[l];
+                q[l] = src[l];
+            }
+            q[3] = b.armax[a.src];
+            nvisit = len[0] * len[1] * len[2];
         }
-        q[3] = b.armax[a.src];
-        nvisit = len[0] * len[1] * len[2];
+        __syncthreads();
+        b1i = win[0];
+        b1j = win[1];
+        b1k = win[2];
+        li = win[3];
+        lj = win[4];
+        lk = win[5];
+        sx = q[0];
+        sy = q[1];
+        sz = q[2];
+        sr = q[3];
+        V = nvisit;
     }
-    __syncthreads();
-    const int b1i = win[0], b1j = win[1], b1k = win[2];
-    const int li = win[3], lj = win[4], lk = win[5];
-    const double sx = q[0], sy = q[1], sz = q[2], sr = q[3];
-    const int V = nvisit;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gstride = gridDim.x * blockDim.x;
-    unsigned long long inwin = 0;
+    unsigned long long cnt = 0;
+    int total;  // entries of the visited cells (or of the whole list for a very wide window)
     if (V <= kMaxVisit) {
-        // sizes of the visited cells in visit order, then their prefix
+        // sizes and starts of the visited cells in visit order, then their prefix
         for (int v = threadIdx.x; v < V; v += blockDim.x) {
             int oi = v / (lj * lk), rem = v - oi * (lj * lk), oj = rem / lk, ok = rem - oj * lk;
             int ci = window_offset(b1i + oi, 0, n), cj = window_offset(b1j + oj, 0, n),
                 ck = window_offset(b1k + ok, 0, n);
             int c = (ci * n + cj) * n + ck;
-            pre[v] = b.cstart[c + 1] - b.cstart[c];
+            const int c0 = b.cstart[c];
+            cbeg[v] = c0;
+            pre[v] = b.cstart[c + 1] - c0;
         }
         __syncthreads();
         block_exclusive_scan(pre, V);
-        const int total = pre[V];
-        for (int w = gtid; w < total; w += gstride) {
-            int lo = 0, hi = V;  // largest v with pre[v] <= w
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (pre[mid] <= w) lo = mid;
-                else hi = mid;
-            }
-            const int v = lo;
-            int oi = v / (lj * lk), rem = v - oi * (lj * lk), oj = rem / lk, ok = rem - oj * lk;
-            int ci = window_offset(b1i + oi, 0, n), cj = window_offset(b1j + oj, 0, n),
-                ck = window_offset(b1k + ok, 0, n);
-            int c = (ci * n + cj) * n + ck;
-            int id = b.citems[b.cstart[c] + (w - pre[v])];
-            if (b.dirty[id] || id == a.src) continue;
-            ++inwin;
-            double d = swept_sphere(sx, sy, sz, sr, b.ax[id], b.ay[id], b.az[id], b.armax[id], a.dir[0], a.dir[1],
-                                    a.dir[2], a.lpm, a.L);
-            if (d < a.lpm) append_candidate(b, a, id, v, d);
-        }
+        total = pre[V];
     } else {
-        // very wide window: scan every built entry
-        for (int w = gtid; w < a.n_built; w += gstride) {
-            int id = b.citems[w];
-            if (b.dirty[id] || id == a.src) continue;
-            int c = b.acell[id];
+        total = a.n_built;
+    }
+    // one index space: cell entries, then the aggregates moved since the build
+    // (at their current cell), so that both kinds run side by side
+    const int nd = a.dirty_n;
+    for (int w = gtid; w < total + nd; w += gstride) {
+        int id, rank;
+        double x, y, z, r;
+        if (w < total) {
+            int pos;
+            if (V <= kMaxVisit) {
+                int lo = 0, hi = V;  // largest v with pre[v] <= w
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (pre[mid] <= w) lo = mid;
+                    else hi = mid;
+                }
+                rank = lo;
+                pos = cbeg[lo] + (w - pre[lo]);
+            } else {
+                pos = w;
+            }
+            id = b.citems[pos];
+            x = b.cpx[pos];
+            y = b.cpy[pos];
+            z = b.cpz[pos];
+            r = b.cpr[pos];
+            if (r < 0.0 || id == a.src) continue;  // moved since the build, or the mover
+            if (V > kMaxVisit) {
+                const int c = ref_cell(x, y, z, n, a.L);
+                int oi = window_offset(c / (n * n), b1i, n), oj = window_offset((c / n) % n, b1j, n),
+                    ok = window_offset(c % n, b1k, n);
+                if (oi >= li || oj >= lj || ok >= lk) continue;
+                rank = (oi * lj + oj) * lk + ok;
+            }
+        } else {
+            id = b.dirty_list[w - total];
+            const int c = b.acell[id];
+            x = b.ax[id];
+            y = b.ay[id];
+            z = b.az[id];
+            r = b.armax[id];
+            if (c < 0 || id == a.src) continue;
             int oi = window_offset(c / (n * n), b1i, n), oj = window_offset((c / n) % n, b1j, n),
                 ok = window_offset(c % n, b1k, n);
             if (oi >= li || oj >= lj || ok >= lk) continue;
-            ++inwin;
-            double d = swept_sphere(sx, sy, sz, sr, b.ax[id], b.ay[id], b.az[id], b.armax[id], a.dir[0], a.dir[1],
-                                    a.dir[2], a.lpm, a.L);
-            if (d < a.lpm) append_candidate(b, a, id, (oi * lj + oj) * lk + ok, d);
+            rank = (oi * lj + oj) * lk + ok;
+        }
+        cnt += 1;
+        const double d = swept_sphere(sx, sy, sz, sr, x, y, z, r, a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
+        if (d < a.lpm) {
+            append_candidate(b, a, id, rank, d);
+            cnt += kAppendOne;
         }
     }
-    // aggregates moved since the build, at their current cell
-    const int nd = static_cast<int>(b.ss->dirty_count);
-    for (int w = gtid; w < nd; w += gstride) {
-        int id = b.dirty_list[w];
-        int c = b.acell[id];
-        if (c < 0 || id == a.src) continue;
-        int oi = window_offset(c / (n * n), b1i, n), oj = window_offset((c / n) % n, b1j, n),
-            ok = window_offset(c % n, b1k, n);
-        if (oi >= li || oj >= lj || ok >= lk) continue;
-        ++inwin;
-        double d = swept_sphere(sx, sy, sz, sr, b.ax[id], b.ay[id], b.az[id], b.armax[id], a.dir[0], a.dir[1], a.dir[2],
-                                a.lpm, a.L);
-        if (d < a.lpm) append_candidate(b, a, id, (oi * lj + oj) * lk + ok, d);
-    }
-    unsigned long long tot = block_sum(inwin);
+    return cnt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_filter(DevBox b, SearchArgs a) {
+    if (behind_gate(b, a)) return;
+    const unsigned long long tot = block_sum(filter_phase(b, a)) & 0xffffffffull;
     if (threadIdx.x == 0 && tot) atomicAdd(&b.ss->nquery, tot);
+}
+
+// What the last block of a step knows besides the candidates.
+struct StepTotals {
+    unsigned long long nquery;  // aggregates of the visited cells tested
+    unsigned int nfilt;         // candidates appended (may exceed the capacity)
+    int overflow;
+    int src_slot;               // packed cell entry of the mover (-1: none)
+};
+
+// Sphere of the mover a thread places in the finishing block, loaded early
+// when the mover has at most one sphere per thread.
+struct MoverSphere {
+    int loaded;  // the block loaded the mover (uniform)
+    int sp;      // this thread's sphere, -1: none
+    double rx, ry, rz;
+};
+
+__device__ __forceinline__ MoverSphere load_mover_sphere(const DevBox &b, const SearchArgs &a) {
+    MoverSphere m{a.apply && a.n_mov <= kBlock, -1, 0., 0., 0.};
+    if (m.loaded && static_cast<int>(threadIdx.x) < a.n_mov) {
+        m.sp = b.members[a.start + threadIdx.x];
+        m.rx = b.srx[m.sp];
+        m.ry = b.sry[m.sp];
+        m.rz = b.srz[m.sp];
+    }
+    return m;
 }
 
 __device__ __forceinline__ long long chunk_count(long long pairs, long long chunk) {
     return (pairs + chunk - 1) / chunk;
+}
+
+// Last block of a step: orders the candidates like the reference's multimap,
+// runs its early-exit scan over the exact per-candidate distances
+// (cand_exact / cand_pair), moves the aggregate, publishes the record and
+// resets the search counters.
+__device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a, unsigned int F, long long nsrc,
+                                            int ssrc, const StepTotals &t, const MoverSphere &m) {
+    // std::multimap order: (bounding distance, visit rank, label)
+    for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
+        const double dc = b.cand_db[c];
+        const int rc = b.cand_rank[c], ic = b.cand_id[c];
+        unsigned int pos = 0;
+        for (unsigned int e = 0; e < F; ++e) {
+            double de = b.cand_db[e];
+            int re = b.cand_rank[e];
+            pos += (de < dc || (de == dc && (re < rc || (re == rc && b.cand_id[e] < ic)))) ? 1u : 0u;
+        }
+        b.order[pos] = static_cast<int>(c);
+    }
+    __syncthreads();
+    __shared__ double s_ref[3];
+    SearchState *s = b.ss;
+    if (threadIdx.x == 0) {
+        double closest = kInf;
+        int best = -1;
+        for (unsigned int pos = 0; pos < F; ++pos) {
+            const int c = b.order[pos];
+            if (closest <= 0.) break;
+            if (closest < b.cand_db[c]) break;
+            if (b.cand_exact[c] < closest) {
+                closest = b.cand_exact[c];
+                best = c;
+            }
+        }
+        long long ms = -1, os = -1, oagg = -1;
+        if (best >= 0) {
+            long long p = b.cand_pair[best];
+            int k = b.cand_id[best];
+            long long nk = b.acount[k];
+            long long i = p / nk, j = p - i * nk;
+            ms = b.members[ssrc + i];
+            os = b.members[b.astart[k] + j];
+            oagg = k;
+        }
+        // calcul.cpp:128-130 and aggregat.cpp:148-153
+        const int contact = closest <= a.lpm;
+        const double move = contact ? closest : a.lpm;
+        double old[3], rc3[3];
+        if (a.known) {
+            old[0] = a.sx, old[1] = a.sy, old[2] = a.sz;
+            rc3[0] = a.srx, rc3[1] = a.sry, rc3[2] = a.srz;
+        } else {
+            old[0] = b.ax[a.src], old[1] = b.ay[a.src], old[2] = b.az[a.src];
+            rc3[0] = b.arx[a.src], rc3[1] = b.ary[a.src], rc3[2] = b.arz[a.src];
+        }
+        for (int l = 0; l < 3; ++l) {
+            s->newcom[l] = periodic_position(old[l] + a.dir[l] * move, a.L);
+            s->refpos[l] = s->newcom[l] - rc3[l];
+            s_ref[l] = s->refpos[l];
+        }
+        s->newcell = ref_cell(s->newcom[0], s->newcom[1], s->newcom[2], a.n_div, a.L);
+        StepRecord *o = a.out;
+        o->dist = closest;
+        o->move = move;
+        o->ms = ms;
+        o->os = os;
+        o->oagg = oagg;
+        o->contact = contact;
+        o->status = t.overflow ? MCAC_E_CAPACITY : 0;
+        o->nquery = t.nquery;
+        o->nfilt = t.nfilt;
+        unsigned long long tp = 0;
+        for (unsigned int c = 0; c < F; ++c) tp += static_cast<unsigned long long>(nsrc) * b.acount[b.cand_id[c]];
+        o->npairs = tp;
+        if (a.apply) {
+            b.ax[a.src] = s->newcom[0];
+            b.ay[a.src] = s->newcom[1];
+            b.az[a.src] = s->newcom[2];
+            b.acell[a.src] = s->newcell;
+            if (a.dirty_slot >= 0) {  // first move since the cell-list build
+                b.dirty[a.src] = 1;
+                b.dirty_list[a.dirty_slot] = a.src;
+                s->dirty_count = static_cast<unsigned int>(a.dirty_slot + 1);
+                if (t.src_slot >= 0) b.cpr[t.src_slot] = -1.0;
+            }
+        }
+        if (a.gated) {
+            if (contact) s->gate = a.seq;     // the steps behind wait for the merge
+            else if (a.reopen) s->gate = 0ull;  // second-launch solve without contact
+        }
+        o->skipped = 0;
+        o->dirty_count = a.dirty_n + ((a.apply && a.dirty_slot >= 0) ? 1 : 0);
+        s->nfilt = 0;
+        s->nquery = 0;
+        s->overflow = 0;
+        s->done_solve = 0;
+        s->tick = 0ull;
+    }
+    __syncthreads();
+    if (a.apply && a.translate_in_finish && m.loaded) {
+        if (m.sp >= 0) {
+            b.sx[m.sp] = s_ref[0] + m.rx;
+            b.sy[m.sp] = s_ref[1] + m.ry;
+            b.sz[m.sp] = s_ref[2] + m.rz;
+        }
+    } else if (a.apply && a.translate_in_finish) {
+        const double r0 = s_ref[0], r1 = s_ref[1], r2 = s_ref[2];
+        for (int t = threadIdx.x; t < nsrc; t += blockDim.x) {
+            int sp = b.members[ssrc + t];
+            b.sx[sp] = r0 + b.srx[sp];
+            b.sy[sp] = r1 + b.sry[sp];
+            b.sz[sp] = r2 + b.srz[sp];
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
@@ -149,8 +343,8 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     __shared__ bool last;
     unsigned int F = b.ss->nfilt;
     if (F > static_cast<unsigned int>(a.cap)) F = static_cast<unsigned int>(a.cap);
-    const long long nsrc = b.acount[a.src];
-    const int ssrc = b.astart[a.src];
+    const long long nsrc = a.n_mov;
+    const int ssrc = a.start;
     // plan: chunks of the pair space of each candidate
     long long tpairs = 0;
     for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) tpairs += nsrc * b.acount[b.cand_id[c]];
@@ -250,96 +444,103 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
         b.cand_exact[c] = bd;
         b.cand_pair[c] = (bd < kInf) ? bp : -1;
     }
-    // std::multimap order: (bounding distance, visit rank, label)
-    for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
-        const double dc = b.cand_db[c];
-        const int rc = b.cand_rank[c], ic = b.cand_id[c];
-        unsigned int pos = 0;
-        for (unsigned int e = 0; e < F; ++e) {
-            double de = b.cand_db[e];
-            int re = b.cand_rank[e];
-            pos += (de < dc || (de == dc && (re < rc || (re == rc && b.cand_id[e] < ic)))) ? 1u : 0u;
+    StepTotals t;
+    t.nquery = b.ss->nquery;
+    t.nfilt = b.ss->nfilt;
+    t.overflow = b.ss->overflow != 0u;
+    t.src_slot = b.cslot[a.src];
+    finish_step(b, a, F, nsrc, ssrc, t, MoverSphere{0, -1, 0., 0., 0.});
+}
+
+// One kernel per MC step for small movers: every block filters its share of
+// the Verlet window; the last block to finish (ticket) solves the sphere pairs
+// of the survivors itself, one candidate at a time, and finishes the step.
+// When the survivors hold more pairs than one block should take, it leaves
+// them in place, closes the gate for the steps queued behind and publishes a
+// pending record: the host then launches k_solve for the same step.
+__global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
+    if (behind_gate(b, a)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.out->skipped = 1;
+            a.out->contact = 0;
+            a.out->status = 0;
+            __threadfence_system();
+            __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
-        b.order[pos] = static_cast<int>(c);
+        return;
+    }
+    // loads the finishing block will need, issued before the filter work
+    const int src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
+    const MoverSphere mover = load_mover_sphere(b, a);
+    const unsigned long long cnt = block_sum(filter_phase(b, a));
+    // one atomic per block: finished blocks, appended candidates, visited aggregates
+    __shared__ bool last;
+    __shared__ unsigned long long s_total;
+    if (threadIdx.x == 0) {
+        const unsigned long long mine = kTicketOne + cnt;
+        __threadfence();
+        const unsigned long long old = atomicAdd(&b.ss->tick, mine);
+        last = (old >> 56) == gridDim.x - 1u;
+        s_total = old + mine;
     }
     __syncthreads();
-    __shared__ double s_ref[3];
+    if (!last) return;
+    __threadfence();
     SearchState *s = b.ss;
-    if (threadIdx.x == 0) {
-        double closest = kInf;
-        int best = -1;
-        for (unsigned int pos = 0; pos < F; ++pos) {
-            const int c = b.order[pos];
-            if (closest <= 0.) break;
-            if (closest < b.cand_db[c]) break;
-            if (b.cand_exact[c] < closest) {
-                closest = b.cand_exact[c];
-                best = c;
+    const unsigned long long total = s_total;
+    StepTotals t;
+    t.nquery = total & 0xffffffffull;
+    t.nfilt = static_cast<unsigned int>((total >> 32) & 0xffffffull);
+    t.overflow = t.nfilt > static_cast<unsigned int>(a.cap);
+    t.src_slot = src_slot;
+    const unsigned int F = t.overflow ? static_cast<unsigned int>(a.cap) : t.nfilt;
+    const long long nsrc = a.n_mov;
+    const int ssrc = a.start;
+    if (F > 0) {
+        long long tpairs = 0;
+        for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) tpairs += nsrc * b.acount[b.cand_id[c]];
+        tpairs = block_sum(tpairs);
+        __shared__ bool handover;
+        if (threadIdx.x == 0) handover = F > static_cast<unsigned int>(kStepCands) || tpairs > kStepPairs || t.overflow;
+        __syncthreads();
+        if (handover) {
+            if (threadIdx.x == 0) {
+                s->tick = 0ull;
+                s->nquery = t.nquery;  // k_solve reports it
+                if (a.gated) s->gate = a.seq;
+                StepRecord *o = a.out;
+                o->status = kPendingSolve;
+                o->contact = 0;
+                o->skipped = 0;
+                __threadfence_system();
+                __hip_atomic_store(&o->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            return;
+        }
+        // exact sphere-pair distances, lexicographic (d, i, j) minimum per candidate
+        for (unsigned int c = 0; c < F; ++c) {
+            const int k = b.cand_id[c];
+            const long long nk = b.acount[k];
+            const int sk = b.astart[k];
+            double bd = kInf;
+            long long bp = 0x7fffffffffffffffLL;
+            for (long long p = threadIdx.x; p < nsrc * nk; p += blockDim.x) {
+                long long i = p / nk;
+                long long j = p - i * nk;
+                int si = b.members[ssrc + i];
+                int sj = b.members[sk + j];
+                double d = swept_sphere(b.sx[si], b.sy[si], b.sz[si], b.sr[si], b.sx[sj], b.sy[sj], b.sz[sj],
+                                        b.sr[sj], a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
+                lexmin(bd, bp, d, p);
+            }
+            block_lexmin(bd, bp);
+            if (threadIdx.x == 0) {
+                b.cand_exact[c] = bd;
+                b.cand_pair[c] = (bd < kInf) ? bp : -1;
             }
         }
-        long long ms = -1, os = -1, oagg = -1;
-        if (best >= 0) {
-            long long p = b.cand_pair[best];
-            int k = b.cand_id[best];
-            long long nk = b.acount[k];
-            long long i = p / nk, j = p - i * nk;
-            ms = b.members[ssrc + i];
-            os = b.members[b.astart[k] + j];
-            oagg = k;
-        }
-        // calcul.cpp:128-130 and aggregat.cpp:148-153
-        const int contact = closest <= a.lpm;
-        const double move = contact ? closest : a.lpm;
-        const double old[3] = {b.ax[a.src], b.ay[a.src], b.az[a.src]};
-        const double rc3[3] = {b.arx[a.src], b.ary[a.src], b.arz[a.src]};
-        for (int l = 0; l < 3; ++l) {
-            s->newcom[l] = periodic_position(old[l] + a.dir[l] * move, a.L);
-            s->refpos[l] = s->newcom[l] - rc3[l];
-            s_ref[l] = s->refpos[l];
-        }
-        s->newcell = ref_cell(s->newcom[0], s->newcom[1], s->newcom[2], a.n_div, a.L);
-        StepRecord *o = a.out;
-        o->dist = closest;
-        o->move = move;
-        o->ms = ms;
-        o->os = os;
-        o->oagg = oagg;
-        o->contact = contact;
-        o->status = s->overflow ? MCAC_E_CAPACITY : 0;
-        o->nquery = s->nquery;
-        o->nfilt = s->nfilt;
-        unsigned long long tp = 0;
-        for (unsigned int c = 0; c < F; ++c) tp += static_cast<unsigned long long>(nsrc) * b.acount[b.cand_id[c]];
-        o->npairs = tp;
-        if (a.apply) {
-            b.ax[a.src] = s->newcom[0];
-            b.ay[a.src] = s->newcom[1];
-            b.az[a.src] = s->newcom[2];
-            b.acell[a.src] = s->newcell;
-            mark_dirty(b, a.src);
-        }
-        if (a.gated && contact) s->gate = a.seq;
-        o->skipped = 0;
-        o->dirty_count = static_cast<int>(s->dirty_count);
-        s->nfilt = 0;
-        s->nquery = 0;
-        s->overflow = 0;
-        s->done_solve = 0;
     }
-    __syncthreads();
-    if (a.apply && a.translate_in_finish) {
-        const double r0 = s_ref[0], r1 = s_ref[1], r2 = s_ref[2];
-        for (int t = threadIdx.x; t < nsrc; t += blockDim.x) {
-            int sp = b.members[ssrc + t];
-            b.sx[sp] = r0 + b.srx[sp];
-            b.sy[sp] = r1 + b.sry[sp];
-            b.sz[sp] = r2 + b.srz[sp];
-        }
-    }
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    finish_step(b, a, F, nsrc, ssrc, t, mover);
 }
 
 __global__ void k_release(DevBox b) { b.ss->gate = 0ull; }

@@ -38,7 +38,16 @@ struct mcac_box {
     // host mirrors for allocation, launch sizing and the dirty list
     std::vector<int> count, start, cap;
     std::vector<int> alive_ids;  // ascending, may hold dead ids until rebuilt
-    int64_t dirty_count = 0;         // upper bound of the device dirty list
+    // host mirrors of the device aggregate state: position, relative CoM and
+    // rmax are exact for every aggregate that no queued step moves (pend == 0);
+    // the dirty flags and list length are those after every queued step that
+    // will run (steps known to be skipped are taken out)
+    std::vector<double> h_x, h_y, h_z, h_rx, h_ry, h_rz, h_rmax;
+    std::vector<int> pend;
+    std::vector<unsigned char> h_dirty;
+    int64_t dirty_count = 0;         // device dirty-list length after the queued steps
+    int64_t dirty_epoch = 0;         // cell-list builds so far
+    bool gate_closed = false;        // a consumed queued step found a contact; not released yet
     int64_t dead_in_list = 0;
     int64_t pool_used = 0;
     // pinned records (host pointer, device pointer)
@@ -48,6 +57,15 @@ struct mcac_box {
     unsigned long long oldest = 1;   // oldest ticket not yet consumed
     int slot_apply[kRing] = {0}, slot_translate[kRing] = {0};
     int64_t slot_agg[kRing] = {0};
+    SearchArgs slot_args[kRing]{};   // arguments of queued searches (second-launch solves)
+    int slot_dirty_new[kRing] = {0}; // 1: the step's move appends its mover to the dirty list
+    int64_t slot_epoch[kRing] = {0};
+    int slot_void[kRing] = {0};      // known to be skipped: its predictions were taken out
+    int slot_fused[kRing] = {0};
+    int max_count = 0;               // upper bound of the aggregate sizes
+    int64_t n_handover = 0;          // k_step searches finished by k_solve
+    int64_t n_fused = 0, n_split = 0;  // searches launched as k_step / k_filter + k_solve
+    int64_t n_compactions = 0;
     MergeRecord *h_merge = nullptr, *d_merge = nullptr;
     unsigned long long seq = 0;        // step tickets (searches)
     unsigned long long merge_seq = 0;  // merge records
@@ -137,6 +155,8 @@ int rebuild_cells(mcac_box *b) {
     hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(kBlock), 0, b->stream, b->d, n_alive);
     HIP_TRY(hipGetLastError());
     b->dirty_count = 0;
+    std::fill(b->h_dirty.begin(), b->h_dirty.end(), 0);
+    b->dirty_epoch += 1;
     b->n_built = n_alive;
     b->n_rebuilds++;
     return 0;
@@ -150,6 +170,7 @@ int64_t rebuild_threshold(const mcac_box *b) {
 }
 
 int compact_pool(mcac_box *b, int64_t need) {
+    b->n_compactions += 1;
     // new layout: every live aggregate gets capacity 2*count (at least 4)
     std::vector<int> ids, nstart;
     int64_t used = 0;
@@ -203,11 +224,41 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
         int rc = rebuild_cells(b);
         if (rc) return rc;
     }
-    const int n_mov = b->count[static_cast<size_t>(agg)];
+    const size_t A = static_cast<size_t>(agg);
+    const int n_mov = b->count[A];
     const unsigned long long seq = ++b->seq;
     const int slot = static_cast<int>(seq % mcac_box::kRing);
     SearchArgs a{};
     a.src = static_cast<int>(agg);
+    a.n_mov = n_mov;
+    a.start = b->start[A];
+    // a queued step behind a known closed gate does nothing: no predictions
+    const bool doomed = gated && b->gate_closed;
+    if (b->pend[A] == 0) {
+        a.known = 1;
+        a.sx = b->h_x[A];
+        a.sy = b->h_y[A];
+        a.sz = b->h_z[A];
+        a.srmax = b->h_rmax[A];
+        a.srx = b->h_rx[A];
+        a.sry = b->h_ry[A];
+        a.srz = b->h_rz[A];
+        const double src[3] = {a.sx, a.sy, a.sz};
+        const double vec[3] = {lpm * dir[0], lpm * dir[1], lpm * dir[2]};
+        verlet_window(src, vec, a.srmax + maxradius, static_cast<int>(b->p.n_div), b->p.box_length, a.b1, a.len);
+    }
+    a.dirty_n = static_cast<int>(b->dirty_count);
+    a.dirty_slot = -1;
+    int dirty_new = 0;
+    if (apply && !doomed) {
+        if (!b->h_dirty[A]) {
+            a.dirty_slot = static_cast<int>(b->dirty_count);
+            b->h_dirty[A] = 1;
+            b->dirty_count += 1;
+            dirty_new = 1;
+        }
+        b->pend[A] += 1;
+    }
     a.n_div = static_cast<int>(b->p.n_div);
     a.cap = static_cast<int>(b->cand_cap);
     a.n_built = b->n_built;
@@ -230,25 +281,50 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     __atomic_store_n(&b->h_step[slot].seq, 0ull, __ATOMIC_RELEASE);
     hipEvent_t *ev = b->sev[slot];
     const bool timed = b->timing > 0 && seq % static_cast<unsigned long long>(b->timing) == 0;
+    // one launch when the mover is small enough for the finishing block
+    const bool fused = n_mov <= kStepMover && static_cast<long long>(n_mov) * b->max_count <= kStepPairs;
     if (timed) HIP_TRY(hipEventRecord(ev[0], b->stream));
-    hipLaunchKernelGGL(k_filter, dim3(kFilterBlocks), dim3(kBlock), 0, b->stream, b->d, a);
-    if (timed) HIP_TRY(hipEventRecord(ev[1], b->stream));
-    hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, a);
+    if (fused) {
+        hipLaunchKernelGGL(k_step, dim3(kFilterBlocks), dim3(kBlock), 0, b->stream, b->d, a);
+        if (timed) HIP_TRY(hipEventRecord(ev[1], b->stream));
+    } else {
+        hipLaunchKernelGGL(k_filter, dim3(kFilterBlocks), dim3(kBlock), 0, b->stream, b->d, a);
+        if (timed) HIP_TRY(hipEventRecord(ev[1], b->stream));
+        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, a);
+    }
     if (timed) HIP_TRY(hipEventRecord(ev[2], b->stream));
     if (apply && !a.translate_in_finish) {
         const int tb = std::min(1024, (n_mov + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, a.src, gated, seq);
     }
     if (timed) HIP_TRY(hipEventRecord(ev[3], b->stream));
+    b->slot_args[slot] = a;
+    b->slot_fused[slot] = fused ? 1 : 0;
+    (fused ? b->n_fused : b->n_split) += 1;
     b->slot_timed[slot] = timed ? 1 : 0;
     HIP_TRY(hipGetLastError());
-    if (apply) b->dirty_count += 1;
     b->slot_apply[slot] = apply;
+    b->slot_dirty_new[slot] = dirty_new;
+    b->slot_epoch[slot] = b->dirty_epoch;
+    b->slot_void[slot] = doomed ? 1 : 0;
     b->slot_agg[slot] = agg;
     b->slot_translate[slot] = apply && !a.translate_in_finish;
     b->inflight += 1;
     *ticket = seq;
     return 0;
+}
+
+// Takes the predictions of a queued step that will not run out of the mirrors.
+void take_out(mcac_box *b, int slot) {
+    if (b->slot_void[slot]) return;
+    b->slot_void[slot] = 1;
+    if (!b->slot_apply[slot]) return;
+    const size_t A = static_cast<size_t>(b->slot_agg[slot]);
+    b->pend[A] -= 1;
+    if (b->slot_dirty_new[slot] && b->slot_epoch[slot] == b->dirty_epoch) {
+        b->h_dirty[A] = 0;
+        b->dirty_count -= 1;
+    }
 }
 
 // Adds the event times of a consumed, timed step to the totals.
@@ -276,13 +352,27 @@ int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contac
     const int slot = static_cast<int>(ticket % mcac_box::kRing);
     int rc = wait_seq(b, &b->h_step[slot].seq, ticket);
     if (rc) return rc;
+    bool handed_over = false;
+    if (!b->h_step[slot].skipped && b->h_step[slot].status == kPendingSolve) {
+        handed_over = true;
+        // k_step left its survivors to the multi-block solve; the gate it
+        // closed keeps the queued steps behind it from running meanwhile
+        __atomic_store_n(&b->h_step[slot].seq, 0ull, __ATOMIC_RELEASE);
+        SearchArgs sa = b->slot_args[slot];
+        sa.reopen = sa.gated;
+        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, sa);
+        HIP_TRY(hipGetLastError());
+        b->n_handover += 1;
+        rc = wait_seq(b, &b->h_step[slot].seq, ticket);
+        if (rc) return rc;
+    }
     b->inflight -= 1;
     b->oldest = ticket + 1;
     const StepRecord &s = b->h_step[slot];
     // events are read when the slot is reused or the totals are asked for,
     // so that the host never blocks on them here
     if (b->slot_timed[slot] == 1) b->slot_timed[slot] = s.skipped ? 0 : 2;
-    if (skipped) *skipped = s.skipped;
+    if (skipped) *skipped = s.skipped ? 1 : ((handed_over && b->slot_args[slot].gated && !s.contact) ? 2 : 0);
     if (out) {
         out->distance = s.dist;
         out->move = s.move;
@@ -296,7 +386,22 @@ int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contac
         out->n_filtered = static_cast<int64_t>(s.nfilt);
         out->n_pairs = static_cast<int64_t>(s.npairs);
     }
-    if (!s.skipped && s.dirty_count >= 0) b->dirty_count = std::max<int64_t>(s.dirty_count, 0) + b->inflight;
+    if (s.skipped) {
+        take_out(b, slot);
+    } else if (b->slot_apply[slot]) {
+        // the mover's new CoM, as the finishing block computed it (aggregat.cpp:148-161)
+        const size_t A = static_cast<size_t>(agg);
+        const SearchArgs &sa = b->slot_args[slot];
+        b->h_x[A] = periodic_position(b->h_x[A] + sa.dir[0] * s.move, b->p.box_length);
+        b->h_y[A] = periodic_position(b->h_y[A] + sa.dir[1] * s.move, b->p.box_length);
+        b->h_z[A] = periodic_position(b->h_z[A] + sa.dir[2] * s.move, b->p.box_length);
+        b->pend[A] -= 1;
+    }
+    if (!s.skipped && b->slot_args[slot].gated && (s.contact || handed_over)) {
+        // the steps queued behind this one will be skipped
+        for (unsigned long long t = ticket + 1; t <= b->seq; ++t) take_out(b, static_cast<int>(t % mcac_box::kRing));
+        if (s.contact) b->gate_closed = true;
+    }
     return s.skipped ? 0 : s.status;
 }
 
@@ -379,6 +484,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.members, b->pool_cap);
     ALLOC(d.alive, Na);
     ALLOC(d.cstart, b->n_cells + 1); ALLOC(d.citems, Na); ALLOC(d.ccursor, b->n_cells + 1);
+    ALLOC(d.cpx, Na); ALLOC(d.cpy, Na); ALLOC(d.cpz, Na); ALLOC(d.cpr, Na); ALLOC(d.cslot, Na);
     ALLOC(d.dirty, Na); ALLOC(d.dirty_list, Na + 1);
     ALLOC(d.cand_db, b->cand_cap); ALLOC(d.cand_exact, b->cand_cap); ALLOC(d.cand_rank, b->cand_cap);
     ALLOC(d.cand_id, b->cand_cap); ALLOC(d.order, b->cand_cap); ALLOC(d.cand_pair, b->cand_cap);
@@ -504,6 +610,8 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
     b->n_spheres = ns;
     b->n_aggs = na;
     b->count = cnt;
+    b->max_count = 0;
+    for (int c : b->count) b->max_count = std::max(b->max_count, c);
     b->start = st;
     b->cap = cnt;
     b->alive_ids = alive;
@@ -511,6 +619,17 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
     b->pool_used = static_cast<int64_t>(pool.size());
     b->inflight = 0;
     b->oldest = b->seq + 1;
+    b->h_x = hx;
+    b->h_y = hy;
+    b->h_z = hz;
+    b->h_rx = hrx;
+    b->h_ry = hry;
+    b->h_rz = hrz;
+    b->h_rmax.assign(rmax, rmax + A);
+    b->pend.assign(A, 0);
+    b->h_dirty.assign(A, 0);
+    b->dirty_count = 0;
+    b->gate_closed = false;
     rc = rebuild_cells(b);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -537,12 +656,21 @@ int mcac_translate(mcac_box *b, int64_t agg, const double vec[3]) {
     }
     hipLaunchKernelGGL(k_set_move, dim3(1), dim3(1), 0, b->stream, b->d, static_cast<int>(agg), vec[0], vec[1], vec[2],
                        static_cast<int>(b->p.n_div), b->p.box_length);
+    {
+        const size_t A = static_cast<size_t>(agg);
+        b->h_x[A] = periodic_position(b->h_x[A] + vec[0], b->p.box_length);
+        b->h_y[A] = periodic_position(b->h_y[A] + vec[1], b->p.box_length);
+        b->h_z[A] = periodic_position(b->h_z[A] + vec[2], b->p.box_length);
+        if (!b->h_dirty[A]) {
+            b->h_dirty[A] = 1;
+            b->dirty_count += 1;
+        }
+    }
     const int n = b->count[static_cast<size_t>(agg)];
     const int tb = std::min(1024, std::max(1, (n + kBlock - 1) / kBlock));
     hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, static_cast<int>(agg), 0, 0ull);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(b->stream));
-    b->dirty_count += 1;
     return 0;
 }
 
@@ -628,9 +756,22 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
         b->cap[K] = new_cap;
     }
     b->count[K] = n;
+    b->max_count = std::max(b->max_count, static_cast<int>(n));
     b->count[R] = 0;
     b->dead_in_list += 1;
-    b->dirty_count += 2;
+    // store_merged marks both aggregates as moved since the cell-list build
+    for (size_t id : {K, R})
+        if (!b->h_dirty[id]) {
+            b->h_dirty[id] = 1;
+            b->dirty_count += 1;
+        }
+    b->h_x[K] = out->com[0];
+    b->h_y[K] = out->com[1];
+    b->h_z[K] = out->com[2];
+    b->h_rx[K] = out->rcm[0];
+    b->h_ry[K] = out->rcm[1];
+    b->h_rz[K] = out->rcm[2];
+    b->h_rmax[K] = out->rmax;
     return out->status;
 }
 
@@ -649,6 +790,7 @@ int mcac_move_result(mcac_box *b, uint64_t ticket, mcac_contact *out, int *skipp
 
 int mcac_release(mcac_box *b) {
     if (!b) return MCAC_E_ARGUMENT;
+    b->gate_closed = false;
     hipLaunchKernelGGL(k_release, dim3(1), dim3(1), 0, b->stream, b->d);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -725,6 +867,17 @@ int mcac_box_set_timing(mcac_box *b, int enable) {
     for (int k = 0; k < mcac_box::kRing; ++k) b->slot_timed[k] = 0;
     b->t_search = b->t_translate = b->t_merge = b->t_filter = b->t_solve = 0;
     b->n_search = b->n_translate = b->n_merge = 0;
+    return 0;
+}
+
+int mcac_box_stats(mcac_box *b, int64_t out[6]) {
+    if (!b || !out) return MCAC_E_ARGUMENT;
+    out[0] = b->n_fused;
+    out[1] = b->n_split;
+    out[2] = b->n_handover;
+    out[3] = b->n_rebuilds;
+    out[4] = b->n_compactions;
+    out[5] = b->pool_used;
     return 0;
 }
 
