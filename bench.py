@@ -48,7 +48,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--timing-stride", type=int, default=8,
+    ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -68,7 +68,8 @@ def main() -> int:
     box = sim.box()
     box.set_timing(max(1, args.timing_stride))
     s0 = {k: sim.scalar(k) for k in ("query", "filtered", "pairs", "merges", "t_host_pick_ms", "t_host_merge_ms",
-                                     "t_device_ms", "pick_rescales", "ties")}
+                                     "t_device_ms", "pick_rescales", "ties", "t_tie_ms", "t_enqueue_ms", "t_wait_ms",
+                                     "t_merge_call_ms", "t_drain_ms")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done = sim.run(args.steps)
@@ -117,6 +118,8 @@ def main() -> int:
         "breakdown": {"init_s": t_init, "host_pick_ms_per_step": d["t_host_pick_ms"] / args.steps,
                       "host_merge_ms_per_step": d["t_host_merge_ms"] / args.steps,
                       "pick_rescales": d["pick_rescales"], "pick_ties": d["ties"],
+                      "host_us_per_step": {k[2:-3]: 1e3 * d[k] / args.steps for k in
+                                           ("t_tie_ms", "t_enqueue_ms", "t_wait_ms", "t_merge_call_ms", "t_drain_ms")},
                       "device_call_ms_per_step": d["t_device_ms"] / args.steps,
                       "search_launch1_ms": filt_ms, "search_launch2_ms": solve_ms,
                       "k_translate_ms": tm["translate_ms"] / max(tm["n_translate"], 1.0),

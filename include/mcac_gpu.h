@@ -130,6 +130,13 @@ int mcac_move_enqueue(mcac_box *box, int64_t agg, const double dir[3], double lp
 int mcac_move_result(mcac_box *box, uint64_t ticket, mcac_contact *out, int *skipped);
 /* Reopens the gate closed by a step that found a contact. */
 int mcac_release(mcac_box *box);
+/* With auto-merge on, a queued move that finds a contact also merges the two
+ * aggregates in the same launch when the merged aggregate has at most 256
+ * spheres and its member segment fits (calcul.cpp:156-160 always merges on
+ * contact); mcac_merge then returns that result without a launch.  The
+ * caller must call mcac_merge for such a contact before enqueueing further
+ * moves.  Off by default; set while no move is queued. */
+int mcac_box_set_auto_merge(mcac_box *box, int enable);
 
 int mcac_box_download_spheres(mcac_box *box, double *x, double *y, double *z, double *rx, double *ry,
                               double *rz);
@@ -148,8 +155,9 @@ int mcac_box_timing(mcac_box *box, double out[8]);
 /* Launch statistics: [searches in one launch (k_step), searches in two
  * launches (k_filter + k_solve), one-launch searches finished by a second
  * launch, cell-list rebuilds, member-pool compactions, member-pool slots in
- * use].  No reference counterpart (diagnostics). */
-int mcac_box_stats(mcac_box *box, int64_t out[6]);
+ * use, merges done inside a queued move's launch, merges in their own
+ * launch].  No reference counterpart (diagnostics). */
+int mcac_box_stats(mcac_box *box, int64_t out[8]);
 
 /* distance_to_contact(Sphere, Sphere, dir, dist) (src/spheres/sphere_contact.cpp:47-125)
  * evaluated on the device for n independent inputs: p1, p2, dir are (n,3)

@@ -57,7 +57,8 @@ int64_t mcac_sim_n_merges(mcac_sim *sim);
 int64_t mcac_sim_get_merges(mcac_sim *sim, mcac_merge_rec *out, int64_t max);
 /* Named scalars: box_length, time, maxradius, n_agg, n_sph, avg_npp,
  * max_time_step, steps, merges, events, total_volume, volume_fraction, query,
- * filtered, pairs, ties, t_host_pick_ms, t_host_merge_ms, t_device_ms, init_ms,
+ * filtered, pairs, ties, t_host_pick_ms, t_host_merge_ms, t_device_ms, t_tie_ms,
+ * t_enqueue_ms, t_wait_ms, t_merge_call_ms, t_drain_ms, init_ms,
  * pick_exact_steps, pick_chunks, pick_rescales. NaN for unknown names. */
 double mcac_sim_scalar(mcac_sim *sim, const char *name);
 /* Host view of the live aggregates in label order (stable ids). */

@@ -51,7 +51,7 @@ MERGE_RESULT_DTYPE = np.dtype([
 # Every symbol declared in include/mcac_gpu.h and include/mcac_sim.h.
 GPU_SYMBOLS = (
     "mcac_box_create", "mcac_box_destroy", "mcac_box_upload", "mcac_next_contact", "mcac_translate",
-    "mcac_move", "mcac_merge", "mcac_move_enqueue", "mcac_move_result", "mcac_release",
+    "mcac_move", "mcac_merge", "mcac_move_enqueue", "mcac_move_result", "mcac_release", "mcac_box_set_auto_merge",
     "mcac_box_download_spheres", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_swept_sphere_batch", "mcac_version",
 )
@@ -105,6 +105,7 @@ def load_gpu():
         L.mcac_move_enqueue.argtypes = [_P, _I64, _DP, _D, _D, ctypes.POINTER(ctypes.c_uint64)]
         L.mcac_move_result.argtypes = [_P, ctypes.c_uint64, _P, ctypes.POINTER(_I)]
         L.mcac_release.argtypes = [_P]
+        L.mcac_box_set_auto_merge.argtypes = [_P, _I]
         L.mcac_box_download_spheres.argtypes = [_P] + [_DP] * 6
         L.mcac_box_download_aggs.argtypes = [_P] + [_DP] * 6 + [_I64P]
         L.mcac_box_members.argtypes = [_P, _I64, _I64P, _I64, _I64P]
@@ -298,10 +299,15 @@ class Box:
         return dict(zip(keys, out.tolist()))
 
 
+    def set_auto_merge(self, enable: bool):
+        """Queued moves that find a contact merge in the same launch (small aggregates)."""
+        _check(load_gpu().mcac_box_set_auto_merge(self.h, int(enable)), "mcac_box_set_auto_merge")
+
     def stats(self):
-        out = np.zeros(6, np.int64)
+        out = np.zeros(8, np.int64)
         _check(load_gpu().mcac_box_stats(self.h, _ip(out)), "mcac_box_stats")
-        keys = ("one_launch", "two_launch", "handover", "cell_rebuilds", "pool_compactions", "pool_used")
+        keys = ("one_launch", "two_launch", "handover", "cell_rebuilds", "pool_compactions", "pool_used",
+                "inline_merges", "launched_merges")
         return dict(zip(keys, (int(v) for v in out)))
 
 

@@ -11,7 +11,7 @@ constexpr int kBlock = 256;            // threads per block (4 waves of 64)
 constexpr int kFilterBlocks = 128;     // contact-search filter grid
 constexpr int kSolveBlocks = 256;      // pair-solve grid (one block per CU)
 constexpr long long kChunk = 4096;     // minimum sphere pairs per solve chunk
-constexpr int kMaxVisit = 4096;        // visited cells handled with an LDS prefix
+constexpr int kMaxVisit = 2048;        // visited cells handled with an LDS prefix
 constexpr int kMaxPlan = 4096;         // filtered candidates planned in LDS
 constexpr int kOverlapSlots = 16;      // contacts kept per sphere row (multi-kernel merge)
 constexpr int kSmallMerge = 256;       // merged aggregates up to this size use one fused kernel
@@ -42,6 +42,8 @@ struct StepRecord {
     int contact, status;
     unsigned long long nquery, nfilt, npairs;
     int skipped, dirty_count;  // step skipped behind a closed gate; dirty list length after it
+    mcac_merge_result merge;   // merge done in the same launch (merged_inline)
+    int merged_inline, m_new_start, m_new_cap, pad;
     unsigned long long seq;    // written last (system-scope release)
 };
 
@@ -67,6 +69,7 @@ struct DevBox {
     double *ax, *ay, *az, *arx, *ary, *arz, *armax, *avol, *asurf, *arg;
     double *csvx, *csvy, *csvz, *csr, *csvr2;  // list-order partial sums the merge continues
     int *astart, *acount, *acell;              // member segment, size, reference Verlet cell (-1 dead)
+    int *acap;                                 // member segment capacity
     int *members;
     int *alive;                                // live ids, ascending (may hold dead ids)
     // cell list: CSR by reference cell, built from the live ids
@@ -110,12 +113,16 @@ struct SearchArgs {
     StepRecord *out;           // pinned host record
     unsigned long long seq;
     int gated;                 // pipelined step: closes the gate on contact, skipped behind it
-    int reopen;                // second-launch solve of a gated step: reopens the gate without contact
+    unsigned long long released;  // gate values up to this one are open for this step
+    int automerge;             // merge in the same launch on contact when small enough
+    int pad2;
+    long long pool_used, pool_cap;  // member pool state when this step runs
 };
 
 struct MergeArgs {
     long long ms, os, magg, oagg;
     int keep, rem, n_k, n_r, new_start, start_k, start_r, n_div;
+    int cap_k, pad;            // keeper's segment capacity after the merge
     double L;
     MergeRecord *out;          // pinned host record
     unsigned long long seq;

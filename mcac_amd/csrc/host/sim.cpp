@@ -129,6 +129,7 @@ struct Simulation {
     std::vector<mcac_merge_rec> merge_recs;
     double q_query = 0, q_filtered = 0, q_pairs = 0, n_ties = 0;
     double t_pick = 0, t_merge_host = 0, t_device = 0, t_init = 0;
+    double t_tie = 0, t_enqueue = 0, t_wait = 0, t_merge_call = 0, t_drain = 0;
     bool initialized = false;
     bool final_written = false;
 
@@ -304,6 +305,8 @@ struct Simulation {
                              ssurf.data(), zero.data(), zero.data(), zero.data(), static_cast<int64_t>(N), pos3.data(),
                              rcm3.data(), rmax.data(), vol.data(), surf.data(), cnt.data(), mem.data());
         if (rc) return rc;
+        rc = mcac_box_set_auto_merge(box, 1);  // every contact merges (calcul.cpp:156-160)
+        if (rc) return rc;
         pick.build(live.ids(), ts);  // first sort_time_steps (calcul.cpp:99-101)
         event = true;
         initialized = true;
@@ -412,12 +415,16 @@ struct Simulation {
         bool tie = false;
         if (num < 0) {
             tie = true;
+            auto tt0 = Clock::now();
             num = pick.resolve_tie(tie_pos, live.ids(), ts);
+            t_tie += ms_since(tt0);
         }
         const double full = lpm[static_cast<size_t>(num)];
         t_pick += ms_since(t0);
         uint64_t ticket = 0;
+        auto te0 = Clock::now();
         int rc = mcac_move_enqueue(box, num, d.dir.data(), full, maxradius, &ticket);
+        t_enqueue += ms_since(te0);
         if (rc) return rc;
         flight.push_back({ticket, num, tie, full, d});
         return 0;
@@ -448,6 +455,7 @@ struct Simulation {
         auto td0 = Clock::now();
         int rc = mcac_move_result(box, P.ticket, &c, &skipped);
         t_device += ms_since(td0);
+        t_wait += ms_since(td0);
         if (rc) return rc;
         flight.pop_front();
         if (skipped == 1) return drain_skipped();
@@ -477,12 +485,15 @@ struct Simulation {
         pm.time = pm.time + deltatemps;
         bool merged = false;
         if (contact_flag) {
+            auto tdr0 = Clock::now();
             rc = drain_skipped();
+            t_drain += ms_since(tdr0);
             if (rc) return rc;
             mcac_merge_result mr{};
             auto tm0 = Clock::now();
             rc = mcac_merge(box, &c, &mr);
             t_device += ms_since(tm0);
+            t_merge_call += ms_since(tm0);
             if (rc) return rc;
             if (mr.merged) {
                 merged = true;
@@ -702,6 +713,11 @@ double mcac_sim_scalar(mcac_sim *sim, const char *name) {
     if (n == "t_host_pick_ms") return s.t_pick;
     if (n == "t_host_merge_ms") return s.t_merge_host;
     if (n == "t_device_ms") return s.t_device;
+    if (n == "t_tie_ms") return s.t_tie;
+    if (n == "t_enqueue_ms") return s.t_enqueue;
+    if (n == "t_wait_ms") return s.t_wait;
+    if (n == "t_merge_call_ms") return s.t_merge_call;
+    if (n == "t_drain_ms") return s.t_drain;
     if (n == "init_ms") return s.t_init;
     if (n == "total_volume") {
         double t = 0.0;

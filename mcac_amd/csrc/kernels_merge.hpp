@@ -72,6 +72,7 @@ __device__ __forceinline__ void store_merged(const DevBox &b, const MergeArgs &a
     b.az[k] = com[2];
     b.acell[k] = ref_cell(com[0], com[1], com[2], a.n_div, a.L);
     b.astart[k] = a.new_start;
+    b.acap[k] = a.cap_k;
     b.acount[k] = a.n_k + a.n_r;
     b.acell[a.rem] = -1;
     b.acount[a.rem] = 0;
@@ -87,7 +88,9 @@ __device__ __forceinline__ void publish_merge(const MergeArgs &a, const mcac_mer
 }
 
 // ------------------------------------------------------------------ fused merge (n <= kSmallMerge)
-__global__ __launch_bounds__(kBlock) void k_merge_small(DevBox b, MergeArgs a) {
+// One block merges; thread 0 ends with the result in res.  Used by
+// k_merge_small and by the finishing block of a step that found a contact.
+__device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeArgs &a, mcac_merge_result &res_out) {
     __shared__ double diffpos[3], refpos[3], r[3];
     __shared__ int ok;
     __shared__ double dcs[kSmallMerge], vols[kSmallMerge], rads[kSmallMerge];
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_small(DevBox b, MergeArgs a) {
             res.keeper = a.keep;
             res.removed = a.rem;
             res.merged = 0;
-            publish_merge(a, res);
+            res_out = res;
         }
         return;
     }
@@ -221,8 +224,14 @@ __global__ __launch_bounds__(kBlock) void k_merge_small(DevBox b, MergeArgs a) {
         res.overlapping = overlapping;
         res.coordination = coord;
         if (row_over) res.status = MCAC_E_CAPACITY;
-        publish_merge(a, res);
+        res_out = res;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_small(DevBox b, MergeArgs a) {
+    __shared__ mcac_merge_result res;
+    merge_small_block(b, a, res);
+    if (threadIdx.x == 0) publish_merge(a, res);
 }
 
 // ------------------------------------------------------------------ multi-kernel merge

@@ -16,6 +16,7 @@
 // k_translate moves the spheres of a large mover.
 #pragma once
 #include "kernels_common.hpp"
+#include "kernels_merge.hpp"
 
 namespace mcac_dev {
 
@@ -34,11 +35,13 @@ __device__ __forceinline__ void append_candidate(const DevBox &b, const SearchAr
 constexpr unsigned long long kAppendOne = 1ull << 32;
 constexpr unsigned long long kTicketOne = 1ull << 56;
 
-// A gated step enqueued behind a step that found a contact does nothing.
+// A gated step enqueued behind a step that found a contact (or handed its
+// solve over) does nothing; the host releases a gate value by passing it as
+// `released` to the steps it enqueues afterwards.
 __device__ __forceinline__ bool behind_gate(const DevBox &b, const SearchArgs &a) {
     if (!a.gated) return false;
     const unsigned long long g = b.ss->gate;
-    return g != 0ull && g < a.seq;
+    return g != 0ull && g < a.seq && g > a.released;
 }
 
 // Filter work of one block: the aggregates of the reference's visited Verlet
@@ -113,7 +116,7 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
         block_exclusive_scan(pre, V);
         total = pre[V];
     } else {
-        total = a.n_built;
+        total = b.cstart[n * n * n];  // every entry of the build
     }
     // one index space: cell entries, then the aggregates moved since the build
     // (at their current cell), so that both kinds run side by side
@@ -228,6 +231,7 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
     }
     __syncthreads();
     __shared__ double s_ref[3];
+    __shared__ StepRecord srec;  // assembled here, copied to host memory by one wave
     SearchState *s = b.ss;
     if (threadIdx.x == 0) {
         double closest = kInf;
@@ -268,7 +272,7 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
             s_ref[l] = s->refpos[l];
         }
         s->newcell = ref_cell(s->newcom[0], s->newcom[1], s->newcom[2], a.n_div, a.L);
-        StepRecord *o = a.out;
+        StepRecord *o = &srec;
         o->dist = closest;
         o->move = move;
         o->ms = ms;
@@ -293,11 +297,12 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
                 if (t.src_slot >= 0) b.cpr[t.src_slot] = -1.0;
             }
         }
-        if (a.gated) {
-            if (contact) s->gate = a.seq;     // the steps behind wait for the merge
-            else if (a.reopen) s->gate = 0ull;  // second-launch solve without contact
-        }
+        if (a.gated && contact) s->gate = a.seq;  // the steps behind wait for the merge
         o->skipped = 0;
+        o->merged_inline = 0;
+        o->m_new_start = 0;
+        o->m_new_cap = 0;
+        o->pad = 0;
         o->dirty_count = a.dirty_n + ((a.apply && a.dirty_slot >= 0) ? 1 : 0);
         s->nfilt = 0;
         s->nquery = 0;
@@ -321,9 +326,70 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
             b.sz[sp] = r2 + b.srz[sp];
         }
     }
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    // merge in the same launch (AggregatList::merge, aggregat_list.cpp:367-410)
+    // when the merged aggregate is small and its member segment fits
+    if (a.automerge && a.apply && a.translate_in_finish) {
+        __shared__ MergeArgs s_ma;
+        __shared__ int s_inline;
+        __syncthreads();  // the mover's spheres are in place
+        if (threadIdx.x == 0) {
+            s_inline = 0;
+            if (srec.contact) {
+                const int oagg = static_cast<int>(srec.oagg);
+                const int keep = a.src < oagg ? a.src : oagg, rem = a.src < oagg ? oagg : a.src;
+                const int nk = (keep == a.src) ? a.n_mov : b.acount[keep];
+                const int nr = (rem == a.src) ? a.n_mov : b.acount[rem];
+                const int start_k = (keep == a.src) ? a.start : b.astart[keep];
+                const int start_r = (rem == a.src) ? a.start : b.astart[rem];
+                const int n = nk + nr;
+                int new_start = start_k, new_cap = b.acap[keep];
+                bool fits = n <= kSmallMerge;
+                if (fits && n > new_cap) {  // a fresh segment of twice the merged size
+                    new_cap = (2 * n > 4) ? 2 * n : 4;
+                    fits = a.pool_used + new_cap <= a.pool_cap;
+                    new_start = static_cast<int>(a.pool_used);
+                }
+                if (fits) {
+                    MergeArgs m{};
+                    m.ms = srec.ms;
+                    m.os = srec.os;
+                    m.magg = a.src;
+                    m.oagg = oagg;
+                    m.keep = keep;
+                    m.rem = rem;
+                    m.n_k = nk;
+                    m.n_r = nr;
+                    m.new_start = new_start;
+                    m.start_k = start_k;
+                    m.start_r = start_r;
+                    m.n_div = a.n_div;
+                    m.cap_k = new_cap;
+                    m.L = a.L;
+                    s_ma = m;
+                    s_inline = 1;
+                    srec.m_new_start = new_start;
+                    srec.m_new_cap = new_cap;
+                }
+            }
+        }
+        __syncthreads();
+        if (s_inline) {
+            merge_small_block(b, s_ma, srec.merge);
+            if (threadIdx.x == 0) srec.merged_inline = 1;
+        }
+        __syncthreads();
+    }
+    // the record body in one coalesced write, then the sequence number
+    constexpr int kWords = static_cast<int>(offsetof(StepRecord, seq) / sizeof(unsigned long long));
+    static_assert(offsetof(StepRecord, seq) % sizeof(unsigned long long) == 0, "record body in 8-byte words");
+    if (threadIdx.x < 64) {
+        if (static_cast<int>(threadIdx.x) < kWords)
+            reinterpret_cast<unsigned long long *>(a.out)[threadIdx.x] =
+                reinterpret_cast<const unsigned long long *>(&srec)[threadIdx.x];
+        if (threadIdx.x == 0) {
+            __threadfence_system();
+            __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -542,8 +608,6 @@ __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
     }
     finish_step(b, a, F, nsrc, ssrc, t, mover);
 }
-
-__global__ void k_release(DevBox b) { b.ss->gate = 0ull; }
 
 // a13 for a large mover: sphere positions from the refpos of the last search.
 __global__ __launch_bounds__(kBlock) void k_translate(DevBox b, int src, int gated, unsigned long long seq) {

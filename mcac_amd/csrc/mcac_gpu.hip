@@ -48,6 +48,14 @@ struct mcac_box {
     int64_t dirty_count = 0;         // device dirty-list length after the queued steps
     int64_t dirty_epoch = 0;         // cell-list builds so far
     bool gate_closed = false;        // a consumed queued step found a contact; not released yet
+    unsigned long long released = 0; // gate values the steps enqueued from now on ignore
+    unsigned long long last_gate = 0;  // seq of the last consumed step that closed the gate
+    int automerge = 0;               // queued moves merge in the same launch on contact
+    bool inline_pending = false;     // a contact merged in its launch; mcac_merge not called yet
+    StepRecord inline_rec{};
+    int64_t inline_src = -1;
+    int64_t inline_epoch = 0;
+    int64_t n_inline = 0, n_host_merge = 0;
     int64_t dead_in_list = 0;
     int64_t pool_used = 0;
     // pinned records (host pointer, device pointer)
@@ -202,6 +210,7 @@ int compact_pool(mcac_box *b, int64_t need) {
         b->cap[id] = static_cast<int>(std::max<int64_t>(4, 2 * static_cast<int64_t>(b->count[id])));
     }
     HIP_TRY(hipMemcpy(b->d.astart, b->start.data(), b->start.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d.acap, b->cap.data(), b->cap.size() * sizeof(int), hipMemcpyHostToDevice));
     for (auto &p : b->allocs)
         if (p == b->d.members) p = nm;
     HIP_TRY(hipFree(b->d.members));
@@ -220,7 +229,8 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
                    unsigned long long *ticket) {
     if (agg < 0 || agg >= b->n_aggs || b->count[static_cast<size_t>(agg)] <= 0) return MCAC_E_ARGUMENT;
     if (b->inflight >= mcac_box::kRing) return MCAC_E_CAPACITY;
-    if (b->dirty_count + 2 >= rebuild_threshold(b)) {
+    if (b->inline_pending) return MCAC_E_ARGUMENT;  // the caller merges the contact first
+    if (b->dirty_count + 3 >= rebuild_threshold(b)) {
         int rc = rebuild_cells(b);
         if (rc) return rc;
     }
@@ -274,6 +284,10 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     a.out = b->d_step + slot;
     a.seq = seq;
     a.gated = gated;
+    a.released = b->released;
+    a.automerge = (b->automerge && gated && apply) ? 1 : 0;
+    a.pool_used = b->pool_used;
+    a.pool_cap = b->pool_cap;
     if (b->slot_timed[slot] == 2) {
         int rc = harvest_timing(b, slot);
         if (rc) return rc;
@@ -359,7 +373,6 @@ int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contac
         // closed keeps the queued steps behind it from running meanwhile
         __atomic_store_n(&b->h_step[slot].seq, 0ull, __ATOMIC_RELEASE);
         SearchArgs sa = b->slot_args[slot];
-        sa.reopen = sa.gated;
         hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, sa);
         HIP_TRY(hipGetLastError());
         b->n_handover += 1;
@@ -400,7 +413,18 @@ int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contac
     if (!s.skipped && b->slot_args[slot].gated && (s.contact || handed_over)) {
         // the steps queued behind this one will be skipped
         for (unsigned long long t = ticket + 1; t <= b->seq; ++t) take_out(b, static_cast<int>(t % mcac_box::kRing));
-        if (s.contact) b->gate_closed = true;
+        if (s.contact) {
+            b->gate_closed = true;
+            b->last_gate = ticket;
+        } else {
+            b->released = ticket;  // solved in the second launch without contact: open again
+        }
+    }
+    if (!s.skipped && s.contact && s.merged_inline) {
+        b->inline_pending = true;
+        b->inline_rec = s;
+        b->inline_src = agg;
+        b->inline_epoch = b->slot_epoch[slot];
     }
     return s.skipped ? 0 : s.status;
 }
@@ -480,7 +504,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.ax, Na); ALLOC(d.ay, Na); ALLOC(d.az, Na); ALLOC(d.arx, Na); ALLOC(d.ary, Na); ALLOC(d.arz, Na);
     ALLOC(d.armax, Na); ALLOC(d.avol, Na); ALLOC(d.asurf, Na); ALLOC(d.arg, Na);
     ALLOC(d.csvx, Na); ALLOC(d.csvy, Na); ALLOC(d.csvz, Na); ALLOC(d.csr, Na); ALLOC(d.csvr2, Na);
-    ALLOC(d.astart, Na); ALLOC(d.acount, Na); ALLOC(d.acell, Na);
+    ALLOC(d.astart, Na); ALLOC(d.acount, Na); ALLOC(d.acell, Na); ALLOC(d.acap, Na);
     ALLOC(d.members, b->pool_cap);
     ALLOC(d.alive, Na);
     ALLOC(d.cstart, b->n_cells + 1); ALLOC(d.citems, Na); ALLOC(d.ccursor, b->n_cells + 1);
@@ -602,6 +626,7 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
         HIP_TRY(hipMemcpy(b->d.astart, st.data(), A * sizeof(int), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(b->d.acount, cnt.data(), A * sizeof(int), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(b->d.acell, cell.data(), A * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d.acap, cnt.data(), A * sizeof(int), hipMemcpyHostToDevice));
     }
     if (!pool.empty()) HIP_TRY(hipMemcpy(b->d.members, pool.data(), pool.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!alive.empty()) HIP_TRY(hipMemcpy(b->d.alive, alive.data(), alive.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -630,6 +655,9 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
     b->h_dirty.assign(A, 0);
     b->dirty_count = 0;
     b->gate_closed = false;
+    b->released = b->seq;
+    b->last_gate = 0;
+    b->inline_pending = false;
     rc = rebuild_cells(b);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -674,9 +702,58 @@ int mcac_translate(mcac_box *b, int64_t agg, const double vec[3]) {
     return 0;
 }
 
+// Host bookkeeping after a merge ran on the device (kernel or inline).
+static int merge_done(mcac_box *b, size_t K, size_t R, int new_start, int new_cap, bool dirty_now,
+                      mcac_merge_result *out) {
+    if (!out->merged) {
+        out->keeper = static_cast<int64_t>(K);
+        out->removed = static_cast<int64_t>(R);
+        return 0;
+    }
+    if (new_start != b->start[K]) {
+        b->pool_used += new_cap;
+        b->start[K] = new_start;
+        b->cap[K] = new_cap;
+    }
+    const int n = static_cast<int>(out->n);
+    b->count[K] = n;
+    b->max_count = std::max(b->max_count, n);
+    b->count[R] = 0;
+    b->dead_in_list += 1;
+    // store_merged marks both aggregates as moved since the cell-list build
+    // (unless a build ran after the merge on the device)
+    if (dirty_now)
+        for (size_t id : {K, R})
+            if (!b->h_dirty[id]) {
+                b->h_dirty[id] = 1;
+                b->dirty_count += 1;
+            }
+    b->h_x[K] = out->com[0];
+    b->h_y[K] = out->com[1];
+    b->h_z[K] = out->com[2];
+    b->h_rx[K] = out->rcm[0];
+    b->h_ry[K] = out->rcm[1];
+    b->h_rz[K] = out->rcm[2];
+    b->h_rmax[K] = out->rmax;
+    return out->status;
+}
+
 int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     if (!b || !c || !out) return MCAC_E_ARGUMENT;
     std::memset(out, 0, sizeof(*out));
+    if (b->inline_pending) {
+        // the contact step already merged in its own launch
+        const StepRecord &r = b->inline_rec;
+        if (c->moving_agg != b->inline_src || c->other_agg != r.oagg || c->moving_sphere != r.ms ||
+            c->other_sphere != r.os)
+            return MCAC_E_ARGUMENT;
+        b->inline_pending = false;
+        b->n_inline += 1;
+        *out = r.merge;
+        const size_t K = static_cast<size_t>(std::min(c->moving_agg, c->other_agg));
+        const size_t R = static_cast<size_t>(std::max(c->moving_agg, c->other_agg));
+        return merge_done(b, K, R, r.m_new_start, r.m_new_cap, b->inline_epoch == b->dirty_epoch, out);
+    }
     if (c->moving_sphere < 0 || c->other_sphere < 0 || c->moving_sphere >= b->n_spheres ||
         c->other_sphere >= b->n_spheres || c->moving_agg < 0 || c->other_agg < 0 || c->moving_agg >= b->n_aggs ||
         c->other_agg >= b->n_aggs || c->moving_agg == c->other_agg)
@@ -714,6 +791,7 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     a.new_start = new_start;
     a.start_k = b->start[K];
     a.start_r = b->start[R];
+    a.cap_k = new_cap;
     a.n_div = static_cast<int>(b->p.n_div);
     a.L = b->p.box_length;
     a.out = b->d_merge;
@@ -745,34 +823,8 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
         b->n_merge += 1;
     }
     *out = b->h_merge->res;
-    if (!out->merged) {
-        out->keeper = keep;
-        out->removed = rem;
-        return 0;
-    }
-    if (new_start != b->start[K]) {
-        b->pool_used += new_cap;
-        b->start[K] = new_start;
-        b->cap[K] = new_cap;
-    }
-    b->count[K] = n;
-    b->max_count = std::max(b->max_count, static_cast<int>(n));
-    b->count[R] = 0;
-    b->dead_in_list += 1;
-    // store_merged marks both aggregates as moved since the cell-list build
-    for (size_t id : {K, R})
-        if (!b->h_dirty[id]) {
-            b->h_dirty[id] = 1;
-            b->dirty_count += 1;
-        }
-    b->h_x[K] = out->com[0];
-    b->h_y[K] = out->com[1];
-    b->h_z[K] = out->com[2];
-    b->h_rx[K] = out->rcm[0];
-    b->h_ry[K] = out->rcm[1];
-    b->h_rz[K] = out->rcm[2];
-    b->h_rmax[K] = out->rmax;
-    return out->status;
+    b->n_host_merge += 1;
+    return merge_done(b, K, R, new_start, new_cap, true, out);
 }
 
 int mcac_move_enqueue(mcac_box *b, int64_t agg, const double dir[3], double lpm, double maxradius, uint64_t *ticket) {
@@ -790,9 +842,15 @@ int mcac_move_result(mcac_box *b, uint64_t ticket, mcac_contact *out, int *skipp
 
 int mcac_release(mcac_box *b) {
     if (!b) return MCAC_E_ARGUMENT;
+    // the steps enqueued from now on pass the gate value of the contact
     b->gate_closed = false;
-    hipLaunchKernelGGL(k_release, dim3(1), dim3(1), 0, b->stream, b->d);
-    HIP_TRY(hipGetLastError());
+    b->released = std::max(b->released, b->last_gate);
+    return 0;
+}
+
+int mcac_box_set_auto_merge(mcac_box *b, int enable) {
+    if (!b || b->inflight) return MCAC_E_ARGUMENT;
+    b->automerge = enable ? 1 : 0;
     return 0;
 }
 
@@ -870,8 +928,10 @@ int mcac_box_set_timing(mcac_box *b, int enable) {
     return 0;
 }
 
-int mcac_box_stats(mcac_box *b, int64_t out[6]) {
+int mcac_box_stats(mcac_box *b, int64_t out[8]) {
     if (!b || !out) return MCAC_E_ARGUMENT;
+    out[6] = b->n_inline;
+    out[7] = b->n_host_merge;
     out[0] = b->n_fused;
     out[1] = b->n_split;
     out[2] = b->n_handover;

@@ -153,13 +153,16 @@ def test_c3_full_size_invariants():
     assert np.all(st["dist"][st["contact"] == 1] <= st["lpm"][st["contact"] == 1])
 
 
-def test_queued_moves_match_synchronous_moves():
-    """Gated queued moves (three in flight) give the records and positions of one-at-a-time moves."""
-    say("case: queued moves vs synchronous moves, lognormal_small after 3000 steps")
+@pytest.mark.parametrize("auto_merge", [False, True])
+def test_queued_moves_match_synchronous_moves(auto_merge):
+    """Gated queued moves (three in flight) give the records and positions of one-at-a-time moves;
+    with auto-merge the contact's merge runs in the move's launch and matches a launched merge."""
+    say(f"case: queued moves vs synchronous moves, lognormal_small after 3000 steps, auto_merge={auto_merge}")
     sim = O.OracleSim((CFG / "lognormal_small.ini").read_text(), fast_init=True, trace=False)
     sim.run(3000)
     a, sph, agg = upload_oracle_state(sim)
     b, _, _ = upload_oracle_state(sim)
+    b.set_auto_merge(auto_merge)
     maxr = sim.scalar("maxradius")
     rng = np.random.default_rng(5)
     live = list(range(len(agg["n"])))
@@ -226,6 +229,12 @@ def test_queued_moves_match_synchronous_moves():
     sa, sb = a.spheres(), b.spheres()
     for k in sa:
         assert np.array_equal(sa[k], sb[k]), k
+    ga, gb = a.aggregates(), b.aggregates()
+    for k in ga:
+        assert np.array_equal(ga[k], gb[k]), k
+    st = b.stats()
+    say("stats", st)
+    assert (st["inline_merges"] > 0) == auto_merge
 
 
 def test_one_launch_handover_bit_exact():
