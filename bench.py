@@ -22,6 +22,7 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "k_step_n1m_hbm_traffic.json"
 WORKLOAD_INI = ROOT / "mcac_amd" / "configs" / "n1m_lognormal_fv3e-3.ini"
 METRIC = "MC steps/s at N=1M; ensemble runs/hour + p99 latency vs plain GPU & CPU"
 
@@ -48,6 +49,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 20 times (HBM counter calibration)")
     ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
     args = ap.parse_args()
@@ -55,13 +58,25 @@ def main() -> int:
         print("bench.py measures one GPU: this path runs one aggregation box per GPU "
               "(independent boxes shard across GPUs with no exchange); use --gpus 1", file=sys.stderr)
         return 2
+    print("bench: importing torch", file=sys.stderr, flush=True)
     import torch
+    if args.copy_calibration:
+        n = 256 << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n, "copy_calibration_bytes_written": n, "launches": 20}))
+        return 0
     import mcac_amd
 
     text = WORKLOAD_INI.read_text()
+    print("bench: placing N=1e6 monomers", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     sim = mcac_amd.Simulation(text, device=0)
     t_init = time.perf_counter() - t0
+    print("bench: warm-up", file=sys.stderr, flush=True)
     done = sim.run(args.warmup)
     if done != args.warmup:
         raise RuntimeError("warm-up ended early")
@@ -87,6 +102,9 @@ def main() -> int:
     # large movers), HBM-bound.  Algorithmic bytes per launch: 36 B per
     # aggregate of the visited Verlet cells (CoM x,y,z + rmax 8 B each, 4 B id).
     one_launch = st["one_launch"] >= st["two_launch"]
+    traffic = None
+    if TRAFFIC_FILE.exists():  # rocprofv3 FETCH_SIZE + WRITE_SIZE per k_step dispatch, copy-calibrated
+        traffic = json.loads(TRAFFIC_FILE.read_text()).get("k_step_bytes_per_dispatch")
     n_search = max(tm["n_search"], 1.0)
     if tm["n_search"] < 1:
         raise RuntimeError("no timed contact search: lower --timing-stride")
@@ -112,7 +130,7 @@ def main() -> int:
                                f"MC steps {args.warmup}..{args.warmup + args.steps} from t=0",
                    "n_spheres": 1_000_000, "parallelism": "single box, 1 GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_step" if one_launch else "k_filter", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": filt_ms},
         "candidate_pair_tests_per_s": pair_tests / elapsed,
         "breakdown": {"init_s": t_init, "host_pick_ms_per_step": d["t_host_pick_ms"] / args.steps,

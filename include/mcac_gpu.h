@@ -145,11 +145,13 @@ int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rm
 /* Members of one aggregate in list order; returns the count via *n. */
 int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int64_t *n);
 
-/* Kernel timing on the box's stream (HIP events).  enable = n >= 1 records
- * events around every n-th search (every merge when n >= 1), 0 turns timing
- * off; the host never blocks on the events of a search.  mcac_box_timing
- * returns accumulated milliseconds and launch counts: [search_ms,
- * translate_ms, merge_ms, n_search, n_translate, n_merge, filter_ms, solve_ms]. */
+/* Kernel timing on the box's stream.  enable = n >= 1 times every n-th
+ * search with start/stop events stamped by the runtime at the kernel's own
+ * start and end (hipExtLaunchKernel), and every merge launch with events
+ * around it; 0 turns timing off.  The host never blocks on the events of a
+ * search.  mcac_box_timing returns accumulated milliseconds and counts:
+ * [search_ms, translate_ms (not timed, 0), merge_ms, n_search, n_translate,
+ * n_merge, first-launch ms (k_step or k_filter), second-launch ms (k_solve)]. */
 int mcac_box_set_timing(mcac_box *box, int enable);
 int mcac_box_timing(mcac_box *box, double out[8]);
 /* Launch statistics: [searches in one launch (k_step), searches in two

@@ -18,6 +18,7 @@ constexpr int kSmallMerge = 256;       // merged aggregates up to this size use 
 constexpr int kStepCands = 32;         // k_step solves up to this many survivors itself
 constexpr long long kStepPairs = 16384;  // ... holding up to this many sphere pairs
 constexpr int kStepMover = 1024;       // movers up to this size may take k_step
+constexpr int kThreadPairs = 4;        // candidates with this few sphere pairs are solved by one thread
 constexpr int kPendingSolve = -1;      // StepRecord.status: pairs left for k_solve
 
 // Counters and results of the contact search that stay on the device.
@@ -77,7 +78,10 @@ struct DevBox {
     double *cpx, *cpy, *cpz, *cpr;             // CoM and rmax in CSR order (rmax < 0: entry dropped)
     int *cslot;                                // CSR position of each aggregate (-1: none)
     unsigned char *dirty;                      // moved, merged or removed since the build
-    int *dirty_list;
+    int *dirty_list;                           // ids in the order they became dirty
+    double *dpx, *dpy, *dpz, *dpr;             // their current CoM and rmax, by dirty-list index
+    int *dpcell;                               // their current reference cell (-1: removed)
+    int *dslot;                                // dirty-list index of each dirty aggregate
     // search scratch
     double *cand_db, *cand_exact;
     int *cand_rank, *cand_id, *order;
@@ -100,6 +104,7 @@ struct SearchArgs {
     int b1[3], len[3];
     int dirty_n;               // dirty-list length when this step runs
     int dirty_slot;            // dirty-list index this step's move appends, -1 when already listed
+    int dirty_pos;             // dirty-list index of the mover after this step
     int n_div;
     int cap;
     int n_built;               // items in the CSR cell list

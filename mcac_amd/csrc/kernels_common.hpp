@@ -46,6 +46,14 @@ template <class T> __device__ __forceinline__ T block_sum(T v) {
     return t;
 }
 
+// Barrier for data exchanged through LDS only: waits for this wave's LDS
+// operations but not for its global stores, which __syncthreads also drains.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave.
 template <class T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
     const int lane = threadIdx.x & 63;
@@ -70,7 +78,7 @@ template <class T> __device__ __forceinline__ void block_exclusive_scan(T *a, in
     const T incl = wave_inclusive_scan(s);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 63) wsum[w] = incl;
-    __syncthreads();
+    lds_barrier();
     T off = 0;
     for (int k = 0; k < w; ++k) off += wsum[k];
     T acc = off + incl - s;
@@ -80,7 +88,7 @@ template <class T> __device__ __forceinline__ void block_exclusive_scan(T *a, in
         a[i] = acc;
         acc += v;
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // aggregat.cpp:699-705 — reference Verlet cell of an aggregate position.
@@ -131,8 +139,19 @@ __device__ __forceinline__ void mark_dirty(const DevBox &b, int id) {
         b.dirty[id] = 1;
         unsigned int k = b.ss->dirty_count;
         b.dirty_list[k] = id;
+        b.dslot[id] = static_cast<int>(k);
         b.ss->dirty_count = k + 1;
     }
+}
+
+// Current CoM, rmax and cell of a dirty aggregate, read by the search.
+__device__ __forceinline__ void set_dirty_entry(const DevBox &b, int slot, double x, double y, double z, double r,
+                                                int cell) {
+    b.dpx[slot] = x;
+    b.dpy[slot] = y;
+    b.dpz[slot] = z;
+    b.dpr[slot] = r;
+    b.dpcell[slot] = cell;
 }
 
 }  // namespace mcac_dev

@@ -78,6 +78,11 @@ __device__ __forceinline__ void store_merged(const DevBox &b, const MergeArgs &a
     b.acount[a.rem] = 0;
     mark_dirty(b, k);
     mark_dirty(b, a.rem);
+    b.dpx[b.dslot[k]] = com[0];  // rmax follows once computed
+    b.dpy[b.dslot[k]] = com[1];
+    b.dpz[b.dslot[k]] = com[2];
+    b.dpcell[b.dslot[k]] = b.acell[k];
+    b.dpcell[b.dslot[a.rem]] = -1;
     b.cslot[a.rem] = -1;  // removed for good: no entry of a later build is its
 }
 
@@ -88,27 +93,80 @@ __device__ __forceinline__ void publish_merge(const MergeArgs &a, const mcac_mer
 }
 
 // ------------------------------------------------------------------ fused merge (n <= kSmallMerge)
-// One block merges; thread 0 ends with the result in res.  Used by
+// One block merges; thread 0 ends with the result in res_out.  Used by
 // k_merge_small and by the finishing block of a step that found a contact.
+// Global loads are issued in two batches (member indices with the keeper's
+// caches and flags, then the sphere rows); everything after works from LDS,
+// so the block synchronises through LDS only (no thread reads another
+// thread's global stores here).
 __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeArgs &a, mcac_merge_result &res_out) {
     __shared__ double diffpos[3], refpos[3], r[3];
     __shared__ int ok;
-    __shared__ double dcs[kSmallMerge], vols[kSmallMerge], rads[kSmallMerge];
+    __shared__ int sidx[kSmallMerge];
+    __shared__ double dcs[kSmallMerge], vols[kSmallMerge], rads[kSmallMerge], surfs[kSmallMerge];
     __shared__ double rlx[kSmallMerge], rly[kSmallMerge], rlz[kSmallMerge];
+    __shared__ double px[kSmallMerge], py[kSmallMerge], pz[kSmallMerge];
     __shared__ int nrow[kSmallMerge];
     __shared__ double crow[kSmallMerge * 8];
     __shared__ unsigned int row_over;
+    __shared__ double s_V, s_svr2;
     const int n = a.n_k + a.n_r;
+    const int k = a.keep;
+    // batch 1: member indices; thread 0 also takes the contact spheres, the
+    // keeper's cached sums and the dirty-list state
+    double cV = 0, cS = 0, cvx = 0, cvy = 0, cvz = 0, csr = 0, csvr2 = 0;
+    int dk = 0, dr = 0, csk = -1, csr_slot = -1, dsk = 0, dsr = 0;
+    unsigned int dcount = 0;
+    double m4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0}, mrel[3] = {0, 0, 0}, orel[3] = {0, 0, 0};
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+        sidx[t] = (t < a.n_k) ? b.members[a.start_k + t] : b.members[a.start_r + (t - a.n_k)];
     if (threadIdx.x == 0) {
-        double dp[3], rp[3];
-        ok = merge_prepare(b, a, dp, rp) ? 1 : 0;
-        for (int l = 0; l < 3; ++l) {
-            diffpos[l] = dp[l];
-            refpos[l] = rp[l];
+        const int ms = static_cast<int>(a.ms), os = static_cast<int>(a.os);
+        int mysphere = ms, othersphere = os;
+        if (a.keep != a.magg) {
+            mysphere = os;
+            othersphere = ms;
         }
+        m4[0] = b.sx[mysphere], m4[1] = b.sy[mysphere], m4[2] = b.sz[mysphere], m4[3] = b.sr[mysphere];
+        o4[0] = b.sx[othersphere], o4[1] = b.sy[othersphere], o4[2] = b.sz[othersphere], o4[3] = b.sr[othersphere];
+        mrel[0] = b.srx[mysphere], mrel[1] = b.sry[mysphere], mrel[2] = b.srz[mysphere];
+        orel[0] = b.srx[othersphere], orel[1] = b.sry[othersphere], orel[2] = b.srz[othersphere];
+        cV = b.avol[k], cS = b.asurf[k], cvx = b.csvx[k], cvy = b.csvy[k], cvz = b.csvz[k];
+        csr = b.csr[k], csvr2 = b.csvr2[k];
+        dk = b.dirty[k], dr = b.dirty[a.rem];
+        csk = b.cslot[k], csr_slot = b.cslot[a.rem];
+        dsk = b.dslot[k], dsr = b.dslot[a.rem];
+        dcount = b.ss->dirty_count;
         row_over = 0;
     }
-    __syncthreads();
+    lds_barrier();
+    // batch 2: the sphere rows
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int sp = sidx[t];
+        vols[t] = b.svol[sp];
+        surfs[t] = b.ssurf[sp];
+        rads[t] = b.sr[sp];
+        rlx[t] = b.srx[sp];
+        rly[t] = b.sry[sp];
+        rlz[t] = b.srz[sp];
+        px[t] = b.sx[sp];
+        py[t] = b.sy[sp];
+        pz[t] = b.sz[sp];
+    }
+    if (threadIdx.x == 0) {
+        // re-check of the contact and the removed spheres' shift
+        // (aggregat_list.cpp:375-378, aggregat.cpp:496-535)
+        // contact(moving sphere, other sphere) in the reference's argument order
+        const bool mine_first = a.keep == a.magg;
+        const double *p1 = mine_first ? m4 : o4, *p2 = mine_first ? o4 : m4;
+        ok = contact(p1[0], p1[1], p1[2], p1[3], p2[0], p2[1], p2[2], p2[3], a.L) ? 1 : 0;
+        const double mpos[3] = {m4[0], m4[1], m4[2]}, opos[3] = {o4[0], o4[1], o4[2]};
+        for (int l = 0; l < 3; ++l) {
+            const double dcontact = periodic_distance(opos[l] - mpos[l], a.L);
+            diffpos[l] = (mrel[l] + dcontact) - orel[l];
+        }
+    }
+    lds_barrier();
     if (!ok) {
         if (threadIdx.x == 0) {
             mcac_merge_result res{};
@@ -119,28 +177,32 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         }
         return;
     }
-    // member list and removed spheres
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int s;
-        if (t < a.n_k) {
-            s = b.members[a.start_k + t];
-            if (a.new_start != a.start_k) b.members[a.new_start + t] = s;
-        } else {
-            s = b.members[a.start_r + (t - a.n_k)];
-            b.members[a.new_start + t] = s;
-            remap_sphere(b, s, diffpos, refpos);
-        }
-        vols[t] = b.svol[s];
-        rads[t] = b.sr[s];
-        rlx[t] = b.srx[s];
-        rly[t] = b.sry[s];
-        rlz[t] = b.srz[s];
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
-        const int k = a.keep;
-        double V = b.avol[k], S = b.asurf[k], vx = b.csvx[k], vy = b.csvy[k], vz = b.csvz[k];
-        double sr = b.csr[k], svr2 = b.csvr2[k];
+        refpos[0] = px[0];  // the keeper's first sphere
+        refpos[1] = py[0];
+        refpos[2] = pz[0];
+    }
+    lds_barrier();
+    // member list, and the removed spheres moved into the keeper's frame
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int sp = sidx[t];
+        if (t >= a.n_k || a.new_start != a.start_k) b.members[a.new_start + t] = sp;
+        if (t >= a.n_k) {
+            const double rx = rlx[t] + diffpos[0], ry = rly[t] + diffpos[1], rz = rlz[t] + diffpos[2];
+            rlx[t] = rx;
+            rly[t] = ry;
+            rlz[t] = rz;
+            b.srx[sp] = rx;
+            b.sry[sp] = ry;
+            b.srz[sp] = rz;
+            b.sx[sp] = rx + refpos[0];
+            b.sy[sp] = ry + refpos[1];
+            b.sz[sp] = rz + refpos[2];
+        }
+    }
+    lds_barrier();
+    if (threadIdx.x == 0) {
+        double V = cV, S = cS, vx = cvx, vy = cvy, vz = cvz, sr = csr, svr2 = csvr2;
         for (int i = a.n_k; i < n; ++i) {
             const double v = vols[i];
             V = V + v;                              // aggregat.cpp:388
@@ -150,13 +212,58 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
             sr += rads[i];                          // aggregat.cpp:259
             svr2 = svr2 + v * (rads[i] * rads[i]);  // aggregat.cpp:477
         }
-        for (int i = a.n_k; i < n; ++i) S = S + b.ssurf[b.members[a.new_start + i]];  // aggregat.cpp:389
-        double rr[3] = {vx / V, vy / V, vz / V};    // aggregat.cpp:442
-        const int s0 = b.members[a.new_start];
-        double com[3] = {periodic_position(b.sx[s0] + rr[0], a.L), periodic_position(b.sy[s0] + rr[1], a.L),
-                         periodic_position(b.sz[s0] + rr[2], a.L)};  // aggregat.cpp:448
-        store_merged(b, a, V, S, vx, vy, vz, sr, svr2, rr, com);
+        for (int i = a.n_k; i < n; ++i) S = S + surfs[i];  // aggregat.cpp:389
+        double rr[3] = {vx / V, vy / V, vz / V};           // aggregat.cpp:442
+        double com[3] = {periodic_position(refpos[0] + rr[0], a.L), periodic_position(refpos[1] + rr[1], a.L),
+                         periodic_position(refpos[2] + rr[2], a.L)};  // aggregat.cpp:448
+        const int cell = ref_cell(com[0], com[1], com[2], a.n_div, a.L);
+        b.avol[k] = V;
+        b.asurf[k] = S;
+        b.csvx[k] = vx;
+        b.csvy[k] = vy;
+        b.csvz[k] = vz;
+        b.csr[k] = sr;
+        b.csvr2[k] = svr2;
+        b.arx[k] = rr[0];
+        b.ary[k] = rr[1];
+        b.arz[k] = rr[2];
+        b.ax[k] = com[0];
+        b.ay[k] = com[1];
+        b.az[k] = com[2];
+        b.acell[k] = cell;
+        b.astart[k] = a.new_start;
+        b.acap[k] = a.cap_k;
+        b.acount[k] = n;
+        b.acell[a.rem] = -1;
+        b.acount[a.rem] = 0;
+        // both join the dirty list (keeper first) and leave the packed cell list
+        unsigned int cnt = dcount;
+        if (!dk) {
+            if (csk >= 0) b.cpr[csk] = -1.0;
+            b.dirty[k] = 1;
+            b.dirty_list[cnt] = k;
+            b.dslot[k] = static_cast<int>(cnt);
+            dsk = static_cast<int>(cnt);
+            ++cnt;
+        }
+        if (!dr) {
+            if (csr_slot >= 0) b.cpr[csr_slot] = -1.0;
+            b.dirty[a.rem] = 1;
+            b.dirty_list[cnt] = a.rem;
+            b.dslot[a.rem] = static_cast<int>(cnt);
+            dsr = static_cast<int>(cnt);
+            ++cnt;
+        }
+        b.ss->dirty_count = cnt;
+        b.dpx[dsk] = com[0];
+        b.dpy[dsk] = com[1];
+        b.dpz[dsk] = com[2];
+        b.dpcell[dsk] = cell;
+        b.dpcell[dsr] = -1;
+        b.cslot[a.rem] = -1;
         for (int l = 0; l < 3; ++l) r[l] = rr[l];
+        s_V = V;
+        s_svr2 = svr2;
         mcac_merge_result &res = b.ms->res;
         res.keeper = k;
         res.removed = a.rem;
@@ -171,7 +278,7 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
             res.rcm[l] = rr[l];
         }
     }
-    __syncthreads();
+    lds_barrier();
     // distances to the centre of mass (aggregat.cpp:444-447) and contacts per row
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
         double d0 = rlx[t] - r[0], d1 = rly[t] - r[1], d2 = rlz[t] - r[2];
@@ -194,17 +301,17 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         }
         nrow[i] = cnt;
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         double rmax = 0.0, arg = 0.;
         for (int i = 0; i < n; ++i) {
             rmax = smax(rmax, rads[i] + dcs[i]);       // aggregat.cpp:462
             arg = arg + vols[i] * (dcs[i] * dcs[i]);   // aggregat.cpp:476
         }
-        const int k = a.keep;
-        const double V = b.avol[k];
-        const double rg = __builtin_sqrt(__builtin_fabs((arg + 3. / 5. * b.csvr2[k]) / V));  // aggregat.cpp:481
+        const double V = s_V;
+        const double rg = __builtin_sqrt(__builtin_fabs((arg + 3. / 5. * s_svr2) / V));  // aggregat.cpp:481
         b.armax[k] = rmax;
+        b.dpr[dsk] = rmax;
         b.arg[k] = rg;
         double overlapping = 0.0;
         long long inter = 0;
@@ -375,6 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b) {
     const double rmax = __longlong_as_double(static_cast<long long>(m->rmax_bits));
     const double rg = __builtin_sqrt(__builtin_fabs((arg + 3. / 5. * b.csvr2[k]) / b.avol[k]));
     b.armax[k] = rmax;
+    b.dpr[b.dslot[k]] = rmax;
     b.arg[k] = rg;
     m->res.rmax = rmax;
     m->res.rg = rg;

@@ -101,7 +101,30 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
     const int gstride = gridDim.x * blockDim.x;
     unsigned long long cnt = 0;
     int total;  // entries of the visited cells (or of the whole list for a very wide window)
-    if (V <= kMaxVisit) {
+    if (V <= 64) {
+        // one wave: lane v loads cell v's start and size and the prefix runs in
+        // registers (wave shuffles); one barrier publishes it
+        if (threadIdx.x < 64) {
+            const int v = threadIdx.x;
+            int sz = 0, c0 = 0;
+            if (v < V) {
+                int oi = v / (lj * lk), rem = v - oi * (lj * lk), oj = rem / lk, ok = rem - oj * lk;
+                int ci = window_offset(b1i + oi, 0, n), cj = window_offset(b1j + oj, 0, n),
+                    ck = window_offset(b1k + ok, 0, n);
+                int c = (ci * n + cj) * n + ck;
+                c0 = b.cstart[c];
+                sz = b.cstart[c + 1] - c0;
+            }
+            const int incl = wave_inclusive_scan(sz);
+            if (v < V) {
+                cbeg[v] = c0;
+                pre[v] = incl - sz;
+            }
+            if (v == 63) pre[V] = incl;
+        }
+        lds_barrier();
+        total = pre[V];
+    } else if (V <= kMaxVisit) {
         // sizes and starts of the visited cells in visit order, then their prefix
         for (int v = threadIdx.x; v < V; v += blockDim.x) {
             int oi = v / (lj * lk), rem = v - oi * (lj * lk), oj = rem / lk, ok = rem - oj * lk;
@@ -112,7 +135,7 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
             cbeg[v] = c0;
             pre[v] = b.cstart[c + 1] - c0;
         }
-        __syncthreads();
+        lds_barrier();
         block_exclusive_scan(pre, V);
         total = pre[V];
     } else {
@@ -152,12 +175,13 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
                 rank = (oi * lj + oj) * lk + ok;
             }
         } else {
-            id = b.dirty_list[w - total];
-            const int c = b.acell[id];
-            x = b.ax[id];
-            y = b.ay[id];
-            z = b.az[id];
-            r = b.armax[id];
+            const int i = w - total;
+            id = b.dirty_list[i];
+            const int c = b.dpcell[i];
+            x = b.dpx[i];
+            y = b.dpy[i];
+            z = b.dpz[i];
+            r = b.dpr[i];
             if (c < 0 || id == a.src) continue;
             int oi = window_offset(c / (n * n), b1i, n), oj = window_offset((c / n) % n, b1j, n),
                 ok = window_offset(c % n, b1k, n);
@@ -229,7 +253,7 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
         }
         b.order[pos] = static_cast<int>(c);
     }
-    __syncthreads();
+    if (F > 0) __syncthreads();  // order and exact distances come from other threads
     __shared__ double s_ref[3];
     __shared__ StepRecord srec;  // assembled here, copied to host memory by one wave
     SearchState *s = b.ss;
@@ -293,9 +317,12 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
             if (a.dirty_slot >= 0) {  // first move since the cell-list build
                 b.dirty[a.src] = 1;
                 b.dirty_list[a.dirty_slot] = a.src;
+                b.dslot[a.src] = a.dirty_slot;
                 s->dirty_count = static_cast<unsigned int>(a.dirty_slot + 1);
                 if (t.src_slot >= 0) b.cpr[t.src_slot] = -1.0;
             }
+            set_dirty_entry(b, a.dirty_pos, s->newcom[0], s->newcom[1], s->newcom[2],
+                            a.known ? a.srmax : b.armax[a.src], s->newcell);
         }
         if (a.gated && contact) s->gate = a.seq;  // the steps behind wait for the merge
         o->skipped = 0;
@@ -310,7 +337,7 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
         s->done_solve = 0;
         s->tick = 0ull;
     }
-    __syncthreads();
+    lds_barrier();  // s_ref
     if (a.apply && a.translate_in_finish && m.loaded) {
         if (m.sp >= 0) {
             b.sx[m.sp] = s_ref[0] + m.rx;
@@ -331,7 +358,6 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
     if (a.automerge && a.apply && a.translate_in_finish) {
         __shared__ MergeArgs s_ma;
         __shared__ int s_inline;
-        __syncthreads();  // the mover's spheres are in place
         if (threadIdx.x == 0) {
             s_inline = 0;
             if (srec.contact) {
@@ -372,12 +398,12 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (s_inline) {
+            __syncthreads();  // the mover's spheres, placed by every thread, are in memory
             merge_small_block(b, s_ma, srec.merge);
-            if (threadIdx.x == 0) srec.merged_inline = 1;
+            if (threadIdx.x == 0) srec.merged_inline = 1;  // thread 0 also copies the record
         }
-        __syncthreads();
     }
     // the record body in one coalesced write, then the sequence number
     constexpr int kWords = static_cast<int>(offsetof(StepRecord, seq) / sizeof(unsigned long long));
@@ -394,16 +420,7 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
 }
 
 __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
-    if (behind_gate(b, a)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            a.out->skipped = 1;
-            a.out->contact = 0;
-            a.out->status = 0;
-            __threadfence_system();
-            __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-        return;
-    }
+    if (behind_gate(b, a)) return;  // the host knows it is skipped: no record
     __shared__ long long cpre[kMaxPlan + 1];
     __shared__ long long s_chunk;
     __shared__ bool last;
@@ -525,16 +542,7 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
 // them in place, closes the gate for the steps queued behind and publishes a
 // pending record: the host then launches k_solve for the same step.
 __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
-    if (behind_gate(b, a)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            a.out->skipped = 1;
-            a.out->contact = 0;
-            a.out->status = 0;
-            __threadfence_system();
-            __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-        return;
-    }
+    if (behind_gate(b, a)) return;  // the host knows it is skipped: no record
     // loads the finishing block will need, issued before the filter work
     const int src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
     const MoverSphere mover = load_mover_sphere(b, a);
@@ -583,11 +591,40 @@ __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
             }
             return;
         }
-        // exact sphere-pair distances, lexicographic (d, i, j) minimum per candidate
-        for (unsigned int c = 0; c < F; ++c) {
+        // exact sphere-pair distances, lexicographic (d, i, j) minimum per
+        // candidate: a candidate with at most kThreadPairs pairs is walked by
+        // one thread in (i, j) order (strict < keeps the first minimum), the
+        // others by the whole block
+        __shared__ int s_nk[kStepCands], s_sk[kStepCands], s_done[kStepCands];
+        if (threadIdx.x < F) {
+            const unsigned int c = threadIdx.x;
             const int k = b.cand_id[c];
-            const long long nk = b.acount[k];
+            const int nk = b.acount[k];
             const int sk = b.astart[k];
+            s_nk[c] = nk;
+            s_sk[c] = sk;
+            int done = 0;
+            if (nsrc * nk <= kThreadPairs) {
+                double bd = kInf;
+                long long bp = 0x7fffffffffffffffLL;
+                for (long long p = 0; p < nsrc * nk; ++p) {
+                    const long long i = p / nk, j = p - i * nk;
+                    const int si = b.members[ssrc + i], sj = b.members[sk + j];
+                    const double d = swept_sphere(b.sx[si], b.sy[si], b.sz[si], b.sr[si], b.sx[sj], b.sy[sj],
+                                                  b.sz[sj], b.sr[sj], a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
+                    lexmin(bd, bp, d, p);
+                }
+                b.cand_exact[c] = bd;
+                b.cand_pair[c] = (bd < kInf) ? bp : -1;
+                done = 1;
+            }
+            s_done[c] = done;
+        }
+        lds_barrier();
+        for (unsigned int c = 0; c < F; ++c) {
+            if (s_done[c]) continue;
+            const long long nk = s_nk[c];
+            const int sk = s_sk[c];
             double bd = kInf;
             long long bp = 0x7fffffffffffffffLL;
             for (long long p = threadIdx.x; p < nsrc * nk; p += blockDim.x) {
@@ -610,9 +647,11 @@ __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
 }
 
 // a13 for a large mover: sphere positions from the refpos of the last search.
-__global__ __launch_bounds__(kBlock) void k_translate(DevBox b, int src, int gated, unsigned long long seq) {
+__global__ __launch_bounds__(kBlock) void k_translate(DevBox b, int src, int gated, unsigned long long seq,
+                                                      unsigned long long released) {
     const SearchState *s = b.ss;
-    if (gated && s->gate != 0ull && s->gate < seq) return;
+    const unsigned long long g = s->gate;
+    if (gated && g != 0ull && g < seq && g > released) return;  // the step was skipped
     const int n = b.acount[src];
     const int st = b.astart[src];
     const double r0 = s->refpos[0], r1 = s->refpos[1], r2 = s->refpos[2];
@@ -640,6 +679,7 @@ __global__ void k_set_move(DevBox b, int src, double v0, double v1, double v2, i
     b.az[src] = s->newcom[2];
     b.acell[src] = s->newcell;
     mark_dirty(b, src);
+    set_dirty_entry(b, b.dslot[src], s->newcom[0], s->newcom[1], s->newcom[2], b.armax[src], s->newcell);
 }
 
 __global__ void k_swept_batch(long long n, const double *p1, const double *r1, const double *p2, const double *r2,

@@ -10,6 +10,7 @@
 // One MC step is two launches (k_filter, k_solve; plus k_translate for movers
 // of more than 1024 spheres) and no copy: the last block publishes a small
 // record into pinned host memory, which the host polls by sequence number.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,6 +46,7 @@ struct mcac_box {
     std::vector<double> h_x, h_y, h_z, h_rx, h_ry, h_rz, h_rmax;
     std::vector<int> pend;
     std::vector<unsigned char> h_dirty;
+    std::vector<int> h_dslot;        // dirty-list index of each dirty aggregate
     int64_t dirty_count = 0;         // device dirty-list length after the queued steps
     int64_t dirty_epoch = 0;         // cell-list builds so far
     bool gate_closed = false;        // a consumed queued step found a contact; not released yet
@@ -259,14 +261,17 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     }
     a.dirty_n = static_cast<int>(b->dirty_count);
     a.dirty_slot = -1;
+    a.dirty_pos = 0;
     int dirty_new = 0;
     if (apply && !doomed) {
         if (!b->h_dirty[A]) {
             a.dirty_slot = static_cast<int>(b->dirty_count);
             b->h_dirty[A] = 1;
+            b->h_dslot[A] = a.dirty_slot;
             b->dirty_count += 1;
             dirty_new = 1;
         }
+        a.dirty_pos = b->h_dslot[A];
         b->pend[A] += 1;
     }
     a.n_div = static_cast<int>(b->p.n_div);
@@ -297,21 +302,33 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     const bool timed = b->timing > 0 && seq % static_cast<unsigned long long>(b->timing) == 0;
     // one launch when the mover is small enough for the finishing block
     const bool fused = n_mov <= kStepMover && static_cast<long long>(n_mov) * b->max_count <= kStepPairs;
-    if (timed) HIP_TRY(hipEventRecord(ev[0], b->stream));
+    // filter grid sized to the expected work: entries of the visited cells
+    // (1.5x the mean cell occupancy) plus the dirty list, one per thread
+    int grid = kFilterBlocks;
+    if (a.known && b->n_cells > 0) {
+        const double per_cell = static_cast<double>(b->n_built) / b->n_cells;
+        const double items = 1.5 * per_cell * a.len[0] * a.len[1] * a.len[2] + a.dirty_n + 1;
+        grid = static_cast<int>(std::min<double>(kFilterBlocks, std::ceil(items / kBlock)));
+        grid = std::max(grid, 1);
+    }
+    // timed launches carry start/stop events that the runtime stamps with the
+    // kernel's own execution interval; the queue is drained first so that the
+    // interval does not include waiting behind the steps queued before it
+    if (timed && b->inflight > 0) HIP_TRY(hipStreamSynchronize(b->stream));
     if (fused) {
-        hipLaunchKernelGGL(k_step, dim3(kFilterBlocks), dim3(kBlock), 0, b->stream, b->d, a);
-        if (timed) HIP_TRY(hipEventRecord(ev[1], b->stream));
+        if (timed) hipExtLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, b->stream, ev[0], ev[1], 0, b->d, a);
+        else hipLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, b->stream, b->d, a);
+    } else if (timed) {
+        hipExtLaunchKernelGGL(k_filter, dim3(grid), dim3(kBlock), 0, b->stream, ev[0], ev[1], 0, b->d, a);
+        hipExtLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, ev[2], ev[3], 0, b->d, a);
     } else {
-        hipLaunchKernelGGL(k_filter, dim3(kFilterBlocks), dim3(kBlock), 0, b->stream, b->d, a);
-        if (timed) HIP_TRY(hipEventRecord(ev[1], b->stream));
+        hipLaunchKernelGGL(k_filter, dim3(grid), dim3(kBlock), 0, b->stream, b->d, a);
         hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, a);
     }
-    if (timed) HIP_TRY(hipEventRecord(ev[2], b->stream));
     if (apply && !a.translate_in_finish) {
         const int tb = std::min(1024, (n_mov + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, a.src, gated, seq);
+        hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, a.src, gated, seq, a.released);
     }
-    if (timed) HIP_TRY(hipEventRecord(ev[3], b->stream));
     b->slot_args[slot] = a;
     b->slot_fused[slot] = fused ? 1 : 0;
     (fused ? b->n_fused : b->n_split) += 1;
@@ -344,19 +361,15 @@ void take_out(mcac_box *b, int slot) {
 // Adds the event times of a consumed, timed step to the totals.
 int harvest_timing(mcac_box *b, int slot) {
     hipEvent_t *ev = b->sev[slot];
-    HIP_TRY(hipEventSynchronize(ev[3]));
-    float t01 = 0, t12 = 0, t23 = 0;
+    const bool fused = b->slot_fused[slot] != 0;
+    HIP_TRY(hipEventSynchronize(ev[fused ? 1 : 3]));
+    float t01 = 0, t23 = 0;
     HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]));
-    HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]));
-    HIP_TRY(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    if (!fused) HIP_TRY(hipEventElapsedTime(&t23, ev[2], ev[3]));
     b->t_filter += t01;
-    b->t_solve += t12;
-    b->t_search += t01 + t12;
+    b->t_solve += t23;
+    b->t_search += t01 + t23;
     b->n_search += 1;
-    if (b->slot_translate[slot]) {
-        b->t_translate += t23;
-        b->n_translate += 1;
-    }
     b->slot_timed[slot] = 0;
     return 0;
 }
@@ -364,6 +377,19 @@ int harvest_timing(mcac_box *b, int slot) {
 int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contact *out, int *skipped) {
     if (ticket != b->oldest || b->inflight <= 0) return MCAC_E_ARGUMENT;  // results are read in order
     const int slot = static_cast<int>(ticket % mcac_box::kRing);
+    if (b->slot_void[slot]) {
+        // queued behind a closed gate: the device skips it and writes no record
+        b->inflight -= 1;
+        b->oldest = ticket + 1;
+        b->slot_timed[slot] = 0;
+        if (skipped) *skipped = 1;
+        if (out) {
+            std::memset(out, 0, sizeof(*out));
+            out->moving_agg = agg;
+            out->moving_sphere = out->other_sphere = out->other_agg = -1;
+        }
+        return 0;
+    }
     int rc = wait_seq(b, &b->h_step[slot].seq, ticket);
     if (rc) return rc;
     bool handed_over = false;
@@ -510,6 +536,8 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.cstart, b->n_cells + 1); ALLOC(d.citems, Na); ALLOC(d.ccursor, b->n_cells + 1);
     ALLOC(d.cpx, Na); ALLOC(d.cpy, Na); ALLOC(d.cpz, Na); ALLOC(d.cpr, Na); ALLOC(d.cslot, Na);
     ALLOC(d.dirty, Na); ALLOC(d.dirty_list, Na + 1);
+    ALLOC(d.dpx, Na + 1); ALLOC(d.dpy, Na + 1); ALLOC(d.dpz, Na + 1); ALLOC(d.dpr, Na + 1);
+    ALLOC(d.dpcell, Na + 1); ALLOC(d.dslot, Na);
     ALLOC(d.cand_db, b->cand_cap); ALLOC(d.cand_exact, b->cand_cap); ALLOC(d.cand_rank, b->cand_cap);
     ALLOC(d.cand_id, b->cand_cap); ALLOC(d.order, b->cand_cap); ALLOC(d.cand_pair, b->cand_cap);
     ALLOC(d.part_d, b->part_cap); ALLOC(d.part_p, b->part_cap);
@@ -653,6 +681,7 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
     b->h_rmax.assign(rmax, rmax + A);
     b->pend.assign(A, 0);
     b->h_dirty.assign(A, 0);
+    b->h_dslot.assign(A, 0);
     b->dirty_count = 0;
     b->gate_closed = false;
     b->released = b->seq;
@@ -691,12 +720,13 @@ int mcac_translate(mcac_box *b, int64_t agg, const double vec[3]) {
         b->h_z[A] = periodic_position(b->h_z[A] + vec[2], b->p.box_length);
         if (!b->h_dirty[A]) {
             b->h_dirty[A] = 1;
+            b->h_dslot[A] = static_cast<int>(b->dirty_count);
             b->dirty_count += 1;
         }
     }
     const int n = b->count[static_cast<size_t>(agg)];
     const int tb = std::min(1024, std::max(1, (n + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, static_cast<int>(agg), 0, 0ull);
+    hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, static_cast<int>(agg), 0, 0ull, 0ull);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(b->stream));
     return 0;
@@ -726,6 +756,7 @@ static int merge_done(mcac_box *b, size_t K, size_t R, int new_start, int new_ca
         for (size_t id : {K, R})
             if (!b->h_dirty[id]) {
                 b->h_dirty[id] = 1;
+                b->h_dslot[id] = static_cast<int>(b->dirty_count);
                 b->dirty_count += 1;
             }
     b->h_x[K] = out->com[0];

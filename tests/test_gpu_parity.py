@@ -85,9 +85,17 @@ def test_next_contact_bit_exact(cfg, steps):
 def compare_runs(cfg_text, max_steps, fast_init=True):
     ora = O.OracleSim(cfg_text, fast_init=fast_init)
     n_ora = ora.run(max_steps)
+    say(f"  oracle: {n_ora} steps, finished={ora.finished()}")
     sim = mcac().Simulation(cfg_text, trace_steps=True, trace_merges=True)
-    n_gpu = sim.run(max_steps)
+    # the device run is bounded by the oracle's step count, in pieces that report progress
+    n_gpu = 0
+    while n_gpu < n_ora:
+        n_gpu += sim.run(min(20000, n_ora - n_gpu))
+        say(f"  device: {n_gpu} steps")
+        if sim.finished():
+            break
     assert n_gpu == n_ora
+    assert sim.finished() == ora.finished()
     a, b = ora.steps(), sim.steps()
     assert len(a) == len(b)
     for f in a.dtype.names:
