@@ -42,6 +42,47 @@ def cpu_baseline(text: str, steps: int):
                        f"placement {t_init:.1f} s not timed); {dt:.1f} s of CPU work")
 
 
+def _pct(x, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(x, float), q)) if len(x) else None
+
+
+def run_ensemble(args) -> dict:
+    """Batched form: the first K boxes of the 512-box list, W boxes at a time on the GPU."""
+    from mcac_amd import ensemble
+    boxes = ensemble.ensemble_configs(args.ensemble_boxes)
+    print(f"bench: ensemble of {len(boxes)} boxes, {args.workers} workers", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    r = ensemble.batch_run([b["ini"] for b in boxes], workers=args.workers, max_steps=args.max_job_steps)
+    wall = time.perf_counter() - t0
+    done = [j for j in range(len(boxes)) if r["steps"][j] < args.max_job_steps]
+    dur = (r["end"] - r["start"])[done]
+    return {"boxes": len(boxes), "of": 512, "completed": len(done), "workers": args.workers, "wall_s": wall,
+            "runs_per_hour": 3600.0 * len(done) / wall, "mc_steps_per_s": float(r["steps"].sum()) / wall,
+            "box_time_p50_s": _pct(dur, 50), "box_time_p99_s": _pct(dur, 99),
+            "sample": f"boxes 0..{len(boxes) - 1} of the 512-box list (N log-uniform 500..5e4, normal 30/3 nm, "
+                      f"FV U[1e-4,5e-3], T U[1000,2000] K, until N_agg <= N/10), all started at t=0"}
+
+
+def run_serving(args) -> dict:
+    """Serving form: Poisson arrivals, time-to-result percentiles."""
+    import numpy as np
+    from mcac_amd import ensemble
+    reqs = ensemble.serving_requests(args.serving_requests, args.serving_rate, n_max=args.serving_nmax)
+    print(f"bench: serving {len(reqs)} requests at {args.serving_rate}/s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    r = ensemble.batch_run([q["ini"] for q in reqs], arrivals=[q["arrival"] for q in reqs], workers=args.workers,
+                           max_steps=args.max_job_steps)
+    wall = time.perf_counter() - t0
+    arrival = np.array([q["arrival"] for q in reqs])
+    ttr = r["end"] - arrival
+    return {"requests": len(reqs), "rate_per_s": args.serving_rate, "workers": args.workers, "wall_s": wall,
+            "time_to_result_p50_s": _pct(ttr, 50), "time_to_result_p99_s": _pct(ttr, 99),
+            "runs_per_hour": 3600.0 * len(reqs) / wall,
+            "sample": f"N log-uniform 1e3..{args.serving_nmax:g} (full form 2e5), lognormal 30/1.25, FV U[1e-4,5e-3], "
+                      f"T U[1200,2000] K, until N_agg <= N/10, no surface reactions"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +92,14 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 20 times (HBM counter calibration)")
+    ap.add_argument("--ensemble-boxes", type=int, default=16,
+                    help="boxes of the 512-box ensemble list run to completion (0: skip)")
+    ap.add_argument("--serving-requests", type=int, default=16, help="serving requests (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=4.0, help="serving arrival rate, requests/s")
+    ap.add_argument("--serving-nmax", type=float, default=2e4,
+                    help="largest request size in the serving sample (the full form allows 2e5)")
+    ap.add_argument("--workers", type=int, default=8, help="host threads sharing the GPU (ensemble, serving)")
+    ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")
     ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
     args = ap.parse_args()
@@ -146,6 +195,10 @@ def main() -> int:
                       "avg_candidates_filtered": d["filtered"] / args.steps,
                       "timed_searches": tm["n_search"], "launch_stats": st},
     }
+    if args.ensemble_boxes > 0:
+        out["ensemble"] = run_ensemble(args)
+    if args.serving_requests > 0:
+        out["serving"] = run_serving(args)
     if not args.no_cpu:
         cb = cpu_baseline(text, args.cpu_steps)
         cb["ratio"] = steps_per_s / cb["value"]

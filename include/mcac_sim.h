@@ -83,6 +83,16 @@ int mcac_physics_scalars(const char *ini_text, double out[8]);
 int64_t mcac_pick_table_selftest(int64_t n, int64_t events, int64_t chunk, uint64_t seed, int mode);
 int64_t mcac_pick_table_selftest_ties(void);
 
+/* Ensemble / serving runs (no reference counterpart: the reference runs one
+ * box per process, src/main.cpp): n_jobs parameter files run to completion
+ * (or max_steps MC steps) on `workers` host threads sharing one GPU, each
+ * worker with its own box and stream.  Job j starts in list order, not before
+ * arrival_s[j] seconds after the call (NULL: all at once).  Per job: start and
+ * end time (s since the call), MC steps, status.  Returns the first non-zero
+ * job status, else 0. */
+int mcac_batch_run(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers, int device,
+                   int64_t max_steps, double *start_s, double *end_s, int64_t *steps, int *status);
+
 #ifdef __cplusplus
 }
 #endif

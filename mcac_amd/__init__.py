@@ -59,7 +59,7 @@ SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
     "mcac_sim_set_trace", "mcac_sim_n_steps", "mcac_sim_get_steps", "mcac_sim_n_merges",
     "mcac_sim_get_merges", "mcac_sim_scalar", "mcac_sim_n_aggs", "mcac_sim_export_aggs", "mcac_sim_box",
-    "mcac_physics_scalars", "mcac_pick_table_selftest", "mcac_pick_table_selftest_ties",
+    "mcac_physics_scalars", "mcac_pick_table_selftest", "mcac_pick_table_selftest_ties", "mcac_batch_run",
 )
 
 _P = ctypes.c_void_p
@@ -148,6 +148,8 @@ def load_sim():
         L.mcac_pick_table_selftest.restype = _I64
         L.mcac_pick_table_selftest.argtypes = [_I64, _I64, _I64, ctypes.c_uint64, _I]
         L.mcac_pick_table_selftest_ties.restype = _I64
+        L.mcac_batch_run.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I64, _DP, _DP, _I64P,
+                                     ctypes.POINTER(_I)]
         _sim = L
     return _sim
 

@@ -41,7 +41,7 @@ def build(verbose: bool = False) -> None:
     tmp = LIB / "libmcac_gpu.so.tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), str(CSRC / "mcac_gpu.hip")])
     os.replace(tmp, gpu)
-    host_src = [str(CSRC / "host" / "physics.cpp"), str(CSRC / "host" / "pick_table.cpp"), str(CSRC / "host" / "sim.cpp")]
+    host_src = [str(CSRC / "host" / f) for f in ("physics.cpp", "pick_table.cpp", "sim.cpp", "batch.cpp")]
     sim = LIB / "libmcac_sim.so"
     tmp = LIB / "libmcac_sim.so.tmp"
     _run([CXX, *CXX_FLAGS, "-shared", "-o", str(tmp), *host_src, f"-L{LIB}", "-lmcac_gpu",

@@ -54,8 +54,11 @@ class Workers {
         static Workers w;
         return w;
     }
+    // Runs fn(k) for k in [0, n); one caller at a time uses the pool, the
+    // others (simulations on other host threads) run serially.
     template <class F> void run(size_t n, F &fn) {
-        if (n < 2 || threads_.empty()) {
+        std::unique_lock<std::mutex> own(user_, std::try_to_lock);
+        if (n < 2 || threads_.empty() || !own.owns_lock()) {
             for (size_t k = 0; k < n; ++k) fn(k);
             return;
         }
@@ -104,6 +107,7 @@ class Workers {
         }
     }
     std::vector<std::thread> threads_;
+    std::mutex user_;  // held by the thread whose job the pool runs
     std::mutex m_;
     std::condition_variable cv_, done_;
     std::function<void(size_t)> job_;

@@ -1,0 +1,86 @@
+"""Ensemble and serving workloads over many independent boxes on one GPU.
+
+Batched form: 512 boxes, N log-uniform in [500, 5e4], normal diameters (mean
+30 nm, sd 3 nm), FV uniform in [1e-4, 5e-3], T uniform in [1000, 2000] K, box b
+seeded s0 + b, each run until N_agg <= N/10.  Serving form: requests with N
+log-uniform in [1e3, 2e5], FV in [1e-4, 5e-3], T in [1200, 2000] K arriving as
+a Poisson process; time-to-result = completion - arrival.
+
+Parameter files use the reference's INI keys (src/physical_model/physical_model.cpp).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _check, _dp, _ip, load_sim
+
+
+def box_ini(n: int, fv: float, temperature: float, seed: int, mean_d: float = 30.0, sd_d: float = 3.0,
+            mode: str = "normal", stop_fraction: float = 0.1) -> str:
+    """One parameter file; the run stops at N_agg <= stop_fraction * N."""
+    return f"""[monomers]
+number = {n}
+density = 1800
+mean_diameter = {mean_d}
+dispersion_diameter = {sd_d}
+initialisation_mode = {mode}
+
+[environment]
+volume_fraction = {fv!r}
+temperature = {temperature!r}
+pressure = 101300
+fractal_prefactor = 1.30
+fractal_dimension = 1.78
+
+[limits]
+number_of_aggregates = {max(1, int(n * stop_fraction))}
+
+[numerics]
+random_seed = {seed}
+with_domain_duplication = false
+n_verlet_divisions = 10
+"""
+
+
+def ensemble_configs(n_boxes: int = 512, s0: int = 1000, rng_seed: int = 4) -> list[dict]:
+    """The batched form's box list (the first n_boxes of the 512-box list)."""
+    rng = np.random.default_rng(rng_seed)
+    out = []
+    for b in range(512):
+        n = int(round(np.exp(rng.uniform(np.log(500), np.log(5e4)))))
+        fv = float(rng.uniform(1e-4, 5e-3))
+        t = float(rng.uniform(1000, 2000))
+        out.append(dict(n=n, fv=fv, temperature=t, seed=s0 + b, ini=box_ini(n, fv, t, s0 + b)))
+    return out[:n_boxes]
+
+
+def serving_requests(n_requests: int, rate_per_s: float, s0: int = 5000, rng_seed: int = 5,
+                     n_max: float = 2e5) -> list[dict]:
+    """Serving form: Poisson arrivals at rate_per_s (first arrival at 0)."""
+    rng = np.random.default_rng(rng_seed)
+    t = 0.0
+    out = []
+    for r in range(n_requests):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(n_max)))))
+        fv = float(rng.uniform(1e-4, 5e-3))
+        temp = float(rng.uniform(1200, 2000))
+        out.append(dict(n=n, fv=fv, temperature=temp, seed=s0 + r, arrival=t,
+                        ini=box_ini(n, fv, temp, s0 + r, mode="lognormal", sd_d=1.25)))
+        t += float(rng.exponential(1.0 / rate_per_s))
+    return out
+
+
+def batch_run(inis: list[str], arrivals=None, workers: int = 8, device: int = 0, max_steps: int = 2 ** 62) -> dict:
+    """Runs the jobs on `workers` host threads sharing the GPU (include/mcac_sim.h mcac_batch_run)."""
+    n = len(inis)
+    texts = (ctypes.c_char_p * n)(*[s.encode() for s in inis])
+    arr = None if arrivals is None else np.ascontiguousarray(arrivals, np.float64)
+    start, end = np.zeros(n), np.zeros(n)
+    steps = np.zeros(n, np.int64)
+    status = np.zeros(n, np.int32)
+    rc = load_sim().mcac_batch_run(texts, None if arr is None else _dp(arr), n, workers, device, max_steps,
+                                   _dp(start), _dp(end), _ip(steps), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    _check(rc, "mcac_batch_run")
+    return dict(start=start, end=end, steps=steps, status=status)
