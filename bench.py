@@ -42,6 +42,27 @@ def cpu_baseline(text: str, steps: int):
                        f"placement {t_init:.1f} s not timed); {dt:.1f} s of CPU work")
 
 
+def c3_windows(args) -> int:
+    """The N=1e6 box run on for a wall-time budget in 20,000-step chunks; one
+    JSON line per chunk on stderr, the summary on stdout (steps/s by N_agg)."""
+    import mcac_amd
+    sim = mcac_amd.Simulation(WORKLOAD_INI.read_text(), device=0)
+    t_end = time.perf_counter() + args.c3_windows
+    rows = []
+    while time.perf_counter() < t_end and not sim.finished():
+        n0 = sim.scalar("n_agg")
+        t0 = time.perf_counter()
+        done = sim.run(20000)
+        dt = time.perf_counter() - t0
+        rows.append({"n_agg_start": n0, "n_agg_end": sim.scalar("n_agg"), "steps": done, "seconds": dt,
+                     "steps_per_s": done / dt, "time": sim.scalar("time"), "merges": sim.scalar("merges")})
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+        if done == 0:
+            break
+    print(json.dumps({"c3_windows": rows}), flush=True)
+    return 0
+
+
 def _pct(x, q):
     import numpy as np
     return float(np.percentile(np.asarray(x, float), q)) if len(x) else None
@@ -100,6 +121,8 @@ def main() -> int:
                     help="largest request size in the serving sample (the full form allows 2e5)")
     ap.add_argument("--workers", type=int, default=8, help="host threads sharing the GPU (ensemble, serving)")
     ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")
+    ap.add_argument("--c3-windows", type=float, default=0.0,
+                    help="seconds: run the N=1e6 box on and report MC steps/s per window of N_agg (0: skip)")
     ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
     args = ap.parse_args()
@@ -119,6 +142,8 @@ def main() -> int:
         print(json.dumps({"copy_calibration_bytes_read": n, "copy_calibration_bytes_written": n, "launches": 20}))
         return 0
     import mcac_amd
+    if args.c3_windows > 0:
+        return c3_windows(args)
 
     text = WORKLOAD_INI.read_text()
     print("bench: placing N=1e6 monomers", file=sys.stderr, flush=True)
