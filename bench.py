@@ -123,6 +123,8 @@ def main() -> int:
     ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")
     ap.add_argument("--c3-windows", type=float, default=0.0,
                     help="seconds: run the N=1e6 box on and report MC steps/s per window of N_agg (0: skip)")
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
+                    help="contact search: 0 FP64, 1 FP32 broad phase + FP64 solve, 2 as 1 with miss counting")
     ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
     args = ap.parse_args()
@@ -130,6 +132,7 @@ def main() -> int:
         print("bench.py measures one GPU: this path runs one aggregation box per GPU "
               "(independent boxes shard across GPUs with no exchange); use --gpus 1", file=sys.stderr)
         return 2
+    os.environ["MCAC_PRECISION"] = str(args.precision)
     print("bench: importing torch", file=sys.stderr, flush=True)
     import torch
     if args.copy_calibration:
@@ -170,6 +173,7 @@ def main() -> int:
     tm = box.timing()
     box.set_timing(0)
     st = box.stats()
+    prune = box.prune_stats() if args.precision > 0 else None
     d = {k: s1[k] - s0[k] for k in s0}
     steps_per_s = args.steps / elapsed
     # dominant kernel: the contact search (k_step, or k_filter + k_solve for
@@ -200,9 +204,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded lognormal monomers placed by the reference algorithm)",
-        "config": {"workload": "N=1e6 spheres lognormal 30nm/1.25, FV=3e-3, 1700 K, plain FP64 path, "
-                               f"MC steps {args.warmup}..{args.warmup + args.steps} from t=0",
-                   "n_spheres": 1_000_000, "parallelism": "single box, 1 GPU"},
+        "config": {"workload": "N=1e6 spheres lognormal 30nm/1.25, FV=3e-3, 1700 K, "
+                               + ("plain FP64 path, " if args.precision == 0 else "FP32 broad phase + FP64 solve, ")
+                               + f"MC steps {args.warmup}..{args.warmup + args.steps} from t=0",
+                   "n_spheres": 1_000_000, "parallelism": "single box, 1 GPU", "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": "k_step" if one_launch else "k_filter", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": filt_ms},
@@ -218,7 +223,7 @@ def main() -> int:
                       "merge_ms": tm["merge_ms"] / max(tm["n_merge"], 1.0),
                       "merges": d["merges"], "avg_candidates_visited": d["query"] / args.steps,
                       "avg_candidates_filtered": d["filtered"] / args.steps,
-                      "timed_searches": tm["n_search"], "launch_stats": st},
+                      "timed_searches": tm["n_search"], "launch_stats": st, "fp32_prune": prune},
     }
     if args.ensemble_boxes > 0:
         out["ensemble"] = run_ensemble(args)

@@ -63,7 +63,9 @@ typedef struct mcac_box_params {
     int64_t member_pool;        /* member-list pool entries (0: auto)               */
     int32_t device;             /* HIP device ordinal                               */
     int32_t precision;          /* 0: FP64 throughout; 1: FP32 broad phase, FP64    */
-                                /*    final solve (results identical to 0)          */
+                                /*    final solve (results identical to 0); 2: as   */
+                                /*    1, and every entry the FP32 test drops is     */
+                                /*    also tested in FP64 to count misses           */
 } mcac_box_params;
 
 /* Result of a contact search (AggregateContactInfo, tools/contact_info.hpp:73-85). */
@@ -160,6 +162,10 @@ int mcac_box_timing(mcac_box *box, double out[8]);
  * use, merges done inside a queued move's launch, merges in their own
  * launch].  No reference counterpart (diagnostics). */
 int mcac_box_stats(mcac_box *box, int64_t out[8]);
+/* FP32 broad phase, counted with precision 2: [cell entries tested in FP32,
+ * entries passed on to the FP64 filter, entries dropped in FP32 that the FP64
+ * filter keeps (must be 0)].  No reference counterpart. */
+int mcac_box_prune_stats(mcac_box *box, int64_t out[3]);
 
 /* distance_to_contact(Sphere, Sphere, dir, dist) (src/spheres/sphere_contact.cpp:47-125)
  * evaluated on the device for n independent inputs: p1, p2, dir are (n,3)

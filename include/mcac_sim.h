@@ -41,7 +41,10 @@ typedef struct mcac_merge_rec {
 } mcac_merge_rec;
 
 /* ini_text: content of an MCAC parameter file.  output_dir: directory for
- * advancement.dat and params.ini (NULL or "" disables file output). */
+ * advancement.dat and params.ini (NULL or "" disables file output).
+ * Environment: MCAC_PRECISION (0, 1, 2) selects the box's contact-search
+ * precision (mcac_box_params.precision; results are identical);
+ * MCAC_PIPELINE_DEPTH (1..6, default 3) the MC steps kept in flight. */
 int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mcac_sim **out, char *err,
                     int errlen);
 int mcac_sim_destroy(mcac_sim *sim);

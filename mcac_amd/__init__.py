@@ -53,7 +53,8 @@ GPU_SYMBOLS = (
     "mcac_box_create", "mcac_box_destroy", "mcac_box_upload", "mcac_next_contact", "mcac_translate",
     "mcac_move", "mcac_merge", "mcac_move_enqueue", "mcac_move_result", "mcac_release", "mcac_box_set_auto_merge",
     "mcac_box_download_spheres", "mcac_box_download_aggs", "mcac_box_members",
-    "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_swept_sphere_batch", "mcac_version",
+    "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
+    "mcac_version",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
@@ -112,6 +113,7 @@ def load_gpu():
         L.mcac_box_set_timing.argtypes = [_P, _I]
         L.mcac_box_timing.argtypes = [_P, _DP]
         L.mcac_box_stats.argtypes = [_P, _I64P]
+        L.mcac_box_prune_stats.argtypes = [_P, _I64P]
         L.mcac_swept_sphere_batch.argtypes = [_I64, _DP, _DP, _DP, _DP, _DP, _DP, _D, _DP, _I]
         L.mcac_version.restype = ctypes.c_char_p
         _gpu = L
@@ -300,6 +302,12 @@ class Box:
         keys = ("search_ms", "translate_ms", "merge_ms", "n_search", "n_translate", "n_merge", "filter_ms", "solve_ms")
         return dict(zip(keys, out.tolist()))
 
+
+    def prune_stats(self):
+        """FP32 broad phase: entries tested, passed to FP64, dropped-but-FP64-kept (precision 2)."""
+        out = np.zeros(3, np.int64)
+        _check(load_gpu().mcac_box_prune_stats(self.h, _ip(out)), "mcac_box_prune_stats")
+        return dict(zip(("tested", "kept", "missed"), (int(v) for v in out)))
 
     def set_auto_merge(self, enable: bool):
         """Queued moves that find a contact merge in the same launch (small aggregates)."""

@@ -34,6 +34,7 @@ struct SearchState {
     int pad;
     unsigned long long gate;   // seq of a gated step that found a contact (0: open)
     unsigned long long tick;   // k_step blocks: finished << 56 | appended << 32 | visited
+    unsigned long long fp32_tested, fp32_kept, fp32_missed;  // FP32 broad-phase accounting
 };
 
 // Record published to pinned host memory by the last block of a search.
@@ -77,6 +78,7 @@ struct DevBox {
     int *cstart, *citems, *ccursor;
     double *cpx, *cpy, *cpz, *cpr;             // CoM and rmax in CSR order (rmax < 0: entry dropped)
     int *cslot;                                // CSR position of each aggregate (-1: none)
+    float4 *cf;                                // FP32 copy of (CoM, rmax) in CSR order (w < 0: dropped)
     unsigned char *dirty;                      // moved, merged or removed since the build
     int *dirty_list;                           // ids in the order they became dirty
     double *dpx, *dpy, *dpz, *dpr;             // their current CoM and rmax, by dirty-list index
@@ -120,7 +122,8 @@ struct SearchArgs {
     int gated;                 // pipelined step: closes the gate on contact, skipped behind it
     unsigned long long released;  // gate values up to this one are open for this step
     int automerge;             // merge in the same launch on contact when small enough
-    int pad2;
+    int prec;                  // 0 FP64; 1 FP32 capsule prune of cell entries; 2 as 1 plus miss count
+    float margin32;            // FP32 prune margin added to the radius sum [m]
     long long pool_used, pool_cap;  // member pool state when this step runs
 };
 

@@ -110,6 +110,7 @@ struct Simulation {
     std::string out_dir;
     RandomStream rng{0};
     int device = 0;
+    int precision = 0;  // box contact-search precision (mcac_box_params)
     mcac_box *box = nullptr;
     // spheres (index order) — host copy of the initial state only
     size_t N = 0;
@@ -293,6 +294,7 @@ struct Simulation {
         bp.max_spheres = static_cast<int64_t>(N);
         bp.max_aggregates = static_cast<int64_t>(N);
         bp.device = device;
+        bp.precision = precision;
         rc = mcac_box_create(&bp, &box);
         if (rc) return rc;
         std::vector<double> pos3(3 * N), rcm3(3 * N, 0.0);
@@ -630,6 +632,14 @@ int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mc
     }
     h->s.out_dir = output_dir ? output_dir : "";
     h->s.device = device;
+    if (const char *e = std::getenv("MCAC_PRECISION")) {  // 0 FP64, 1 FP32 broad phase, 2 with miss count
+        const long v = std::strtol(e, nullptr, 10);
+        h->s.precision = static_cast<int>(v < 0 ? 0 : (v > 2 ? 2 : v));
+    }
+    if (const char *e = std::getenv("MCAC_PIPELINE_DEPTH")) {  // steps kept in flight, 1..6
+        const long v = std::strtol(e, nullptr, 10);
+        h->s.depth = static_cast<size_t>(v < 1 ? 1 : (v > 6 ? 6 : v));
+    }
     *out = h;
     return 0;
 }

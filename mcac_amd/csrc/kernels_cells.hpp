@@ -65,6 +65,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(DevBox b, int n_alive) 
             b.cpy[pos] = b.ay[id];
             b.cpz[pos] = b.az[id];
             b.cpr[pos] = b.armax[id];
+            b.cf[pos] = make_float4(static_cast<float>(b.ax[id]), static_cast<float>(b.ay[id]),
+                                    static_cast<float>(b.az[id]), static_cast<float>(b.armax[id]));
         }
         b.cslot[id] = pos;
     }

@@ -135,7 +135,10 @@ __device__ __forceinline__ int window_offset(int c, int b1, int n) {
 __device__ __forceinline__ void mark_dirty(const DevBox &b, int id) {
     if (!b.dirty[id]) {
         const int cs = b.cslot[id];
-        if (cs >= 0) b.cpr[cs] = -1.0;
+        if (cs >= 0) {
+            b.cpr[cs] = -1.0;
+            b.cf[cs].w = -1.f;
+        }
         b.dirty[id] = 1;
         unsigned int k = b.ss->dirty_count;
         b.dirty_list[k] = id;

@@ -239,7 +239,10 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         // both join the dirty list (keeper first) and leave the packed cell list
         unsigned int cnt = dcount;
         if (!dk) {
-            if (csk >= 0) b.cpr[csk] = -1.0;
+            if (csk >= 0) {
+                b.cpr[csk] = -1.0;
+                b.cf[csk].w = -1.f;
+            }
             b.dirty[k] = 1;
             b.dirty_list[cnt] = k;
             b.dslot[k] = static_cast<int>(cnt);
@@ -247,7 +250,10 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
             ++cnt;
         }
         if (!dr) {
-            if (csr_slot >= 0) b.cpr[csr_slot] = -1.0;
+            if (csr_slot >= 0) {
+                b.cpr[csr_slot] = -1.0;
+                b.cf[csr_slot].w = -1.f;
+            }
             b.dirty[a.rem] = 1;
             b.dirty_list[cnt] = a.rem;
             b.dslot[a.rem] = static_cast<int>(cnt);

@@ -31,6 +31,28 @@ __device__ __forceinline__ void append_candidate(const DevBox &b, const SearchAr
     }
 }
 
+// FP32 broad phase: can the candidate's bounding sphere (centre c, radius rc)
+// come within the mover's radius rs of the sweep segment p1 + t dir, t in
+// [0, dist]?  Every swept-sphere contact within dist passes (the contact point
+// lies on the segment); with dist + rs + rc < L/2 only the minimum image can
+// be that close, and `margin` (far above FP32 rounding of box-sized
+// coordinates) keeps the test conservative.
+__device__ __forceinline__ bool capsule_pass32(float p1x, float p1y, float p1z, float rs, float4 c, float dx_,
+                                               float dy_, float dz_, float dist, float L, float margin) {
+    const float half = 0.5f * L;
+    float d[3] = {c.x - p1x, c.y - p1y, c.z - p1z};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        if (d[l] < -half) d[l] += L;
+        else if (d[l] >= half) d[l] -= L;
+    }
+    float t = d[0] * dx_ + d[1] * dy_ + d[2] * dz_;
+    t = t < 0.f ? 0.f : (t > dist ? dist : t);
+    const float ex = d[0] - t * dx_, ey = d[1] - t * dy_, ez = d[2] - t * dz_;
+    const float R = rs + c.w + margin;
+    return ex * ex + ey * ey + ez * ez <= R * R;
+}
+
 // Per-thread counts of the filter: appended candidates << 32 | aggregates in the window.
 constexpr unsigned long long kAppendOne = 1ull << 32;
 constexpr unsigned long long kTicketOne = 1ull << 56;
@@ -144,6 +166,10 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
     // one index space: cell entries, then the aggregates moved since the build
     // (at their current cell), so that both kinds run side by side
     const int nd = a.dirty_n;
+    unsigned long long n32 = 0, kept32 = 0, miss32 = 0;
+    const float fx = static_cast<float>(sx), fy = static_cast<float>(sy), fz = static_cast<float>(sz);
+    const float fr = static_cast<float>(sr), fdx = static_cast<float>(a.dir[0]), fdy = static_cast<float>(a.dir[1]);
+    const float fdz = static_cast<float>(a.dir[2]), flpm = static_cast<float>(a.lpm), fL = static_cast<float>(a.L);
     for (int w = gtid; w < total + nd; w += gstride) {
         int id, rank;
         double x, y, z, r;
@@ -160,6 +186,23 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
                 pos = cbeg[lo] + (w - pre[lo]);
             } else {
                 pos = w;
+            }
+            if (a.prec > 0) {
+                // FP32 broad phase: 16 B per entry; only its survivors read the FP64 row
+                const float4 f = b.cf[pos];
+                if (f.w < 0.f) continue;  // moved since the build
+                const bool pass = capsule_pass32(fx, fy, fz, fr, f, fdx, fdy, fdz, flpm, fL, a.margin32);
+                n32 += 1;
+                if (!pass) {
+                    if (a.prec == 2) {  // would the FP64 filter have kept it?
+                        const int id2 = b.citems[pos];
+                        const double d2 = swept_sphere(sx, sy, sz, sr, b.cpx[pos], b.cpy[pos], b.cpz[pos],
+                                                       b.cpr[pos], a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
+                        if (id2 != a.src && d2 < a.lpm) miss32 += 1;
+                    }
+                    continue;
+                }
+                kept32 += 1;
             }
             id = b.citems[pos];
             x = b.cpx[pos];
@@ -193,6 +236,16 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
         if (d < a.lpm) {
             append_candidate(b, a, id, rank, d);
             cnt += kAppendOne;
+        }
+    }
+    if (a.prec == 2) {  // FP32 broad-phase accounting, one atomic per block and counter
+        n32 = block_sum(n32);
+        kept32 = block_sum(kept32);
+        miss32 = block_sum(miss32);
+        if (threadIdx.x == 0) {
+            atomicAdd(&b.ss->fp32_tested, n32);
+            atomicAdd(&b.ss->fp32_kept, kept32);
+            if (miss32) atomicAdd(&b.ss->fp32_missed, miss32);
         }
     }
     return cnt;
@@ -319,7 +372,10 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
                 b.dirty_list[a.dirty_slot] = a.src;
                 b.dslot[a.src] = a.dirty_slot;
                 s->dirty_count = static_cast<unsigned int>(a.dirty_slot + 1);
-                if (t.src_slot >= 0) b.cpr[t.src_slot] = -1.0;
+                if (t.src_slot >= 0) {
+                    b.cpr[t.src_slot] = -1.0;
+                    b.cf[t.src_slot].w = -1.f;
+                }
             }
             set_dirty_entry(b, a.dirty_pos, s->newcom[0], s->newcom[1], s->newcom[2],
                             a.known ? a.srmax : b.armax[a.src], s->newcell);

@@ -291,6 +291,11 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     a.gated = gated;
     a.released = b->released;
     a.automerge = (b->automerge && gated && apply) ? 1 : 0;
+    // FP32 broad phase only where the minimum image decides (sweep + radii < 0.4 L)
+    a.prec = (b->p.precision > 0 && a.known && lpm + a.srmax + maxradius < 0.4 * b->p.box_length)
+                 ? static_cast<int>(std::min<int32_t>(b->p.precision, 2))
+                 : 0;
+    a.margin32 = static_cast<float>(1e-5 * b->p.box_length);
     a.pool_used = b->pool_used;
     a.pool_cap = b->pool_cap;
     if (b->slot_timed[slot] == 2) {
@@ -534,7 +539,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.members, b->pool_cap);
     ALLOC(d.alive, Na);
     ALLOC(d.cstart, b->n_cells + 1); ALLOC(d.citems, Na); ALLOC(d.ccursor, b->n_cells + 1);
-    ALLOC(d.cpx, Na); ALLOC(d.cpy, Na); ALLOC(d.cpz, Na); ALLOC(d.cpr, Na); ALLOC(d.cslot, Na);
+    ALLOC(d.cpx, Na); ALLOC(d.cpy, Na); ALLOC(d.cpz, Na); ALLOC(d.cpr, Na); ALLOC(d.cslot, Na); ALLOC(d.cf, Na);
     ALLOC(d.dirty, Na); ALLOC(d.dirty_list, Na + 1);
     ALLOC(d.dpx, Na + 1); ALLOC(d.dpy, Na + 1); ALLOC(d.dpz, Na + 1); ALLOC(d.dpr, Na + 1);
     ALLOC(d.dpcell, Na + 1); ALLOC(d.dslot, Na);
@@ -969,6 +974,17 @@ int mcac_box_stats(mcac_box *b, int64_t out[8]) {
     out[3] = b->n_rebuilds;
     out[4] = b->n_compactions;
     out[5] = b->pool_used;
+    return 0;
+}
+
+int mcac_box_prune_stats(mcac_box *b, int64_t out[3]) {
+    if (!b || !out) return MCAC_E_ARGUMENT;
+    SearchState s{};
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(&s, b->d.ss, sizeof(SearchState), hipMemcpyDeviceToHost));
+    out[0] = static_cast<int64_t>(s.fp32_tested);
+    out[1] = static_cast<int64_t>(s.fp32_kept);
+    out[2] = static_cast<int64_t>(s.fp32_missed);
     return 0;
 }
 
