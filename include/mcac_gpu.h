@@ -43,6 +43,7 @@ extern "C" {
 /* Status codes. 0..11 mirror the reference ErrorCodes. */
 #define MCAC_NO_ERROR 0
 #define MCAC_UNKNOWN_ERROR 1
+#define MCAC_IO_ERROR 2
 #define MCAC_VERLET_ERROR 3
 #define MCAC_INPUT_ERROR 4
 #define MCAC_VOL_SURF_ERROR 8
@@ -144,6 +145,10 @@ int mcac_box_download_spheres(mcac_box *box, double *x, double *y, double *z, do
                               double *rz);
 int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rmax, double *volume,
                            double *surface, double *rg, int64_t *count);
+/* Stable id of the aggregate owning each sphere (n_spheres entries; -1 for a
+ * sphere no live aggregate owns): the spheres' aggregate labels written by
+ * SphereList::save (src/io/sphere_list.cpp:44-46) once mapped to labels. */
+int mcac_box_sphere_owners(mcac_box *box, int64_t *owner);
 /* Members of one aggregate in list order; returns the count via *n. */
 int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int64_t *n);
 

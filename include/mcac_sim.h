@@ -85,6 +85,10 @@ int mcac_physics_scalars(const char *ini_text, double out[8]);
  * picks so far that landed in a run of equal keys. */
 int64_t mcac_pick_table_selftest(int64_t n, int64_t events, int64_t chunk, uint64_t seed, int mode);
 int64_t mcac_pick_table_selftest_ties(void);
+/* Snapshot-writer check (host only): `grids` synthetic sphere grids of n
+ * spheres written to <prefix>_<number>.{h5,xmf}, per_file grids per file;
+ * position k of grid g = ((g + k) / 8, 2k / 8, 3k / 8), Radius k / 4, Label k. */
+int mcac_write_test_snapshots(const char *prefix, int64_t n, int64_t grids, int64_t per_file);
 
 /* Ensemble / serving runs (no reference counterpart: the reference runs one
  * box per process, src/main.cpp): n_jobs parameter files run to completion

@@ -52,7 +52,7 @@ MERGE_RESULT_DTYPE = np.dtype([
 GPU_SYMBOLS = (
     "mcac_box_create", "mcac_box_destroy", "mcac_box_upload", "mcac_next_contact", "mcac_translate",
     "mcac_move", "mcac_merge", "mcac_move_enqueue", "mcac_move_result", "mcac_release", "mcac_box_set_auto_merge",
-    "mcac_box_download_spheres", "mcac_box_download_aggs", "mcac_box_members",
+    "mcac_box_download_spheres", "mcac_box_sphere_owners", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version",
 )
@@ -61,6 +61,7 @@ SIM_SYMBOLS = (
     "mcac_sim_set_trace", "mcac_sim_n_steps", "mcac_sim_get_steps", "mcac_sim_n_merges",
     "mcac_sim_get_merges", "mcac_sim_scalar", "mcac_sim_n_aggs", "mcac_sim_export_aggs", "mcac_sim_box",
     "mcac_physics_scalars", "mcac_pick_table_selftest", "mcac_pick_table_selftest_ties", "mcac_batch_run",
+    "mcac_write_test_snapshots",
 )
 
 _P = ctypes.c_void_p
@@ -110,6 +111,7 @@ def load_gpu():
         L.mcac_box_download_spheres.argtypes = [_P] + [_DP] * 6
         L.mcac_box_download_aggs.argtypes = [_P] + [_DP] * 6 + [_I64P]
         L.mcac_box_members.argtypes = [_P, _I64, _I64P, _I64, _I64P]
+        L.mcac_box_sphere_owners.argtypes = [_P, _I64P]
         L.mcac_box_set_timing.argtypes = [_P, _I]
         L.mcac_box_timing.argtypes = [_P, _DP]
         L.mcac_box_stats.argtypes = [_P, _I64P]
@@ -150,6 +152,7 @@ def load_sim():
         L.mcac_pick_table_selftest.restype = _I64
         L.mcac_pick_table_selftest.argtypes = [_I64, _I64, _I64, ctypes.c_uint64, _I]
         L.mcac_pick_table_selftest_ties.restype = _I64
+        L.mcac_write_test_snapshots.argtypes = [ctypes.c_char_p, _I64, _I64, _I64]
         L.mcac_batch_run.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I64, _DP, _DP, _I64P,
                                      ctypes.POINTER(_I)]
         _sim = L

@@ -25,6 +25,8 @@ HIP_FLAGS = [
 ]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-std=c++17", "-O3", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-Wall", "-Wextra"]
+# HDF5 C library for the snapshot files (present in the image under /opt/conda)
+HDF5_ROOT = pathlib.Path(os.environ.get("MCAC_HDF5_ROOT", "/opt/conda"))
 
 
 def _run(cmd):
@@ -41,11 +43,14 @@ def build(verbose: bool = False) -> None:
     tmp = LIB / "libmcac_gpu.so.tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), str(CSRC / "mcac_gpu.hip")])
     os.replace(tmp, gpu)
-    host_src = [str(CSRC / "host" / f) for f in ("physics.cpp", "pick_table.cpp", "sim.cpp", "batch.cpp")]
+    host_src = [str(CSRC / "host" / f) for f in ("physics.cpp", "pick_table.cpp", "sim.cpp", "batch.cpp",
+                                                  "xdmf_writer.cpp")]
+    if not (HDF5_ROOT / "include" / "hdf5.h").exists():
+        raise RuntimeError(f"HDF5 headers not found under {HDF5_ROOT} (set MCAC_HDF5_ROOT)")
     sim = LIB / "libmcac_sim.so"
     tmp = LIB / "libmcac_sim.so.tmp"
-    _run([CXX, *CXX_FLAGS, "-shared", "-o", str(tmp), *host_src, f"-L{LIB}", "-lmcac_gpu",
-          "-Wl,-rpath,$ORIGIN"])
+    _run([CXX, *CXX_FLAGS, f"-I{HDF5_ROOT / 'include'}", "-shared", "-o", str(tmp), *host_src, f"-L{LIB}",
+          "-lmcac_gpu", f"-L{HDF5_ROOT / 'lib'}", "-lhdf5", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{HDF5_ROOT / 'lib'}"])
     os.replace(tmp, sim)
     exe = BIN / "mcac_run"
     _run([CXX, *CXX_FLAGS, "-o", str(exe), str(CSRC / "host" / "main.cpp"), f"-L{LIB}", "-lmcac_sim",

@@ -157,6 +157,7 @@ void PhysicalModel::load(const IniFile &ini) {
     ini.extract("output", "write_between_event_frequency", write_between_event_frequency);
     ini.extract("output", "write_events_frequency", write_events_frequency);
     ini.extract("output", "write_Delta_t", write_Delta_t);
+    ini.extract("output", "n_time_per_file", n_time_per_file);
     derive();
 }
 

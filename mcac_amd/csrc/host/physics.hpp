@@ -82,6 +82,7 @@ struct PhysicalModel {
     int n_iter_without_event_limit = -1;
     int random_seed = -1;
     size_t write_events_frequency = 1, write_between_event_frequency = 100;
+    size_t n_time_per_file = 10;
     bool with_domain_duplication = true, with_collisions = true, enforce_volume_fraction = true;
     std::string output_dir = "MCAC_output";
     std::string unsupported;  // first option this path does not implement

@@ -14,6 +14,7 @@
 #include <memory>
 #include <numeric>
 #include <random>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -21,6 +22,7 @@
 #include "../../../include/mcac_sim.h"
 #include "physics.hpp"
 #include "pick_table.hpp"
+#include "xdmf_writer.hpp"
 
 namespace mcac {
 namespace {
@@ -133,6 +135,8 @@ struct Simulation {
     double t_tie = 0, t_enqueue = 0, t_wait = 0, t_merge_call = 0, t_drain = 0;
     bool initialized = false;
     bool final_written = false;
+    std::vector<double> sphere_r;                            // radii (constant on this path)
+    std::unique_ptr<SnapshotWriter> out_spheres, out_aggs;  // snapshot files when out_dir is set
 
     ~Simulation() {
         if (box) mcac_box_destroy(box);
@@ -317,7 +321,98 @@ struct Simulation {
             mkdir(out_dir.c_str(), 0755);
             std::ofstream os(out_dir + "/params.ini");
             os << IniFile(ini_text).dump();
+            // sphere_list_storage.cpp:74, aggregat_list_storage.cpp:62
+            out_spheres = std::make_unique<SnapshotWriter>(out_dir + "/Spheres", N, pm.n_time_per_file);
+            out_aggs = std::make_unique<SnapshotWriter>(out_dir + "/Aggregats", N, pm.n_time_per_file);
+            sphere_r = sr;
+            depth = 1;  // a snapshot is the state at the top of the loop: no step ahead of it
         }
+        return 0;
+    }
+
+    // physical_model.cpp (io) xmf_write: the "Physics" information, written as
+    // `stream << value` does (six significant digits)
+    std::vector<std::pair<std::string, std::string>> physics_info() {
+        refresh_pm();
+        auto f = [](double v) {
+            std::ostringstream s;
+            s << v;
+            return s.str();
+        };
+        return {{"flux_surfgrowth", f(pm.flux_surfgrowth)}, {"u_sg", f(pm.u_sg)},
+                {"dfe", f(pm.fractal_dimension)},           {"kfe", f(pm.fractal_prefactor)},
+                {"lambda", f(pm.gaz_mean_free_path)},       {"rpeqmass", f(pm.mean_massic_radius)},
+                {"gamma_", f(pm.friction_exponnant)},       {"P [Pa]", f(pm.pressure)},
+                {"T [K]", f(pm.temperature)},               {"Mu", f(pm.viscosity)},
+                {"Rho [kg/m3]", f(pm.density)},             {"Dpm [nm]", f(pm.mean_diameter)},
+                {"sigmaDpm [nm]", f(pm.dispersion_diameter)}, {"FV [ppt]", f(pm.volume_fraction)},
+                {"L", f(pm.box_length)},                    {"N []", std::to_string(static_cast<int>(pm.n_monomeres))}};
+    }
+
+    // SphereList::save and AggregatList::save (src/io/sphere_list.cpp:26-56,
+    // src/io/aggregat_list.cpp:26-60): one grid each, in list order
+    int save_snapshot() {
+        if (!out_spheres || !out_aggs) return 0;
+        const auto phys = physics_info();
+        const size_t S = N;
+        std::vector<double> x(S), y(S), z(S);
+        int rc = mcac_box_download_spheres(box, x.data(), y.data(), z.data(), nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        std::vector<int64_t> owner(S);
+        rc = mcac_box_sphere_owners(box, owner.data());
+        if (rc) return rc;
+        std::vector<double> pos3(3 * S);
+        std::vector<int64_t> slabel(S);
+        std::vector<int32_t> zeros_s(S, 0);
+        for (size_t i = 0; i < S; ++i) {
+            pos3[3 * i] = x[i];
+            pos3[3 * i + 1] = y[i];
+            pos3[3 * i + 2] = z[i];
+            slabel[i] = owner[i] >= 0 ? label_of(static_cast<int>(owner[i])) : -1;
+        }
+        out_spheres->write("Spheres", pm.time, pos3,
+                           {{"electric_charge", 1, zeros_s.data(), S},
+                            {"Radius", 0, sphere_r.data(), S},
+                            {"Label", 2, slabel.data(), S}},
+                           phys);
+        // aggregates in label order (ascending live id)
+        std::vector<double> apos(3 * N), arcm(3 * N), armax(N), avol(N), asurf(N), arg(N);
+        std::vector<int64_t> acount(N);
+        rc = mcac_box_download_aggs(box, apos.data(), arcm.data(), armax.data(), avol.data(), asurf.data(), arg.data(),
+                                    acount.data());
+        if (rc) return rc;
+        const std::vector<int> &ids = live.ids();
+        const size_t A = ids.size();
+        std::vector<double> p3(3 * A), c_rg(A), c_fagg(A), c_lpm(A), c_dt(A), c_rmax(A), c_vol(A), c_surf(A), c_pt(A),
+            c_coord(A), c_ovl(A), c_dm(A);
+        std::vector<int64_t> c_np(A), c_label(A);
+        std::vector<int32_t> zeros_a(A, 0);
+        for (size_t l = 0; l < A; ++l) {
+            const size_t id = static_cast<size_t>(ids[l]);
+            for (int k = 0; k < 3; ++k) p3[3 * l + static_cast<size_t>(k)] = apos[3 * id + static_cast<size_t>(k)];
+            c_rg[l] = rg[id];
+            c_np[l] = nsph[id];
+            c_fagg[l] = f_agg[id];
+            c_lpm[l] = lpm[id];
+            c_dt[l] = ts[id];
+            c_rmax[l] = rmax[id];
+            c_vol[l] = vol[id];
+            c_surf[l] = surf[id];
+            c_pt[l] = ptime[id];
+            c_coord[l] = coord[id];
+            c_ovl[l] = overlapping[id];
+            c_dm[l] = d_m[id];
+            c_label[l] = static_cast<int64_t>(l);
+        }
+        out_aggs->write("Aggregats", pm.time, p3,
+                        {{"Rg", 0, c_rg.data(), A},           {"Np", 2, c_np.data(), A},
+                         {"f_agg", 0, c_fagg.data(), A},      {"lpm", 0, c_lpm.data(), A},
+                         {"Deltat", 0, c_dt.data(), A},       {"Rmax", 0, c_rmax.data(), A},
+                         {"Volume", 0, c_vol.data(), A},      {"Surface", 0, c_surf.data(), A},
+                         {"proper_time", 0, c_pt.data(), A},  {"coordination_number", 0, c_coord.data(), A},
+                         {"overlapping", 0, c_ovl.data(), A}, {"electric_charge", 1, zeros_a.data(), A},
+                         {"d_m", 0, c_dm.data(), A},          {"Label", 2, c_label.data(), A}},
+                        phys);
         return 0;
     }
 
@@ -465,7 +560,11 @@ struct Simulation {
             rc = drain_skipped();  // the steps behind it did nothing; issue them again
             if (rc) return rc;
         }
-        if (pm.time_to_write(total_events)) save_advancement();
+        if (pm.time_to_write(total_events)) {  // calcul.cpp:67-71
+            rc = save_snapshot();
+            if (rc) return rc;
+            save_advancement();
+        }
         if (event && pm.with_domain_duplication && live.size() <= pm.n_monomeres / 8) return MCAC_INPUT_ERROR;
         *done = true;
         const int num = P.num;
@@ -666,7 +765,10 @@ int mcac_sim_run(mcac_sim *sim, int64_t max_steps, int64_t *done) {
     try {
         rc = sim->s.run(max_steps, &n);
         if (rc == 0 && sim->s.finished() && !sim->s.final_written) {
-            sim->s.save_advancement();  // calcul.cpp:282
+            sim->s.save_advancement();  // calcul.cpp:282-284
+            rc = sim->s.save_snapshot();
+            if (sim->s.out_spheres) sim->s.out_spheres->close();
+            if (sim->s.out_aggs) sim->s.out_aggs->close();
             sim->s.final_written = true;
         }
     } catch (const std::exception &e) {
@@ -763,6 +865,36 @@ int mcac_sim_export_aggs(mcac_sim *sim, int64_t *uid, double *lpm, double *time_
 }
 
 mcac_box *mcac_sim_box(mcac_sim *sim) { return sim ? sim->s.box : nullptr; }
+
+int mcac_write_test_snapshots(const char *prefix, int64_t n, int64_t grids, int64_t per_file) {
+    // synthetic grids through the snapshot writer: position k of grid g is
+    // (g + k, 2k, 3k) / 8, Radius k / 4, Label k, electric_charge 0
+    if (!prefix || n < 1 || grids < 0 || per_file < 1) return MCAC_E_ARGUMENT;
+    try {
+        mcac::SnapshotWriter w(prefix, static_cast<size_t>(n), static_cast<size_t>(per_file));
+        const size_t S = static_cast<size_t>(n);
+        std::vector<double> pos3(3 * S), rad(S);
+        std::vector<int64_t> lab(S);
+        std::vector<int32_t> q(S, 0);
+        for (int64_t g = 0; g < grids; ++g) {
+            for (size_t k = 0; k < S; ++k) {
+                pos3[3 * k] = static_cast<double>(g + static_cast<int64_t>(k)) / 8.0;
+                pos3[3 * k + 1] = 2.0 * static_cast<double>(k) / 8.0;
+                pos3[3 * k + 2] = 3.0 * static_cast<double>(k) / 8.0;
+                rad[k] = static_cast<double>(k) / 4.0;
+                lab[k] = static_cast<int64_t>(k);
+            }
+            w.write("Spheres", 0.5 * static_cast<double>(g), pos3,
+                    {{"electric_charge", 1, q.data(), S}, {"Radius", 0, rad.data(), S}, {"Label", 2, lab.data(), S}},
+                    {{"L", "1e-06"}, {"N []", std::to_string(n)}});
+        }
+        w.close();
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "mcac_write_test_snapshots: %s\n", e.what());
+        return MCAC_IO_ERROR;
+    }
+    return 0;
+}
 
 static int64_t selftest_ties = 0;
 

@@ -596,6 +596,15 @@ __global__ void k_gather_members(DevBox b, int agg, int *out, int n) {
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) out[t] = b.members[st + t];
 }
 
+// Aggregate id of every sphere: one block per live aggregate scatters its id.
+__global__ void k_sphere_owner(DevBox b, const int *ids, int n_ids, int *owner) {
+    for (int a = blockIdx.x; a < n_ids; a += gridDim.x) {
+        const int id = ids[a];
+        const int n = b.acount[id], st = b.astart[id];
+        for (int t = threadIdx.x; t < n; t += blockDim.x) owner[b.members[st + t]] = id;
+    }
+}
+
 // Pool compaction: copy each live aggregate's members to its new start.
 __global__ void k_compact(const int *old_members, int *new_members, const int *old_start, const int *new_start,
                           const int *count, const int *ids, int n_ids) {

@@ -955,6 +955,30 @@ int mcac_box_members(mcac_box *b, int64_t agg, int64_t *out, int64_t max, int64_
     return 0;
 }
 
+int mcac_box_sphere_owners(mcac_box *b, int64_t *owner) {
+    if (!b || !owner || b->inflight) return MCAC_E_ARGUMENT;
+    std::vector<int> ids;
+    for (size_t id = 0; id < b->count.size(); ++id)
+        if (b->count[id] > 0) ids.push_back(static_cast<int>(id));
+    const size_t S = static_cast<size_t>(b->n_spheres);
+    int *d_ids = nullptr, *d_own = nullptr;
+    HIP_TRY(hipMalloc(&d_ids, ids.size() * sizeof(int) + 4));
+    HIP_TRY(hipMalloc(&d_own, S * sizeof(int) + 4));
+    HIP_TRY(hipMemset(d_own, 0xff, S * sizeof(int) + 4));
+    HIP_TRY(hipMemcpy(d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!ids.empty())
+        hipLaunchKernelGGL(k_sphere_owner, dim3(static_cast<unsigned>(std::min<size_t>(ids.size(), 65535))),
+                           dim3(kBlock), 0, b->stream, b->d, d_ids, static_cast<int>(ids.size()), d_own);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<int> h(S);
+    if (S) HIP_TRY(hipMemcpy(h.data(), d_own, S * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipFree(d_ids));
+    HIP_TRY(hipFree(d_own));
+    for (size_t i = 0; i < S; ++i) owner[i] = h[i];
+    return 0;
+}
+
 int mcac_box_set_timing(mcac_box *b, int enable) {
     if (!b) return MCAC_E_ARGUMENT;
     b->timing = enable < 0 ? 0 : enable;
