@@ -85,6 +85,41 @@ def run_ensemble(args) -> dict:
                       f"FV U[1e-4,5e-3], T U[1000,2000] K, until N_agg <= N/10), all started at t=0"}
 
 
+def run_ensemble_compare(args) -> dict:
+    """Runs/hour of the same boxes on the GPU (batch runner) and on the CPU
+    (the oracle, one thread per run, P = min(cores, 16, jobs) runs at a time):
+    the boxes of the ensemble sample with N <= --compare-nmax, so that the CPU
+    side stays a bounded sample."""
+    from concurrent.futures import ThreadPoolExecutor
+    from mcac_amd import ensemble
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    boxes = [b for b in ensemble.ensemble_configs(args.ensemble_boxes) if b["n"] <= args.compare_nmax]
+    inis = [b["ini"] for b in boxes]
+    print(f"bench: ensemble comparison on {len(boxes)} boxes (N <= {args.compare_nmax:g})", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    r = ensemble.batch_run(inis, workers=args.workers, max_steps=args.max_job_steps)
+    gpu_wall = time.perf_counter() - t0
+
+    def one(ini):  # ctypes drops the GIL: the oracle runs proceed in parallel
+        sim = oracle_lib.OracleSim(ini, fast_init=True, trace=False)
+        n = sim.run(args.max_job_steps)
+        return n, sim.finished()
+
+    procs = max(1, min(len(os.sched_getaffinity(0)), 16, len(inis)))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(procs) as ex:
+        cpu = list(ex.map(one, inis))
+    cpu_wall = time.perf_counter() - t0
+    same = all(int(c[0]) == int(g) for c, g in zip(cpu, r["steps"]))
+    return {"boxes": len(boxes), "gpu_runs_per_hour": 3600.0 * len(boxes) / gpu_wall, "gpu_wall_s": gpu_wall,
+            "gpu_workers": args.workers, "cpu_runs_per_hour": 3600.0 * len(boxes) / cpu_wall, "cpu_wall_s": cpu_wall,
+            "cpu_threads": procs, "ratio": cpu_wall / gpu_wall, "same_step_counts": same,
+            "sample": f"boxes with N <= {args.compare_nmax:g} among boxes 0..{args.ensemble_boxes - 1} of the "
+                      f"512-box list, run to N_agg <= N/10 on both sides; CPU = the oracle, one thread per run"}
+
+
 def run_serving(args) -> dict:
     """Serving form: Poisson arrivals, time-to-result percentiles."""
     import numpy as np
@@ -115,6 +150,8 @@ def main() -> int:
                     help="only run a plain 256 MiB device copy 20 times (HBM counter calibration)")
     ap.add_argument("--ensemble-boxes", type=int, default=16,
                     help="boxes of the 512-box ensemble list run to completion (0: skip)")
+    ap.add_argument("--compare-nmax", type=float, default=8000,
+                    help="largest box of the ensemble sample also run on the CPU for the runs/hour comparison")
     ap.add_argument("--serving-requests", type=int, default=16, help="serving requests (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=4.0, help="serving arrival rate, requests/s")
     ap.add_argument("--serving-nmax", type=float, default=2e4,
@@ -229,6 +266,8 @@ def main() -> int:
         out["ensemble"] = run_ensemble(args)
     if args.serving_requests > 0:
         out["serving"] = run_serving(args)
+    if args.ensemble_boxes > 0 and not args.no_cpu:
+        out["ensemble_vs_cpu"] = run_ensemble_compare(args)
     if not args.no_cpu:
         cb = cpu_baseline(text, args.cpu_steps)
         cb["ratio"] = steps_per_s / cb["value"]

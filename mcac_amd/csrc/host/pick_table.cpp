@@ -483,6 +483,79 @@ inline size_t partition_pivot(KeyLabel *a, size_t f, size_t l) {
     }
 }
 
+// The same partition computed by the worker pool.  With L_1 < L_2 < ... the
+// positions in (f, l) whose key is not below the pivot p and R_1 > R_2 > ...
+// the positions in [f, l) whose key is not above it, the loop above swaps
+// L_t with R_t for every t before the first t* with L_t* >= R_t*, and returns
+// min(L_t*, R_{t*-1}) (R_0 = l, L_t = +inf past the last L): while L_t < R_t
+// the scans meet only positions no swap has touched yet, and the swapped-in
+// elements stop them exactly where the sequential scans stop.
+constexpr size_t kParallelPartition = size_t(1) << 16;
+constexpr size_t kPartitionBlock = size_t(1) << 14;
+
+size_t partition_pivot_parallel(KeyLabel *a, size_t f, size_t l, std::vector<uint32_t> &Lp,
+                                std::vector<uint32_t> &Rp) {
+    move_median_to_first(a, f, f + 1, f + (l - f) / 2, l - 1);
+    const double p = a[f].k;
+    const size_t n = l - f;
+    const size_t nb = (n + kPartitionBlock - 1) / kPartitionBlock;
+    std::vector<size_t> cl(nb + 1, 0), cr(nb + 1, 0);
+    auto count = [&](size_t b) {
+        const size_t s = f + b * kPartitionBlock, e = std::min(l, s + kPartitionBlock);
+        size_t x = 0, y = 0;
+        for (size_t j = s; j < e; ++j) {
+            x += (a[j].k < p) ? 0 : 1;
+            y += (p < a[j].k) ? 0 : 1;
+        }
+        if (b == 0) x -= 1;  // the pivot's own position is not an L position
+        cl[b + 1] = x;
+        cr[b + 1] = y;
+    };
+    Workers::get().run(nb, count);
+    for (size_t b = 0; b < nb; ++b) {
+        cl[b + 1] += cl[b];
+        cr[b + 1] += cr[b];
+    }
+    const size_t nl = cl[nb], nr = cr[nb];
+    Lp.resize(nl);
+    Rp.resize(nr);
+    auto list = [&](size_t b) {  // ascending positions relative to f, branch-free through a block buffer
+        thread_local std::vector<uint32_t> bl, br;
+        bl.resize(kPartitionBlock + 1);
+        br.resize(kPartitionBlock + 1);
+        const size_t s = f + b * kPartitionBlock, e = std::min(l, s + kPartitionBlock);
+        size_t x = 0, y = 0;
+        for (size_t j = s; j < e; ++j) {
+            const uint32_t r = static_cast<uint32_t>(j - f);
+            bl[x] = r;
+            x += (a[j].k < p || j == f) ? 0 : 1;
+            br[y] = r;
+            y += (p < a[j].k) ? 0 : 1;
+        }
+        std::copy(bl.begin(), bl.begin() + static_cast<std::ptrdiff_t>(x), Lp.begin() + static_cast<std::ptrdiff_t>(cl[b]));
+        std::copy(br.begin(), br.begin() + static_cast<std::ptrdiff_t>(y), Rp.begin() + static_cast<std::ptrdiff_t>(cr[b]));
+    };
+    Workers::get().run(nb, list);
+    // L_t = Lp[t-1], R_t = Rp[nr-t]; first t with L_t >= R_t (nr >= 1: f qualifies)
+    auto crossed = [&](size_t t) { return t > nl || Lp[t - 1] >= Rp[nr - t]; };
+    size_t lo = 1, hi = nr;  // crossed(nr) holds: R_nr = f < any L
+    while (lo < hi) {
+        const size_t mid = lo + (hi - lo) / 2;
+        if (crossed(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    const size_t ts = lo;  // t*
+    const size_t nsw = ts - 1;
+    auto swaps = [&](size_t b) {
+        const size_t s = b * kPartitionBlock, e = std::min(nsw, s + kPartitionBlock);
+        for (size_t t = s + 1; t <= e; ++t) std::swap(a[f + Lp[t - 1]], a[f + Rp[nr - t]]);
+    };
+    Workers::get().run((nsw + kPartitionBlock - 1) / kPartitionBlock, swaps);
+    const size_t Lts = ts > nl ? l : f + Lp[ts - 1];
+    const size_t Rprev = ts == 1 ? l : f + Rp[nr - (ts - 1)];
+    return std::min(Lts, Rprev);
+}
+
 int floor_log2(size_t n) {
     int r = 0;
     while (n > 1) {
@@ -493,7 +566,8 @@ int floor_log2(size_t n) {
 }
 
 // Positions, after the partition phase, of the entries whose key equals v in [f, l).
-void track_equal(KeyLabel *a, size_t f, size_t l, int depth, double v, std::vector<std::pair<size_t, int>> &out) {
+void track_equal(KeyLabel *a, size_t f, size_t l, int depth, double v, std::vector<std::pair<size_t, int>> &out,
+                 std::vector<uint32_t> &Lp, std::vector<uint32_t> &Rp) {
     auto count_in = [a, v](size_t x, size_t y) {
         size_t c = 0;
         for (size_t j = x; j < y; ++j) c += (a[j].k == v) ? 1 : 0;
@@ -505,12 +579,13 @@ void track_equal(KeyLabel *a, size_t f, size_t l, int depth, double v, std::vect
             break;
         }
         --depth;
-        const size_t cut = partition_pivot(a, f, l);
+        const size_t cut =
+            (l - f >= kParallelPartition) ? partition_pivot_parallel(a, f, l, Lp, Rp) : partition_pivot(a, f, l);
         // [f, cut) holds keys <= pivot and [cut, l) keys >= pivot (the pivot stays at f)
         const double p = a[f].k;
         const size_t left = (v < p) ? 1 : (p < v ? 0 : count_in(f, cut));
         const size_t right = (p < v) ? 1 : (v < p ? 0 : count_in(cut, l));
-        if (right) track_equal(a, cut, l, depth, v, out);  // the right part is sorted first
+        if (right) track_equal(a, cut, l, depth, v, out, Lp, Rp);  // the right part is sorted first
         if (!left) return;
         l = cut;
     }
@@ -554,7 +629,7 @@ int PickTable::resolve_tie(size_t pos, const std::vector<int> &live, const std::
         Workers::get().run((live.size() + blk - 1) / blk, fill);
     }
     std::vector<std::pair<size_t, int>> run;
-    if (!a.empty()) track_equal(a.data(), 0, a.size(), 2 * floor_log2(a.size()), v, run);
+    if (!a.empty()) track_equal(a.data(), 0, a.size(), 2 * floor_log2(a.size()), v, run, tie_l_, tie_r_);
     std::sort(run.begin(), run.end());
     if (pos - p0 >= run.size()) throw std::logic_error("pick table: tie run not found");
     return live[static_cast<size_t>(run[pos - p0].second)];

@@ -77,6 +77,7 @@ class PickTable {
 
   private:
     std::vector<KeyLabel> tie_keys_;  // scratch of resolve_tie()
+    std::vector<uint32_t> tie_l_, tie_r_;  // scratch of its parallel partitions
 
     void materialize(Chunk &c, double A);
     void settle(Chunk &c, double S);

@@ -16,7 +16,7 @@ def test_pick_table_matches_resort(n, events, chunk, seed):
 
 @pytest.mark.parametrize("n,events,chunk,seed,mode", [
     (17, 15, 16, 11, 1), (40, 38, 16, 12, 1), (3000, 600, 64, 13, 1), (20000, 60, 512, 14, 1),
-    (200000, 8, 2048, 15, 1), (5000, 400, 128, 16, 2), (100000, 20, 2048, 17, 2),
+    (200000, 8, 2048, 15, 1), (5000, 400, 128, 16, 2), (100000, 20, 2048, 17, 2), (600000, 3, 2048, 18, 2),
 ])
 def test_pick_table_ties_follow_std_sort(n, events, chunk, seed, mode):
     """Picks inside runs of equal keys follow the reference's std::sort permutation."""
