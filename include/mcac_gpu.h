@@ -53,6 +53,7 @@ extern "C" {
 #define MCAC_E_ARGUMENT 34   /* invalid argument (id out of range, dead id...) */
 
 typedef struct mcac_box mcac_box;
+typedef struct mcac_group mcac_group;
 
 typedef struct mcac_box_params {
     double box_length;          /* periodic box side L [m]                          */
@@ -140,6 +141,29 @@ int mcac_release(mcac_box *box);
  * caller must call mcac_merge for such a contact before enqueueing further
  * moves.  Off by default; set while no move is queued. */
 int mcac_box_set_auto_merge(mcac_box *box, int enable);
+
+/* Groups of boxes: one queued move of each of many boxes in a single launch
+ * (one workgroup per box).  No counterpart in the reference, which runs one box
+ * per process (src/main.cpp); for ensembles of small boxes, whose steps are
+ * bound by launch and memory latency rather than by work.  A box joins a group
+ * with no step in flight and then works on the group's stream (its own stream
+ * is released); it leaves the group only when destroyed, which must happen
+ * before the group is destroyed. */
+int mcac_group_create(int device, int max_boxes, mcac_group **out);
+int mcac_group_destroy(mcac_group *group);
+int mcac_group_add(mcac_group *group, mcac_box *box);
+/* mcac_move_enqueue for n moves of boxes of the group: the moves of one box
+ * are contiguous, in the order they are to run (at most 8 in flight per box),
+ * n <= 8 * max_boxes.  One launch takes every move that fits one workgroup,
+ * one workgroup running each box's moves in order; the others take their
+ * usual launches on the group stream, in order with the rest.  dirs holds 3
+ * doubles per move.  Results are read per box with mcac_move_result, contacts
+ * and skipped moves handled as for mcac_move_enqueue.  On an error the moves
+ * enqueued before the failing one keep their tickets and run. */
+int mcac_move_enqueue_group(mcac_group *group, mcac_box *const *boxes, const int64_t *aggs, const double *dirs,
+                            const double *lpm, const double *maxradius, int n, uint64_t *tickets);
+/* [group launches, moves in group launches, moves launched on their own] */
+int mcac_group_stats(mcac_group *group, int64_t out[3]);
 
 int mcac_box_download_spheres(mcac_box *box, double *x, double *y, double *z, double *rx, double *ry,
                               double *rz);

@@ -99,6 +99,16 @@ int mcac_write_test_snapshots(const char *prefix, int64_t n, int64_t grids, int6
  * job status, else 0. */
 int mcac_batch_run(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers, int device,
                    int64_t max_steps, double *start_s, double *end_s, int64_t *steps, int *status);
+/* The same runs in lock step: each worker drives up to boxes_per_worker boxes
+ * that share one stream (mcac_group); per round it picks up to `depth` (1..8)
+ * steps ahead for every box, as the pipelined loop does, and enqueues them all
+ * in one launch, one workgroup running each box's steps in order.  Same
+ * outputs and job order as mcac_batch_run, plus each job's final physical time
+ * (final_time may be NULL); each box's steps and results are those of a run on
+ * its own. */
+int mcac_batch_run_groups(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers,
+                          int boxes_per_worker, int depth, int device, int64_t max_steps, double *start_s,
+                          double *end_s, int64_t *steps, int *status, double *final_time);
 
 #ifdef __cplusplus
 }

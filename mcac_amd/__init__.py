@@ -54,14 +54,15 @@ GPU_SYMBOLS = (
     "mcac_move", "mcac_merge", "mcac_move_enqueue", "mcac_move_result", "mcac_release", "mcac_box_set_auto_merge",
     "mcac_box_download_spheres", "mcac_box_sphere_owners", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
-    "mcac_version",
+    "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
+    "mcac_group_stats",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
     "mcac_sim_set_trace", "mcac_sim_n_steps", "mcac_sim_get_steps", "mcac_sim_n_merges",
     "mcac_sim_get_merges", "mcac_sim_scalar", "mcac_sim_n_aggs", "mcac_sim_export_aggs", "mcac_sim_box",
     "mcac_physics_scalars", "mcac_pick_table_selftest", "mcac_pick_table_selftest_ties", "mcac_batch_run",
-    "mcac_write_test_snapshots",
+    "mcac_write_test_snapshots", "mcac_batch_run_groups",
 )
 
 _P = ctypes.c_void_p
@@ -117,6 +118,12 @@ def load_gpu():
         L.mcac_box_stats.argtypes = [_P, _I64P]
         L.mcac_box_prune_stats.argtypes = [_P, _I64P]
         L.mcac_swept_sphere_batch.argtypes = [_I64, _DP, _DP, _DP, _DP, _DP, _DP, _D, _DP, _I]
+        L.mcac_group_create.argtypes = [_I, _I, ctypes.POINTER(_P)]
+        L.mcac_group_destroy.argtypes = [_P]
+        L.mcac_group_add.argtypes = [_P, _P]
+        L.mcac_move_enqueue_group.argtypes = [_P, ctypes.POINTER(_P), _I64P, _DP, _DP, _DP, _I,
+                                              ctypes.POINTER(ctypes.c_uint64)]
+        L.mcac_group_stats.argtypes = [_P, _I64P]
         L.mcac_version.restype = ctypes.c_char_p
         _gpu = L
     return _gpu
@@ -155,6 +162,8 @@ def load_sim():
         L.mcac_write_test_snapshots.argtypes = [ctypes.c_char_p, _I64, _I64, _I64]
         L.mcac_batch_run.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I64, _DP, _DP, _I64P,
                                      ctypes.POINTER(_I)]
+        L.mcac_batch_run_groups.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I, _I, _I64, _DP,
+                                            _DP, _I64P, ctypes.POINTER(_I), _DP]
         _sim = L
     return _sim
 

@@ -10,7 +10,9 @@
 #include <thread>
 #include <vector>
 
+#include "../../../include/mcac_gpu.h"
 #include "../../../include/mcac_sim.h"
+#include "sim_group.hpp"
 
 extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers,
                               int device, int64_t max_steps, double *start_s, double *end_s, int64_t *steps,
@@ -45,6 +47,150 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
     std::vector<std::thread> pool;
     for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
     for (auto &t : pool) t.join();
+    for (int64_t j = 0; j < n_jobs; ++j)
+        if (status[j]) return status[j];
+    return 0;
+}
+
+// Lock-step form: worker w keeps up to G boxes in a group (one shared stream)
+// and runs rounds: pick up to `depth` steps ahead for every box, enqueue them
+// together (one launch; one block runs a box's steps in order, those behind a
+// contact do nothing), read every box's results.  A box whose job ends leaves
+// the round and the worker admits the next job that has arrived.
+extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs,
+                                     int workers, int boxes_per_worker, int depth, int device, int64_t max_steps,
+                                     double *start_s, double *end_s, int64_t *steps, int *status, double *final_time) {
+    if (!ini_texts || n_jobs < 0 || workers < 1 || boxes_per_worker < 1 || boxes_per_worker > 4096 || depth < 1 ||
+        depth > 8 || !start_s || !end_s || !steps || !status)
+        return MCAC_E_ARGUMENT;
+    using Clock = std::chrono::steady_clock;
+    const auto t0 = Clock::now();
+    auto now_s = [t0] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+    std::atomic<int64_t> next{0};
+    std::atomic<int> worker_rc{0};
+    auto worker = [&] {
+        const int G = boxes_per_worker;
+        mcac_group *group = nullptr;
+        int grc = mcac_group_create(device, G, &group);
+        if (grc) {
+            worker_rc.store(grc);
+            return;
+        }
+        struct Active {
+            mcac_sim *sim;
+            int64_t job;
+        };
+        std::vector<Active> act;
+        auto end_job = [&](size_t k, int rc) {
+            const int64_t j = act[k].job;
+            if (rc == 0) rc = mcac::sim_finalize(act[k].sim);
+            if (final_time) final_time[j] = mcac_sim_scalar(act[k].sim, "time");
+            mcac_sim_destroy(act[k].sim);
+            end_s[j] = now_s();
+            status[j] = rc;
+            if (rc) std::fprintf(stderr, "mcac_batch_run_groups: job %lld status %d\n", static_cast<long long>(j), rc);
+            act[k] = act.back();
+            act.pop_back();
+        };
+        int64_t pending = -1;
+        bool exhausted = false;
+        std::vector<mcac_box *> boxes;
+        std::vector<int64_t> aggs;
+        std::vector<double> dirs, lpms, maxrs;
+        std::vector<uint64_t> tickets;
+        std::vector<size_t> who;
+        while (true) {
+            // admit the jobs that have arrived while there is room
+            while (static_cast<int>(act.size()) < G) {
+                if (pending < 0 && !exhausted) {
+                    pending = next.fetch_add(1);
+                    if (pending >= n_jobs) {
+                        pending = -1;
+                        exhausted = true;
+                    }
+                }
+                if (pending < 0) break;
+                const double at = arrival_s ? arrival_s[pending] : 0.0;
+                if (now_s() < at) {
+                    if (!act.empty()) break;
+                    std::this_thread::sleep_for(std::chrono::microseconds(200));
+                    continue;
+                }
+                const int64_t j = pending;
+                pending = -1;
+                start_s[j] = now_s();
+                steps[j] = 0;
+                char err[256] = {0};
+                mcac_sim *sim = nullptr;
+                int rc = mcac_sim_create(ini_texts[j], nullptr, device, &sim, err, static_cast<int>(sizeof err));
+                if (rc == 0) rc = mcac::sim_join_group(sim, group, depth);
+                if (rc == 0) rc = mcac_sim_init(sim);
+                if (rc) {
+                    if (sim) mcac_sim_destroy(sim);
+                    if (final_time) final_time[j] = 0.0;
+                    end_s[j] = now_s();
+                    status[j] = rc;
+                    std::fprintf(stderr, "mcac_batch_run_groups: job %lld status %d %s\n", static_cast<long long>(j),
+                                 rc, err);
+                    continue;
+                }
+                act.push_back({sim, j});
+            }
+            if (act.empty()) {
+                if (exhausted && pending < 0) break;
+                continue;
+            }
+            // one step of every box that has one due, in one launch
+            boxes.clear();
+            aggs.clear();
+            dirs.clear();
+            lpms.clear();
+            maxrs.clear();
+            who.clear();
+            for (size_t k = 0; k < act.size(); ++k) {
+                mcac::GroupMove m{};
+                while (mcac::sim_next_move(act[k].sim, steps[act[k].job], max_steps, &m)) {
+                    boxes.push_back(mcac::sim_box(act[k].sim));
+                    aggs.push_back(m.agg);
+                    dirs.insert(dirs.end(), m.dir, m.dir + 3);
+                    lpms.push_back(m.lpm);
+                    maxrs.push_back(m.maxradius);
+                    who.push_back(k);
+                }
+            }
+            tickets.assign(boxes.size(), 0);
+            int rc = boxes.empty() ? 0
+                                   : mcac_move_enqueue_group(group, boxes.data(), aggs.data(), dirs.data(), lpms.data(),
+                                                             maxrs.data(), static_cast<int>(boxes.size()),
+                                                             tickets.data());
+            if (rc) {
+                // not expected; every box of the round ends with the status
+                // (destroying a box waits for the group stream)
+                for (size_t k = act.size(); k-- > 0;) end_job(k, rc);
+                continue;
+            }
+            std::vector<int> step_rc(act.size(), 0);
+            for (size_t q = 0; q < who.size(); ++q) mcac::sim_issued(act[who[q]].sim, tickets[q]);
+            for (size_t k = 0; k < act.size(); ++k) {
+                // every step of the box in this launch: run, or skipped behind its contact
+                while (step_rc[k] == 0 && mcac::sim_in_flight(act[k].sim) > 0) {
+                    bool stepped = false;
+                    step_rc[k] = mcac::sim_consume(act[k].sim, &stepped);
+                    if (stepped) steps[act[k].job] += 1;
+                }
+            }
+            // jobs that ended: finished, step budget spent, or an error
+            for (size_t k = act.size(); k-- > 0;) {
+                const bool over = steps[act[k].job] >= max_steps || mcac_sim_finished(act[k].sim);
+                if (step_rc[k] || over) end_job(k, step_rc[k]);
+            }
+        }
+        mcac_group_destroy(group);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    if (worker_rc.load()) return worker_rc.load();
     for (int64_t j = 0; j < n_jobs; ++j)
         if (status[j]) return status[j];
     return 0;

@@ -22,6 +22,7 @@
 #include "../../../include/mcac_sim.h"
 #include "physics.hpp"
 #include "pick_table.hpp"
+#include "sim_group.hpp"
 #include "xdmf_writer.hpp"
 
 namespace mcac {
@@ -114,6 +115,7 @@ struct Simulation {
     int device = 0;
     int precision = 0;  // box contact-search precision (mcac_box_params)
     mcac_box *box = nullptr;
+    mcac_group *group = nullptr;  // lock-step form: the box works on the group's stream
     // spheres (index order) — host copy of the initial state only
     size_t N = 0;
     // aggregates by stable id
@@ -301,6 +303,11 @@ struct Simulation {
         bp.precision = precision;
         rc = mcac_box_create(&bp, &box);
         if (rc) return rc;
+        if (group) {
+            rc = mcac_group_add(group, box);
+            if (rc) return rc;
+            depth = group_depth;  // steps picked ahead per box and group launch
+        }
         std::vector<double> pos3(3 * N), rcm3(3 * N, 0.0);
         std::vector<int64_t> cnt(N, 1), mem(N);
         for (size_t i = 0; i < N; ++i) {
@@ -481,6 +488,8 @@ struct Simulation {
     };
     std::deque<Draw> draws;      // taken from the stream, not yet consumed by a step
     std::deque<Pending> flight;  // enqueued on the device, in order
+    size_t unticketed = 0;       // newest entries of `flight` picked for a group launch, not enqueued yet
+    size_t group_depth = 1;      // steps per box and group launch
     size_t depth = 3;
 
     // the k-th draw after the last consumed step (calcul.cpp:102, tools.cpp:83-84)
@@ -504,7 +513,8 @@ struct Simulation {
         return true;
     }
 
-    int issue() {
+    // The mover and move of the next step to issue (calcul.cpp:99-127).
+    Pending pick_move() {
         auto t0 = Clock::now();
         const Draw &d = draw_at(flight.size());
         size_t tie_pos = 0;
@@ -518,13 +528,28 @@ struct Simulation {
         }
         const double full = lpm[static_cast<size_t>(num)];
         t_pick += ms_since(t0);
-        uint64_t ticket = 0;
+        return {0, num, tie, full, d};
+    }
+
+    int issue() {
+        Pending P = pick_move();
         auto te0 = Clock::now();
-        int rc = mcac_move_enqueue(box, num, d.dir.data(), full, maxradius, &ticket);
+        int rc = mcac_move_enqueue(box, P.num, P.draw.dir.data(), P.full, maxradius, &P.ticket);
         t_enqueue += ms_since(te0);
         if (rc) return rc;
-        flight.push_back({ticket, num, tie, full, d});
+        flight.push_back(P);
         return 0;
+    }
+
+    // Outputs at the end of a run (calcul.cpp:282-284), once.
+    int finalize() {
+        if (!finished() || final_written) return 0;
+        save_advancement();
+        int rc = save_snapshot();
+        if (out_spheres) out_spheres->close();
+        if (out_aggs) out_aggs->close();
+        final_written = true;
+        return rc;
     }
 
     // Device state after a contact: the steps behind it were skipped; forget them
@@ -715,6 +740,51 @@ struct mcac_sim {
     mcac::Simulation s;
 };
 
+namespace mcac {
+
+int sim_join_group(mcac_sim *sim, mcac_group *group, int depth) {
+    if (!sim || !group || sim->s.initialized || depth < 1 || depth > 8) return MCAC_E_ARGUMENT;
+    sim->s.group = group;
+    sim->s.group_depth = static_cast<size_t>(depth);
+    return 0;
+}
+
+bool sim_next_move(mcac_sim *sim, int64_t steps_done, int64_t max_steps, GroupMove *move) {
+    Simulation &S = sim->s;
+    if (!S.initialized || S.finished() || !S.can_issue(steps_done, max_steps)) return false;
+    Simulation::Pending P = S.pick_move();  // the draw after the ones already in flight
+    S.flight.push_back(P);
+    S.unticketed += 1;
+    move->agg = P.num;
+    for (int l = 0; l < 3; ++l) move->dir[l] = P.draw.dir[static_cast<size_t>(l)];
+    move->lpm = P.full;
+    move->maxradius = S.maxradius;
+    return true;
+}
+
+mcac_box *sim_box(mcac_sim *sim) { return sim->s.box; }
+
+void sim_issued(mcac_sim *sim, uint64_t ticket) {
+    Simulation &S = sim->s;
+    S.flight[S.flight.size() - S.unticketed].ticket = ticket;
+    S.unticketed -= 1;
+}
+
+size_t sim_in_flight(mcac_sim *sim) { return sim->s.flight.size(); }
+
+int sim_consume(mcac_sim *sim, bool *stepped) {
+    try {
+        return sim->s.consume(stepped);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "mcac_sim: %s\n", e.what());
+        return MCAC_UNKNOWN_ERROR;
+    }
+}
+
+int sim_finalize(mcac_sim *sim) { return sim->s.finalize(); }
+
+}  // namespace mcac
+
 extern "C" {
 
 int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mcac_sim **out, char *err, int errlen) {
@@ -764,13 +834,7 @@ int mcac_sim_run(mcac_sim *sim, int64_t max_steps, int64_t *done) {
     int rc = 0;
     try {
         rc = sim->s.run(max_steps, &n);
-        if (rc == 0 && sim->s.finished() && !sim->s.final_written) {
-            sim->s.save_advancement();  // calcul.cpp:282-284
-            rc = sim->s.save_snapshot();
-            if (sim->s.out_spheres) sim->s.out_spheres->close();
-            if (sim->s.out_aggs) sim->s.out_aggs->close();
-            sim->s.final_written = true;
-        }
+        if (rc == 0) rc = sim->s.finalize();
     } catch (const std::exception &e) {
         std::fprintf(stderr, "mcac_sim_run: %s\n", e.what());
         rc = MCAC_UNKNOWN_ERROR;

@@ -66,10 +66,11 @@ __device__ __forceinline__ bool behind_gate(const DevBox &b, const SearchArgs &a
     return g != 0ull && g < a.seq && g > a.released;
 }
 
-// Filter work of one block: the aggregates of the reference's visited Verlet
-// cells and the dirty list, bounding sweep, candidate append (shared by
-// k_filter and k_step).
-__device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, const SearchArgs &a) {
+// Filter work of block vb of the vg blocks on this step: the aggregates of the
+// reference's visited Verlet cells and the dirty list, bounding sweep,
+// candidate append (shared by k_filter, k_step and k_step_batch).
+__device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, const SearchArgs &a, unsigned vb,
+                                                           unsigned vg) {
     __shared__ int win[6];
     __shared__ double q[4];
     __shared__ int pre[kMaxVisit + 1];
@@ -119,8 +120,8 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
         sr = q[3];
         V = nvisit;
     }
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gstride = gridDim.x * blockDim.x;
+    const int gtid = static_cast<int>(vb * blockDim.x + threadIdx.x);
+    const int gstride = static_cast<int>(vg * blockDim.x);
     unsigned long long cnt = 0;
     int total;  // entries of the visited cells (or of the whole list for a very wide window)
     if (V <= 64) {
@@ -253,7 +254,7 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
 
 __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, SearchArgs a) {
     if (behind_gate(b, a)) return;
-    const unsigned long long tot = block_sum(filter_phase(b, a)) & 0xffffffffull;
+    const unsigned long long tot = block_sum(filter_phase(b, a, blockIdx.x, gridDim.x)) & 0xffffffffull;
     if (threadIdx.x == 0 && tot) atomicAdd(&b.ss->nquery, tot);
 }
 
@@ -591,31 +592,37 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     finish_step(b, a, F, nsrc, ssrc, t, MoverSphere{0, -1, 0., 0., 0.});
 }
 
-// One kernel per MC step for small movers: every block filters its share of
-// the Verlet window; the last block to finish (ticket) solves the sphere pairs
-// of the survivors itself, one candidate at a time, and finishes the step.
-// When the survivors hold more pairs than one block should take, it leaves
-// them in place, closes the gate for the steps queued behind and publishes a
-// pending record: the host then launches k_solve for the same step.
-__global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
+// One MC step for a small mover, worked on by vg blocks (vb is this one):
+// every block filters its share of the Verlet window; the last block to finish
+// (ticket) solves the sphere pairs of the survivors itself, one candidate at a
+// time, and finishes the step.  When the survivors hold more pairs than one
+// block should take, it leaves them in place, closes the gate for the steps
+// queued behind and publishes a pending record: the host then launches k_solve
+// for the same step.  With vg == 1 the block is the last one by construction.
+__device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, unsigned vb, unsigned vg) {
     if (behind_gate(b, a)) return;  // the host knows it is skipped: no record
     // loads the finishing block will need, issued before the filter work
     const int src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
     const MoverSphere mover = load_mover_sphere(b, a);
-    const unsigned long long cnt = block_sum(filter_phase(b, a));
+    const unsigned long long cnt = block_sum(filter_phase(b, a, vb, vg));
     // one atomic per block: finished blocks, appended candidates, visited aggregates
     __shared__ bool last;
     __shared__ unsigned long long s_total;
-    if (threadIdx.x == 0) {
-        const unsigned long long mine = kTicketOne + cnt;
+    if (vg > 1) {
+        if (threadIdx.x == 0) {
+            const unsigned long long mine = kTicketOne + cnt;
+            __threadfence();
+            const unsigned long long old = atomicAdd(&b.ss->tick, mine);
+            last = (old >> 56) == vg - 1u;
+            s_total = old + mine;
+        }
+        __syncthreads();
+        if (!last) return;
         __threadfence();
-        const unsigned long long old = atomicAdd(&b.ss->tick, mine);
-        last = (old >> 56) == gridDim.x - 1u;
-        s_total = old + mine;
+    } else {
+        if (threadIdx.x == 0) s_total = cnt;
+        __syncthreads();  // the block's appended candidates are visible to all its threads
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
     SearchState *s = b.ss;
     const unsigned long long total = s_total;
     StepTotals t;
@@ -700,6 +707,35 @@ __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
         }
     }
     finish_step(b, a, F, nsrc, ssrc, t, mover);
+}
+
+__global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) { step_body(b, a, blockIdx.x, gridDim.x); }
+
+// Steps of many boxes in one launch: block i runs jobs seg[i] .. seg[i+1]-1,
+// consecutive queued steps of one box, in order (jobs in pinned host memory).
+// Small boxes are bound by launch and memory latency, so their steps share
+// launches instead of each taking one; a step behind a contact of an earlier
+// job of the same block is skipped by the gate as in separate launches.
+struct StepJob {
+    DevBox b;
+    SearchArgs a;
+};
+static_assert(sizeof(StepJob) % sizeof(unsigned long long) == 0, "job copied in 8-byte words");
+
+__global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, const int *seg) {
+    __shared__ StepJob job;
+    constexpr int kWords = static_cast<int>(sizeof(StepJob) / sizeof(unsigned long long));
+    const int j0 = seg[blockIdx.x], j1 = seg[blockIdx.x + 1];
+    for (int j = j0; j < j1; ++j) {
+        // the previous step's writes (positions, lists, gate) and its use of
+        // the shared state are complete before the next step starts
+        __syncthreads();
+        const unsigned long long *src = reinterpret_cast<const unsigned long long *>(jobs + j);
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(&job);
+        for (int i = threadIdx.x; i < kWords; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+        step_body(job.b, job.a, 0u, 1u);
+    }
 }
 
 // a13 for a large mover: sphere positions from the refpos of the last search.

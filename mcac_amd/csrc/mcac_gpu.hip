@@ -32,6 +32,9 @@ using namespace mcac_dev;
 struct mcac_box {
     mcac_box_params p{};
     hipStream_t stream = nullptr;
+    bool own_stream = true;          // false once the box works on a group's stream
+    mcac_group *group = nullptr;
+    int64_t group_call = -1;         // last group enqueue that took a move of this box
     DevBox d{};
     int64_t n_spheres = 0, n_aggs = 0;
     int64_t cand_cap = 0, part_cap = 0, pool_cap = 0;
@@ -87,6 +90,24 @@ struct mcac_box {
     double t_search = 0, t_translate = 0, t_merge = 0, t_filter = 0, t_solve = 0;
     double n_search = 0, n_translate = 0, n_merge = 0;
     int64_t n_rebuilds = 0;
+};
+
+// Boxes that share one stream; their queued steps go into k_step_batch
+// launches, one block per box.  Jobs and block segments are written to pinned
+// memory, alternating between two halves.
+struct mcac_group {
+    static constexpr int kMovesPerBox = 8;  // = mcac_box::kRing, steps a box can have in flight
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cap = 0;                           // boxes
+    int job_cap = 0;                       // jobs per half
+    int parity = 0;
+    StepJob *h_jobs = nullptr, *d_jobs = nullptr;  // 2 * job_cap jobs
+    int *h_seg = nullptr, *d_seg = nullptr;        // 2 * (job_cap + 1) segment starts
+    hipEvent_t read[2]{};                           // recorded after the launch reading each half
+    bool pending[2] = {false, false};
+    int64_t calls = 0;
+    int64_t launches = 0, batched_steps = 0, single_steps = 0;
 };
 
 namespace {
@@ -227,12 +248,31 @@ int compact_pool(mcac_box *b, int64_t need) {
 
 int harvest_timing(mcac_box *b, int slot);
 
+// A step that one block can take whole: the one-launch form (small mover, few
+// sphere pairs possible), the mover moved by the finishing block, not timed.
+bool one_block_step(const mcac_box *b, int n_mov, bool timed) {
+    const bool fused = n_mov <= kStepMover && static_cast<long long>(n_mov) * b->max_count <= kStepPairs;
+    return fused && n_mov <= 4 * kBlock && !timed;
+}
+
+// The next enqueued step rebuilds the cell list first (kernels on the stream).
+bool rebuild_due(const mcac_box *b) { return b->dirty_count + 3 >= rebuild_threshold(b); }
+
+bool next_step_timed(const mcac_box *b) {
+    return b->timing > 0 && (b->seq + 1) % static_cast<unsigned long long>(b->timing) == 0;
+}
+
+// Picks the launch of one step and enqueues it.  With job_out, a step that
+// one block can take whole (one-launch form, mover moved by the finishing
+// block, not timed) is written there instead, for a k_step_batch launch by the
+// caller (*batched = true).
 int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double maxradius, int apply, int gated,
-                   unsigned long long *ticket) {
+                   unsigned long long *ticket, StepJob *job_out = nullptr, bool *batched = nullptr) {
+    if (batched) *batched = false;
     if (agg < 0 || agg >= b->n_aggs || b->count[static_cast<size_t>(agg)] <= 0) return MCAC_E_ARGUMENT;
     if (b->inflight >= mcac_box::kRing) return MCAC_E_CAPACITY;
     if (b->inline_pending) return MCAC_E_ARGUMENT;  // the caller merges the contact first
-    if (b->dirty_count + 3 >= rebuild_threshold(b)) {
+    if (rebuild_due(b)) {
         int rc = rebuild_cells(b);
         if (rc) return rc;
     }
@@ -320,7 +360,11 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     // kernel's own execution interval; the queue is drained first so that the
     // interval does not include waiting behind the steps queued before it
     if (timed && b->inflight > 0) HIP_TRY(hipStreamSynchronize(b->stream));
-    if (fused) {
+    if (job_out && one_block_step(b, n_mov, timed)) {
+        job_out->b = b->d;
+        job_out->a = a;
+        *batched = true;
+    } else if (fused) {
         if (timed) hipExtLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, b->stream, ev[0], ev[1], 0, b->d, a);
         else hipLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, b->stream, b->d, a);
     } else if (timed) {
@@ -579,7 +623,7 @@ int mcac_box_destroy(mcac_box *b) {
     for (auto &row : b->sev)
         for (auto &e : row)
             if (e) (void)hipEventDestroy(e);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream && b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return 0;
 }
@@ -869,6 +913,150 @@ int mcac_move_enqueue(mcac_box *b, int64_t agg, const double dir[3], double lpm,
     int rc = enqueue_search(b, agg, dir, lpm, maxradius, 1, 1, &t);
     *ticket = t;
     return rc;
+}
+
+int mcac_group_create(int device, int max_boxes, mcac_group **out) {
+    if (!out || max_boxes < 1 || max_boxes > 65535) return MCAC_E_ARGUMENT;
+    auto *g = new (std::nothrow) mcac_group();
+    if (!g) return MCAC_E_CAPACITY;
+    g->device = device;
+    g->cap = max_boxes;
+    g->job_cap = max_boxes * mcac_group::kMovesPerBox;
+    const size_t bytes = 2 * static_cast<size_t>(g->job_cap) * sizeof(StepJob);
+    const size_t seg_bytes = 2 * (static_cast<size_t>(g->job_cap) + 1) * sizeof(int);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&g->h_jobs), bytes, hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&g->d_jobs), g->h_jobs, 0) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&g->h_seg), seg_bytes, hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&g->d_seg), g->h_seg, 0) != hipSuccess ||
+        hipEventCreateWithFlags(&g->read[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->read[1], hipEventDisableTiming) != hipSuccess) {
+        mcac_group_destroy(g);
+        return MCAC_E_DEVICE;
+    }
+    std::memset(g->h_jobs, 0, bytes);
+    std::memset(g->h_seg, 0, seg_bytes);
+    *out = g;
+    return 0;
+}
+
+int mcac_group_destroy(mcac_group *g) {
+    if (!g) return 0;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->h_jobs) (void)hipHostFree(g->h_jobs);
+    if (g->h_seg) (void)hipHostFree(g->h_seg);
+    for (auto &e : g->read)
+        if (e) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return 0;
+}
+
+int mcac_group_add(mcac_group *g, mcac_box *b) {
+    if (!g || !b || b->group || b->inflight || b->p.device != g->device) return MCAC_E_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->own_stream) HIP_TRY(hipStreamDestroy(b->stream));
+    b->stream = g->stream;
+    b->own_stream = false;
+    b->group = g;
+    return 0;
+}
+
+namespace {
+
+// Job list of the current half: jobs[0..n), block segments seg[0..blocks].
+struct JobList {
+    StepJob *jobs;
+    int *seg;
+    int n = 0, blocks = 0;
+    const mcac_box *last = nullptr;
+};
+
+int group_open(mcac_group *g, JobList &l) {
+    const int half = g->parity;
+    if (g->pending[half]) {  // the launch that read this half has copied its jobs
+        HIP_TRY(hipEventSynchronize(g->read[half]));
+        g->pending[half] = false;
+    }
+    l.jobs = g->h_jobs + static_cast<size_t>(half) * static_cast<size_t>(g->job_cap);
+    l.seg = g->h_seg + static_cast<size_t>(half) * (static_cast<size_t>(g->job_cap) + 1);
+    l.n = l.blocks = 0;
+    l.last = nullptr;
+    return 0;
+}
+
+int group_flush(mcac_group *g, JobList &l) {
+    if (l.n == 0) return 0;
+    const int half = g->parity;
+    l.seg[l.blocks] = l.n;
+    hipLaunchKernelGGL(k_step_batch, dim3(static_cast<unsigned>(l.blocks)), dim3(kBlock), 0, g->stream,
+                       g->d_jobs + static_cast<size_t>(half) * static_cast<size_t>(g->job_cap),
+                       g->d_seg + static_cast<size_t>(half) * (static_cast<size_t>(g->job_cap) + 1));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g->read[half], g->stream));
+    g->pending[half] = true;
+    g->parity ^= 1;
+    g->launches += 1;
+    g->batched_steps += l.n;
+    return group_open(g, l);
+}
+
+}  // namespace
+
+int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t *aggs, const double *dirs,
+                            const double *lpm, const double *maxradius, int n, uint64_t *tickets) {
+    if (!g || n < 0 || n > g->job_cap || (n > 0 && (!boxes || !aggs || !dirs || !lpm || !maxradius || !tickets)))
+        return MCAC_E_ARGUMENT;
+    // the moves of one box are contiguous and in the order they are to run
+    const int64_t call = ++g->calls;
+    for (int i = 0; i < n; ++i) {
+        mcac_box *b = boxes[i];
+        if (!b || b->group != g) return MCAC_E_ARGUMENT;
+        if (i > 0 && boxes[i - 1] == b) continue;
+        if (b->group_call == call) return MCAC_E_ARGUMENT;
+        b->group_call = call;
+    }
+    JobList l;
+    int rc = group_open(g, l);
+    if (rc) return rc;
+    for (int i = 0; i < n; ++i) {
+        mcac_box *b = boxes[i];
+        const int64_t agg = aggs[i];
+        const bool valid = agg >= 0 && agg < b->n_aggs;
+        const bool batch = valid && one_block_step(b, b->count[static_cast<size_t>(agg)], next_step_timed(b));
+        // a step launched on its own, or the cell-list rebuild that the
+        // enqueue launches, runs after every job already listed
+        if ((!batch || (rebuild_due(b) && l.last == b)) && (rc = group_flush(g, l)) != 0) break;
+        unsigned long long t = 0;
+        bool batched = false;
+        rc = enqueue_search(b, agg, dirs + 3 * i, lpm[i], maxradius[i], 1, 1, &t, batch ? l.jobs + l.n : nullptr,
+                            batch ? &batched : nullptr);
+        if (rc) break;
+        if (batched != batch) {
+            rc = MCAC_UNKNOWN_ERROR;  // the two decisions must agree
+            break;
+        }
+        tickets[i] = t;
+        if (batched) {
+            if (l.last != b) l.seg[l.blocks++] = l.n;  // a new block: the box's first job of this launch
+            l.last = b;
+            l.n += 1;
+        } else {
+            g->single_steps += 1;
+            l.last = nullptr;
+        }
+    }
+    // the steps enqueued before a failing one still run: their results are read as usual
+    const int frc = group_flush(g, l);
+    return rc ? rc : frc;
+}
+
+int mcac_group_stats(mcac_group *g, int64_t out[3]) {
+    if (!g || !out) return MCAC_E_ARGUMENT;
+    out[0] = g->launches;
+    out[1] = g->batched_steps;
+    out[2] = g->single_steps;
+    return 0;
 }
 
 int mcac_move_result(mcac_box *b, uint64_t ticket, mcac_contact *out, int *skipped) {

@@ -72,15 +72,29 @@ def serving_requests(n_requests: int, rate_per_s: float, s0: int = 5000, rng_see
     return out
 
 
-def batch_run(inis: list[str], arrivals=None, workers: int = 8, device: int = 0, max_steps: int = 2 ** 62) -> dict:
-    """Runs the jobs on `workers` host threads sharing the GPU (include/mcac_sim.h mcac_batch_run)."""
+def batch_run(inis: list[str], arrivals=None, workers: int = 8, device: int = 0, max_steps: int = 2 ** 62,
+              boxes_per_worker: int = 0, depth: int = 6) -> dict:
+    """Runs the jobs on `workers` host threads sharing the GPU (include/mcac_sim.h):
+    one box per worker at a time (mcac_batch_run), or with boxes_per_worker >= 1
+    up to that many boxes per worker stepped in lock step, up to `depth` steps
+    of each box per launch (mcac_batch_run_groups)."""
     n = len(inis)
     texts = (ctypes.c_char_p * n)(*[s.encode() for s in inis])
     arr = None if arrivals is None else np.ascontiguousarray(arrivals, np.float64)
     start, end = np.zeros(n), np.zeros(n)
     steps = np.zeros(n, np.int64)
     status = np.zeros(n, np.int32)
-    rc = load_sim().mcac_batch_run(texts, None if arr is None else _dp(arr), n, workers, device, max_steps,
-                                   _dp(start), _dp(end), _ip(steps), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
-    _check(rc, "mcac_batch_run")
-    return dict(start=start, end=end, steps=steps, status=status)
+    st = status.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    L = load_sim()
+    out = dict(start=start, end=end, steps=steps, status=status)
+    if boxes_per_worker >= 1:
+        final_time = np.zeros(n)
+        rc = L.mcac_batch_run_groups(texts, None if arr is None else _dp(arr), n, workers, boxes_per_worker, depth,
+                                     device, max_steps, _dp(start), _dp(end), _ip(steps), st, _dp(final_time))
+        _check(rc, "mcac_batch_run_groups")
+        out["final_time"] = final_time
+    else:
+        rc = L.mcac_batch_run(texts, None if arr is None else _dp(arr), n, workers, device, max_steps, _dp(start),
+                              _dp(end), _ip(steps), st)
+        _check(rc, "mcac_batch_run")
+    return out
