@@ -74,11 +74,13 @@ def run_ensemble(args) -> dict:
     boxes = ensemble.ensemble_configs(args.ensemble_boxes)
     print(f"bench: ensemble of {len(boxes)} boxes, {args.workers} workers", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    r = ensemble.batch_run([b["ini"] for b in boxes], workers=args.workers, max_steps=args.max_job_steps)
+    r = ensemble.batch_run([b["ini"] for b in boxes], workers=args.workers, max_steps=args.max_job_steps,
+                           boxes_per_worker=args.group_boxes, depth=args.group_depth)
     wall = time.perf_counter() - t0
     done = [j for j in range(len(boxes)) if r["steps"][j] < args.max_job_steps]
     dur = (r["end"] - r["start"])[done]
-    return {"boxes": len(boxes), "of": 512, "completed": len(done), "workers": args.workers, "wall_s": wall,
+    return {"boxes": len(boxes), "of": 512, "completed": len(done), "workers": args.workers,
+            "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
             "runs_per_hour": 3600.0 * len(done) / wall, "mc_steps_per_s": float(r["steps"].sum()) / wall,
             "box_time_p50_s": _pct(dur, 50), "box_time_p99_s": _pct(dur, 99),
             "sample": f"boxes 0..{len(boxes) - 1} of the 512-box list (N log-uniform 500..5e4, normal 30/3 nm, "
@@ -99,7 +101,8 @@ def run_ensemble_compare(args) -> dict:
     print(f"bench: ensemble comparison on {len(boxes)} boxes (N <= {args.compare_nmax:g})", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()
-    r = ensemble.batch_run(inis, workers=args.workers, max_steps=args.max_job_steps)
+    r = ensemble.batch_run(inis, workers=args.workers, max_steps=args.max_job_steps, boxes_per_worker=args.group_boxes,
+                           depth=args.group_depth)
     gpu_wall = time.perf_counter() - t0
 
     def one(ini):  # ctypes drops the GIL: the oracle runs proceed in parallel
@@ -114,7 +117,7 @@ def run_ensemble_compare(args) -> dict:
     cpu_wall = time.perf_counter() - t0
     same = all(int(c[0]) == int(g) for c, g in zip(cpu, r["steps"]))
     return {"boxes": len(boxes), "gpu_runs_per_hour": 3600.0 * len(boxes) / gpu_wall, "gpu_wall_s": gpu_wall,
-            "gpu_workers": args.workers, "cpu_runs_per_hour": 3600.0 * len(boxes) / cpu_wall, "cpu_wall_s": cpu_wall,
+            "gpu_workers": args.workers, "gpu_boxes_per_worker": args.group_boxes, "cpu_runs_per_hour": 3600.0 * len(boxes) / cpu_wall, "cpu_wall_s": cpu_wall,
             "cpu_threads": procs, "ratio": cpu_wall / gpu_wall, "same_step_counts": same,
             "sample": f"boxes with N <= {args.compare_nmax:g} among boxes 0..{args.ensemble_boxes - 1} of the "
                       f"512-box list, run to N_agg <= N/10 on both sides; CPU = the oracle, one thread per run"}
@@ -128,11 +131,12 @@ def run_serving(args) -> dict:
     print(f"bench: serving {len(reqs)} requests at {args.serving_rate}/s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     r = ensemble.batch_run([q["ini"] for q in reqs], arrivals=[q["arrival"] for q in reqs], workers=args.workers,
-                           max_steps=args.max_job_steps)
+                           max_steps=args.max_job_steps, boxes_per_worker=args.group_boxes, depth=args.group_depth)
     wall = time.perf_counter() - t0
     arrival = np.array([q["arrival"] for q in reqs])
     ttr = r["end"] - arrival
-    return {"requests": len(reqs), "rate_per_s": args.serving_rate, "workers": args.workers, "wall_s": wall,
+    return {"requests": len(reqs), "rate_per_s": args.serving_rate, "workers": args.workers,
+            "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
             "time_to_result_p50_s": _pct(ttr, 50), "time_to_result_p99_s": _pct(ttr, 99),
             "runs_per_hour": 3600.0 * len(reqs) / wall,
             "sample": f"N log-uniform 1e3..{args.serving_nmax:g} (full form 2e5), lognormal 30/1.25, FV U[1e-4,5e-3], "
@@ -156,7 +160,10 @@ def main() -> int:
     ap.add_argument("--serving-rate", type=float, default=4.0, help="serving arrival rate, requests/s")
     ap.add_argument("--serving-nmax", type=float, default=2e4,
                     help="largest request size in the serving sample (the full form allows 2e5)")
-    ap.add_argument("--workers", type=int, default=8, help="host threads sharing the GPU (ensemble, serving)")
+    ap.add_argument("--workers", type=int, default=4, help="host threads sharing the GPU (ensemble, serving)")
+    ap.add_argument("--group-boxes", type=int, default=4,
+                    help="boxes per worker stepped in lock step, one launch per round (0: one box per worker)")
+    ap.add_argument("--group-depth", type=int, default=6, help="steps of each box per grouped launch (1..8)")
     ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")
     ap.add_argument("--c3-windows", type=float, default=0.0,
                     help="seconds: run the N=1e6 box on and report MC steps/s per window of N_agg (0: skip)")

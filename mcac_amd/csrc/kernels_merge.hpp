@@ -98,49 +98,67 @@ __device__ __forceinline__ void publish_merge(const MergeArgs &a, const mcac_mer
 // Global loads are issued in two batches (member indices with the keeper's
 // caches and flags, then the sphere rows); everything after works from LDS,
 // so the block synchronises through LDS only (no thread reads another
-// thread's global stores here).
-__device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeArgs &a, mcac_merge_result &res_out) {
+// thread's global stores here).  mover_ref (LDS, or null): the finishing
+// block has just moved aggregate a.magg so that its spheres sit at
+// mover_ref + relative position, and thread q placed its q-th sphere; the
+// merge takes those positions from mover_ref instead of reading them back,
+// and thread q also rewrites that sphere when the mover is the one removed.
+__device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeArgs &a, mcac_merge_result &res_out,
+                                                  const double *mover_ref) {
     __shared__ double diffpos[3], refpos[3], r[3];
     __shared__ int ok;
     __shared__ int sidx[kSmallMerge];
     __shared__ double dcs[kSmallMerge], vols[kSmallMerge], rads[kSmallMerge], surfs[kSmallMerge];
     __shared__ double rlx[kSmallMerge], rly[kSmallMerge], rlz[kSmallMerge];
-    __shared__ double px[kSmallMerge], py[kSmallMerge], pz[kSmallMerge];
     __shared__ int nrow[kSmallMerge];
     __shared__ double crow[kSmallMerge * 8];
     __shared__ unsigned int row_over;
     __shared__ double s_V, s_svr2;
+    __shared__ mcac_merge_result s_res;
     const int n = a.n_k + a.n_r;
     const int k = a.keep;
+    const bool mover_keeps = a.keep == a.magg;
     // batch 1: member indices; thread 0 also takes the contact spheres, the
     // keeper's cached sums and the dirty-list state
     double cV = 0, cS = 0, cvx = 0, cvy = 0, cvz = 0, csr = 0, csvr2 = 0;
-    int dk = 0, dr = 0, csk = -1, csr_slot = -1, dsk = 0, dsr = 0;
+    int dk = 0, dr = 0, csk = -1, csr_slot = -1, dsk = 0, dsr = 0, s0 = 0;
     unsigned int dcount = 0;
     double m4[4] = {0, 0, 0, 0}, o4[4] = {0, 0, 0, 0}, mrel[3] = {0, 0, 0}, orel[3] = {0, 0, 0};
     for (int t = threadIdx.x; t < n; t += blockDim.x)
         sidx[t] = (t < a.n_k) ? b.members[a.start_k + t] : b.members[a.start_r + (t - a.n_k)];
     if (threadIdx.x == 0) {
+        // the moving aggregate's contact sphere (ms) and the other's (os); "mine"
+        // is the keeper's side (aggregat.cpp:496-509)
         const int ms = static_cast<int>(a.ms), os = static_cast<int>(a.os);
-        int mysphere = ms, othersphere = os;
-        if (a.keep != a.magg) {
-            mysphere = os;
-            othersphere = ms;
+        double mv[4], ot[4], mvr[3], otr[3];
+        mvr[0] = b.srx[ms], mvr[1] = b.sry[ms], mvr[2] = b.srz[ms];
+        otr[0] = b.srx[os], otr[1] = b.sry[os], otr[2] = b.srz[os];
+        mv[3] = b.sr[ms];
+        ot[0] = b.sx[os], ot[1] = b.sy[os], ot[2] = b.sz[os], ot[3] = b.sr[os];
+        if (mover_ref) {
+            for (int l = 0; l < 3; ++l) mv[l] = mover_ref[l] + mvr[l];
+        } else {
+            mv[0] = b.sx[ms], mv[1] = b.sy[ms], mv[2] = b.sz[ms];
         }
-        m4[0] = b.sx[mysphere], m4[1] = b.sy[mysphere], m4[2] = b.sz[mysphere], m4[3] = b.sr[mysphere];
-        o4[0] = b.sx[othersphere], o4[1] = b.sy[othersphere], o4[2] = b.sz[othersphere], o4[3] = b.sr[othersphere];
-        mrel[0] = b.srx[mysphere], mrel[1] = b.sry[mysphere], mrel[2] = b.srz[mysphere];
-        orel[0] = b.srx[othersphere], orel[1] = b.sry[othersphere], orel[2] = b.srz[othersphere];
+        for (int l = 0; l < 4; ++l) {
+            m4[l] = mover_keeps ? mv[l] : ot[l];
+            o4[l] = mover_keeps ? ot[l] : mv[l];
+        }
+        for (int l = 0; l < 3; ++l) {
+            mrel[l] = mover_keeps ? mvr[l] : otr[l];
+            orel[l] = mover_keeps ? otr[l] : mvr[l];
+        }
         cV = b.avol[k], cS = b.asurf[k], cvx = b.csvx[k], cvy = b.csvy[k], cvz = b.csvz[k];
         csr = b.csr[k], csvr2 = b.csvr2[k];
         dk = b.dirty[k], dr = b.dirty[a.rem];
         csk = b.cslot[k], csr_slot = b.cslot[a.rem];
         dsk = b.dslot[k], dsr = b.dslot[a.rem];
         dcount = b.ss->dirty_count;
+        s0 = b.members[a.start_k];  // the keeper's first sphere
         row_over = 0;
     }
     lds_barrier();
-    // batch 2: the sphere rows
+    // batch 2: the sphere rows (relative positions; absolute ones are not needed)
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
         const int sp = sidx[t];
         vols[t] = b.svol[sp];
@@ -149,16 +167,14 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         rlx[t] = b.srx[sp];
         rly[t] = b.sry[sp];
         rlz[t] = b.srz[sp];
-        px[t] = b.sx[sp];
-        py[t] = b.sy[sp];
-        pz[t] = b.sz[sp];
     }
+    double k0[3] = {0, 0, 0};
+    if (threadIdx.x == 0 && !(mover_ref && mover_keeps)) k0[0] = b.sx[s0], k0[1] = b.sy[s0], k0[2] = b.sz[s0];
     if (threadIdx.x == 0) {
         // re-check of the contact and the removed spheres' shift
         // (aggregat_list.cpp:375-378, aggregat.cpp:496-535)
         // contact(moving sphere, other sphere) in the reference's argument order
-        const bool mine_first = a.keep == a.magg;
-        const double *p1 = mine_first ? m4 : o4, *p2 = mine_first ? o4 : m4;
+        const double *p1 = mover_keeps ? m4 : o4, *p2 = mover_keeps ? o4 : m4;
         ok = contact(p1[0], p1[1], p1[2], p1[3], p2[0], p2[1], p2[2], p2[3], a.L) ? 1 : 0;
         const double mpos[3] = {m4[0], m4[1], m4[2]}, opos[3] = {o4[0], o4[1], o4[2]};
         for (int l = 0; l < 3; ++l) {
@@ -178,27 +194,36 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         return;
     }
     if (threadIdx.x == 0) {
-        refpos[0] = px[0];  // the keeper's first sphere
-        refpos[1] = py[0];
-        refpos[2] = pz[0];
+        // the keeper's first sphere (rlx[0] is its relative position)
+        if (mover_ref && mover_keeps) {
+            refpos[0] = mover_ref[0] + rlx[0];
+            refpos[1] = mover_ref[1] + rly[0];
+            refpos[2] = mover_ref[2] + rlz[0];
+        } else {
+            refpos[0] = k0[0];
+            refpos[1] = k0[1];
+            refpos[2] = k0[2];
+        }
     }
     lds_barrier();
-    // member list, and the removed spheres moved into the keeper's frame
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    // member list, and the removed spheres moved into the keeper's frame (the
+    // removed one's q-th sphere by thread q)
+    if (a.new_start != a.start_k)
+        for (int t = threadIdx.x; t < a.n_k; t += blockDim.x) b.members[a.new_start + t] = sidx[t];
+    for (int q = threadIdx.x; q < a.n_r; q += blockDim.x) {
+        const int t = a.n_k + q;
         const int sp = sidx[t];
-        if (t >= a.n_k || a.new_start != a.start_k) b.members[a.new_start + t] = sp;
-        if (t >= a.n_k) {
-            const double rx = rlx[t] + diffpos[0], ry = rly[t] + diffpos[1], rz = rlz[t] + diffpos[2];
-            rlx[t] = rx;
-            rly[t] = ry;
-            rlz[t] = rz;
-            b.srx[sp] = rx;
-            b.sry[sp] = ry;
-            b.srz[sp] = rz;
-            b.sx[sp] = rx + refpos[0];
-            b.sy[sp] = ry + refpos[1];
-            b.sz[sp] = rz + refpos[2];
-        }
+        b.members[a.new_start + t] = sp;
+        const double rx = rlx[t] + diffpos[0], ry = rly[t] + diffpos[1], rz = rlz[t] + diffpos[2];
+        rlx[t] = rx;
+        rly[t] = ry;
+        rlz[t] = rz;
+        b.srx[sp] = rx;
+        b.sry[sp] = ry;
+        b.srz[sp] = rz;
+        b.sx[sp] = rx + refpos[0];
+        b.sy[sp] = ry + refpos[1];
+        b.sz[sp] = rz + refpos[2];
     }
     lds_barrier();
     if (threadIdx.x == 0) {
@@ -270,7 +295,7 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         for (int l = 0; l < 3; ++l) r[l] = rr[l];
         s_V = V;
         s_svr2 = svr2;
-        mcac_merge_result &res = b.ms->res;
+        mcac_merge_result &res = s_res;
         res.keeper = k;
         res.removed = a.rem;
         res.n = n;
@@ -331,7 +356,7 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
             overlapping /= static_cast<double>(inter);
             coord = static_cast<double>(inter) / static_cast<double>(n);
         }
-        mcac_merge_result res = b.ms->res;
+        mcac_merge_result res = s_res;
         res.rmax = rmax;
         res.rg = rg;
         res.overlapping = overlapping;
@@ -343,7 +368,7 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
 
 __global__ __launch_bounds__(kBlock) void k_merge_small(DevBox b, MergeArgs a) {
     __shared__ mcac_merge_result res;
-    merge_small_block(b, a, res);
+    merge_small_block(b, a, res, nullptr);
     if (threadIdx.x == 0) publish_merge(a, res);
 }
 

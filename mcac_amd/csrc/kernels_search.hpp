@@ -66,11 +66,35 @@ __device__ __forceinline__ bool behind_gate(const DevBox &b, const SearchArgs &a
     return g != 0ull && g < a.seq && g > a.released;
 }
 
+// Bounds of the visited cells, loaded by lane v of wave 0 for cell v of a
+// host-supplied window of at most 64 cells (valid = 0: filter_phase loads them).
+struct CellBounds {
+    int valid, c0, sz;
+};
+
+__device__ __forceinline__ CellBounds load_cell_bounds(const DevBox &b, const SearchArgs &a) {
+    CellBounds cb{0, 0, 0};
+    const int lj = a.len[1], lk = a.len[2];
+    const int V = a.len[0] * lj * lk;
+    if (!a.known || V > 64) return cb;
+    cb.valid = 1;
+    const int v = static_cast<int>(threadIdx.x), n = a.n_div;
+    if (v < V) {
+        const int oi = v / (lj * lk), rem = v - oi * (lj * lk), oj = rem / lk, ok = rem - oj * lk;
+        const int c = (window_offset(a.b1[0] + oi, 0, n) * n + window_offset(a.b1[1] + oj, 0, n)) * n +
+                      window_offset(a.b1[2] + ok, 0, n);
+        cb.c0 = b.cstart[c];
+        cb.sz = b.cstart[c + 1] - cb.c0;
+    }
+    return cb;
+}
+
 // Filter work of block vb of the vg blocks on this step: the aggregates of the
 // reference's visited Verlet cells and the dirty list, bounding sweep,
-// candidate append (shared by k_filter, k_step and k_step_batch).
+// candidate append (shared by k_filter, k_step and k_step_batch).  With skip
+// the work is read-only (a step behind a closed gate appends nothing).
 __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, const SearchArgs &a, unsigned vb,
-                                                           unsigned vg) {
+                                                           unsigned vg, const CellBounds &cb, bool skip) {
     __shared__ int win[6];
     __shared__ double q[4];
     __shared__ int pre[kMaxVisit + 1];
@@ -130,7 +154,10 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
         if (threadIdx.x < 64) {
             const int v = threadIdx.x;
             int sz = 0, c0 = 0;
-            if (v < V) {
+            if (v < V && cb.valid) {
+                c0 = cb.c0;
+                sz = cb.sz;
+            } else if (v < V) {
                 int oi = v / (lj * lk), rem = v - oi * (lj * lk), oj = rem / lk, ok = rem - oj * lk;
                 int ci = window_offset(b1i + oi, 0, n), cj = window_offset(b1j + oj, 0, n),
                     ck = window_offset(b1k + ok, 0, n);
@@ -234,7 +261,7 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
         }
         cnt += 1;
         const double d = swept_sphere(sx, sy, sz, sr, x, y, z, r, a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
-        if (d < a.lpm) {
+        if (d < a.lpm && !skip) {
             append_candidate(b, a, id, rank, d);
             cnt += kAppendOne;
         }
@@ -254,7 +281,8 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
 
 __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, SearchArgs a) {
     if (behind_gate(b, a)) return;
-    const unsigned long long tot = block_sum(filter_phase(b, a, blockIdx.x, gridDim.x)) & 0xffffffffull;
+    const unsigned long long tot =
+        block_sum(filter_phase(b, a, blockIdx.x, gridDim.x, CellBounds{0, 0, 0}, false)) & 0xffffffffull;
     if (threadIdx.x == 0 && tot) atomicAdd(&b.ss->nquery, tot);
 }
 
@@ -264,6 +292,16 @@ struct StepTotals {
     unsigned int nfilt;         // candidates appended (may exceed the capacity)
     int overflow;
     int src_slot;               // packed cell entry of the mover (-1: none)
+    long long tpairs;           // sphere pairs of the survivors
+};
+
+// Survivors of a one-launch step (at most kStepCands), kept in LDS by the
+// finishing block: bounding distance, visit rank, label, size and member
+// segment, exact (d, pair) minimum, and the multimap order.
+struct StepCands {
+    double db[kStepCands], ex[kStepCands];
+    long long pair[kStepCands];
+    int rank[kStepCands], id[kStepCands], nk[kStepCands], sk[kStepCands], order[kStepCands];
 };
 
 // Sphere of the mover a thread places in the finishing block, loaded early
@@ -290,24 +328,41 @@ __device__ __forceinline__ long long chunk_count(long long pairs, long long chun
 }
 
 // Last block of a step: orders the candidates like the reference's multimap,
-// runs its early-exit scan over the exact per-candidate distances
-// (cand_exact / cand_pair), moves the aggregate, publishes the record and
-// resets the search counters.
-__device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a, unsigned int F, long long nsrc,
-                                            int ssrc, const StepTotals &t, const MoverSphere &m) {
+// runs its early-exit scan over the exact per-candidate distances (from LDS
+// when lc is given, else cand_exact / cand_pair), moves the aggregate,
+// publishes the record and resets the search counters.  Returns (in thread 0)
+// whether the step closed the gate for the steps queued behind it.
+__device__ __forceinline__ bool finish_step(const DevBox &b, const SearchArgs &a, unsigned int F, long long nsrc,
+                                            int ssrc, const StepTotals &t, const MoverSphere &m, StepCands *lc) {
+    bool closed = false;
     // std::multimap order: (bounding distance, visit rank, label)
-    for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
-        const double dc = b.cand_db[c];
-        const int rc = b.cand_rank[c], ic = b.cand_id[c];
-        unsigned int pos = 0;
-        for (unsigned int e = 0; e < F; ++e) {
-            double de = b.cand_db[e];
-            int re = b.cand_rank[e];
-            pos += (de < dc || (de == dc && (re < rc || (re == rc && b.cand_id[e] < ic)))) ? 1u : 0u;
+    if (lc) {
+        for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
+            const double dc = lc->db[c];
+            const int rc = lc->rank[c], ic = lc->id[c];
+            unsigned int pos = 0;
+            for (unsigned int e = 0; e < F; ++e) {
+                const double de = lc->db[e];
+                const int re = lc->rank[e];
+                pos += (de < dc || (de == dc && (re < rc || (re == rc && lc->id[e] < ic)))) ? 1u : 0u;
+            }
+            lc->order[pos] = static_cast<int>(c);
         }
-        b.order[pos] = static_cast<int>(c);
+        if (F > 0) lds_barrier();
+    } else {
+        for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
+            const double dc = b.cand_db[c];
+            const int rc = b.cand_rank[c], ic = b.cand_id[c];
+            unsigned int pos = 0;
+            for (unsigned int e = 0; e < F; ++e) {
+                double de = b.cand_db[e];
+                int re = b.cand_rank[e];
+                pos += (de < dc || (de == dc && (re < rc || (re == rc && b.cand_id[e] < ic)))) ? 1u : 0u;
+            }
+            b.order[pos] = static_cast<int>(c);
+        }
+        if (F > 0) __syncthreads();  // order and exact distances come from other threads
     }
-    if (F > 0) __syncthreads();  // order and exact distances come from other threads
     __shared__ double s_ref[3];
     __shared__ StepRecord srec;  // assembled here, copied to host memory by one wave
     SearchState *s = b.ss;
@@ -315,41 +370,47 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
         double closest = kInf;
         int best = -1;
         for (unsigned int pos = 0; pos < F; ++pos) {
-            const int c = b.order[pos];
+            const int c = lc ? lc->order[pos] : b.order[pos];
             if (closest <= 0.) break;
-            if (closest < b.cand_db[c]) break;
-            if (b.cand_exact[c] < closest) {
-                closest = b.cand_exact[c];
+            if (closest < (lc ? lc->db[c] : b.cand_db[c])) break;
+            const double ex = lc ? lc->ex[c] : b.cand_exact[c];
+            if (ex < closest) {
+                closest = ex;
                 best = c;
             }
         }
+        // every load of this thread comes before its stores (no reloads through global memory)
         long long ms = -1, os = -1, oagg = -1;
         if (best >= 0) {
-            long long p = b.cand_pair[best];
-            int k = b.cand_id[best];
-            long long nk = b.acount[k];
-            long long i = p / nk, j = p - i * nk;
+            const long long p = lc ? lc->pair[best] : b.cand_pair[best];
+            const int k = lc ? lc->id[best] : b.cand_id[best];
+            const long long nk = lc ? lc->nk[best] : b.acount[k];
+            const int sk = lc ? lc->sk[best] : b.astart[k];
+            const long long i = p / nk, j = p - i * nk;
             ms = b.members[ssrc + i];
-            os = b.members[b.astart[k] + j];
+            os = b.members[sk + j];
             oagg = k;
         }
         // calcul.cpp:128-130 and aggregat.cpp:148-153
         const int contact = closest <= a.lpm;
         const double move = contact ? closest : a.lpm;
-        double old[3], rc3[3];
+        double old[3], rc3[3], rmax_src;
         if (a.known) {
             old[0] = a.sx, old[1] = a.sy, old[2] = a.sz;
             rc3[0] = a.srx, rc3[1] = a.sry, rc3[2] = a.srz;
+            rmax_src = a.srmax;
         } else {
             old[0] = b.ax[a.src], old[1] = b.ay[a.src], old[2] = b.az[a.src];
             rc3[0] = b.arx[a.src], rc3[1] = b.ary[a.src], rc3[2] = b.arz[a.src];
+            rmax_src = b.armax[a.src];
         }
+        double ncom[3], nref[3];
         for (int l = 0; l < 3; ++l) {
-            s->newcom[l] = periodic_position(old[l] + a.dir[l] * move, a.L);
-            s->refpos[l] = s->newcom[l] - rc3[l];
-            s_ref[l] = s->refpos[l];
+            ncom[l] = periodic_position(old[l] + a.dir[l] * move, a.L);
+            nref[l] = ncom[l] - rc3[l];
+            s_ref[l] = nref[l];
         }
-        s->newcell = ref_cell(s->newcom[0], s->newcom[1], s->newcom[2], a.n_div, a.L);
+        const int ncell = ref_cell(ncom[0], ncom[1], ncom[2], a.n_div, a.L);
         StepRecord *o = &srec;
         o->dist = closest;
         o->move = move;
@@ -360,14 +421,23 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
         o->status = t.overflow ? MCAC_E_CAPACITY : 0;
         o->nquery = t.nquery;
         o->nfilt = t.nfilt;
-        unsigned long long tp = 0;
-        for (unsigned int c = 0; c < F; ++c) tp += static_cast<unsigned long long>(nsrc) * b.acount[b.cand_id[c]];
-        o->npairs = tp;
+        o->npairs = static_cast<unsigned long long>(t.tpairs);
+        o->skipped = 0;
+        o->merged_inline = 0;
+        o->m_new_start = 0;
+        o->m_new_cap = 0;
+        o->pad = 0;
+        o->dirty_count = a.dirty_n + ((a.apply && a.dirty_slot >= 0) ? 1 : 0);
+        for (int l = 0; l < 3; ++l) {
+            s->newcom[l] = ncom[l];
+            s->refpos[l] = nref[l];
+        }
+        s->newcell = ncell;
         if (a.apply) {
-            b.ax[a.src] = s->newcom[0];
-            b.ay[a.src] = s->newcom[1];
-            b.az[a.src] = s->newcom[2];
-            b.acell[a.src] = s->newcell;
+            b.ax[a.src] = ncom[0];
+            b.ay[a.src] = ncom[1];
+            b.az[a.src] = ncom[2];
+            b.acell[a.src] = ncell;
             if (a.dirty_slot >= 0) {  // first move since the cell-list build
                 b.dirty[a.src] = 1;
                 b.dirty_list[a.dirty_slot] = a.src;
@@ -378,16 +448,10 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
                     b.cf[t.src_slot].w = -1.f;
                 }
             }
-            set_dirty_entry(b, a.dirty_pos, s->newcom[0], s->newcom[1], s->newcom[2],
-                            a.known ? a.srmax : b.armax[a.src], s->newcell);
+            set_dirty_entry(b, a.dirty_pos, ncom[0], ncom[1], ncom[2], rmax_src, ncell);
         }
         if (a.gated && contact) s->gate = a.seq;  // the steps behind wait for the merge
-        o->skipped = 0;
-        o->merged_inline = 0;
-        o->m_new_start = 0;
-        o->m_new_cap = 0;
-        o->pad = 0;
-        o->dirty_count = a.dirty_n + ((a.apply && a.dirty_slot >= 0) ? 1 : 0);
+        closed = a.gated && contact;
         s->nfilt = 0;
         s->nquery = 0;
         s->overflow = 0;
@@ -457,8 +521,10 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
         }
         lds_barrier();
         if (s_inline) {
-            __syncthreads();  // the mover's spheres, placed by every thread, are in memory
-            merge_small_block(b, s_ma, srec.merge);
+            // the merge takes the mover's sphere positions from s_ref, and the
+            // thread that placed a sphere of the mover also remaps it when the
+            // mover is removed: no wait for the placements above
+            merge_small_block(b, s_ma, srec.merge, s_ref);
             if (threadIdx.x == 0) srec.merged_inline = 1;  // thread 0 also copies the record
         }
     }
@@ -474,6 +540,7 @@ __device__ __forceinline__ void finish_step(const DevBox &b, const SearchArgs &a
             __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
+    return closed;
 }
 
 __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
@@ -589,7 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     t.nfilt = b.ss->nfilt;
     t.overflow = b.ss->overflow != 0u;
     t.src_slot = b.cslot[a.src];
-    finish_step(b, a, F, nsrc, ssrc, t, MoverSphere{0, -1, 0., 0., 0.});
+    t.tpairs = tpairs;
+    finish_step(b, a, F, nsrc, ssrc, t, MoverSphere{0, -1, 0., 0., 0.}, nullptr);
 }
 
 // One MC step for a small mover, worked on by vg blocks (vb is this one):
@@ -599,12 +667,28 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
 // block should take, it leaves them in place, closes the gate for the steps
 // queued behind and publishes a pending record: the host then launches k_solve
 // for the same step.  With vg == 1 the block is the last one by construction.
-__device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, unsigned vb, unsigned vg) {
-    if (behind_gate(b, a)) return;  // the host knows it is skipped: no record
-    // loads the finishing block will need, issued before the filter work
+// gate_open: the caller knows the gate is open for this step; otherwise the
+// gate is read with the first loads and a step behind it does no more than
+// read-only filter work (the host knows it is skipped: no record).  Returns (in
+// thread 0) whether the step closed the gate for the steps behind it.
+__device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, unsigned vb, unsigned vg,
+                                          bool gate_open) {
+    // independent loads first, so that their latencies overlap: the gate, the
+    // mover's cell entry and sphere indices, the bounds of the visited cells
+    const unsigned long long gate = (!gate_open && a.gated) ? b.ss->gate : 0ull;
     const int src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
-    const MoverSphere mover = load_mover_sphere(b, a);
-    const unsigned long long cnt = block_sum(filter_phase(b, a, vb, vg));
+    const bool mload = a.apply && a.n_mov <= kBlock;
+    const int msp = (mload && static_cast<int>(threadIdx.x) < a.n_mov) ? b.members[a.start + threadIdx.x] : -1;
+    const CellBounds cb = load_cell_bounds(b, a);
+    const bool skip = gate != 0ull && gate < a.seq && gate > a.released;
+    MoverSphere mover{mload, msp, 0., 0., 0.};
+    if (msp >= 0) {
+        mover.rx = b.srx[msp];
+        mover.ry = b.sry[msp];
+        mover.rz = b.srz[msp];
+    }
+    const unsigned long long cnt = block_sum(filter_phase(b, a, vb, vg, cb, skip));
+    if (skip) return false;
     // one atomic per block: finished blocks, appended candidates, visited aggregates
     __shared__ bool last;
     __shared__ unsigned long long s_total;
@@ -617,7 +701,7 @@ __device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, 
             s_total = old + mine;
         }
         __syncthreads();
-        if (!last) return;
+        if (!last) return false;
         __threadfence();
     } else {
         if (threadIdx.x == 0) s_total = cnt;
@@ -633,13 +717,29 @@ __device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, 
     const unsigned int F = t.overflow ? static_cast<unsigned int>(a.cap) : t.nfilt;
     const long long nsrc = a.n_mov;
     const int ssrc = a.start;
+    __shared__ StepCands sc;
+    __shared__ int s_done[kStepCands];
+    t.tpairs = 0;
     if (F > 0) {
-        long long tpairs = 0;
-        for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) tpairs += nsrc * b.acount[b.cand_id[c]];
-        tpairs = block_sum(tpairs);
+        // survivors into LDS: label, size, member segment, bounding key
+        long long tp = 0;
+        for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
+            const int k = b.cand_id[c];
+            const int nk = b.acount[k];
+            tp += nsrc * nk;
+            if (c < static_cast<unsigned int>(kStepCands)) {
+                sc.id[c] = k;
+                sc.nk[c] = nk;
+                sc.sk[c] = b.astart[k];
+                sc.db[c] = b.cand_db[c];
+                sc.rank[c] = b.cand_rank[c];
+            }
+        }
+        const long long tpairs = block_sum(tp);
+        t.tpairs = tpairs;
         __shared__ bool handover;
         if (threadIdx.x == 0) handover = F > static_cast<unsigned int>(kStepCands) || tpairs > kStepPairs || t.overflow;
-        __syncthreads();
+        lds_barrier();
         if (handover) {
             if (threadIdx.x == 0) {
                 s->tick = 0ull;
@@ -652,20 +752,16 @@ __device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, 
                 __threadfence_system();
                 __hip_atomic_store(&o->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
             }
-            return;
+            return a.gated != 0;
         }
         // exact sphere-pair distances, lexicographic (d, i, j) minimum per
         // candidate: a candidate with at most kThreadPairs pairs is walked by
         // one thread in (i, j) order (strict < keeps the first minimum), the
         // others by the whole block
-        __shared__ int s_nk[kStepCands], s_sk[kStepCands], s_done[kStepCands];
         if (threadIdx.x < F) {
             const unsigned int c = threadIdx.x;
-            const int k = b.cand_id[c];
-            const int nk = b.acount[k];
-            const int sk = b.astart[k];
-            s_nk[c] = nk;
-            s_sk[c] = sk;
+            const int nk = sc.nk[c];  // this thread's own entries
+            const int sk = sc.sk[c];
             int done = 0;
             if (nsrc * nk <= kThreadPairs) {
                 double bd = kInf;
@@ -677,8 +773,8 @@ __device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, 
                                                   b.sz[sj], b.sr[sj], a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
                     lexmin(bd, bp, d, p);
                 }
-                b.cand_exact[c] = bd;
-                b.cand_pair[c] = (bd < kInf) ? bp : -1;
+                sc.ex[c] = bd;
+                sc.pair[c] = (bd < kInf) ? bp : -1;
                 done = 1;
             }
             s_done[c] = done;
@@ -686,8 +782,8 @@ __device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, 
         lds_barrier();
         for (unsigned int c = 0; c < F; ++c) {
             if (s_done[c]) continue;
-            const long long nk = s_nk[c];
-            const int sk = s_sk[c];
+            const long long nk = sc.nk[c];
+            const int sk = sc.sk[c];
             double bd = kInf;
             long long bp = 0x7fffffffffffffffLL;
             for (long long p = threadIdx.x; p < nsrc * nk; p += blockDim.x) {
@@ -701,40 +797,62 @@ __device__ __forceinline__ void step_body(const DevBox &b, const SearchArgs &a, 
             }
             block_lexmin(bd, bp);
             if (threadIdx.x == 0) {
-                b.cand_exact[c] = bd;
-                b.cand_pair[c] = (bd < kInf) ? bp : -1;
+                sc.ex[c] = bd;
+                sc.pair[c] = (bd < kInf) ? bp : -1;
             }
         }
     }
-    finish_step(b, a, F, nsrc, ssrc, t, mover);
+    return finish_step(b, a, F, nsrc, ssrc, t, mover, &sc);
 }
 
-__global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) { step_body(b, a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
+    step_body(b, a, blockIdx.x, gridDim.x, false);
+}
 
 // Steps of many boxes in one launch: block i runs jobs seg[i] .. seg[i+1]-1,
-// consecutive queued steps of one box, in order (jobs in pinned host memory).
-// Small boxes are bound by launch and memory latency, so their steps share
-// launches instead of each taking one; a step behind a contact of an earlier
-// job of the same block is skipped by the gate as in separate launches.
+// consecutive queued steps of one box (at most kMaxBlockJobs), in order; jobs
+// are in pinned host memory.  Small boxes are bound by launch and memory
+// latency, so their steps share launches instead of each taking one.  A step
+// behind a contact (or a hand-over to k_solve) of an earlier job of the block
+// does nothing, as it would in separate launches behind the gate.
 struct StepJob {
     DevBox b;
     SearchArgs a;
 };
-static_assert(sizeof(StepJob) % sizeof(unsigned long long) == 0, "job copied in 8-byte words");
+constexpr int kMaxBlockJobs = 8;
+static_assert(sizeof(DevBox) % sizeof(unsigned long long) == 0 && sizeof(SearchArgs) % sizeof(unsigned long long) == 0,
+              "job copied in 8-byte words");
 
 __global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, const int *seg) {
-    __shared__ StepJob job;
-    constexpr int kWords = static_cast<int>(sizeof(StepJob) / sizeof(unsigned long long));
-    const int j0 = seg[blockIdx.x], j1 = seg[blockIdx.x + 1];
-    for (int j = j0; j < j1; ++j) {
-        // the previous step's writes (positions, lists, gate) and its use of
-        // the shared state are complete before the next step starts
-        __syncthreads();
-        const unsigned long long *src = reinterpret_cast<const unsigned long long *>(jobs + j);
-        unsigned long long *dst = reinterpret_cast<unsigned long long *>(&job);
-        for (int i = threadIdx.x; i < kWords; i += blockDim.x) dst[i] = src[i];
-        __syncthreads();
-        step_body(job.b, job.a, 0u, 1u);
+    __shared__ DevBox box;
+    __shared__ SearchArgs args[kMaxBlockJobs];
+    __shared__ int s_closed;
+    constexpr int kBoxWords = static_cast<int>(sizeof(DevBox) / sizeof(unsigned long long));
+    constexpr int kArgWords = static_cast<int>(sizeof(SearchArgs) / sizeof(unsigned long long));
+    const int j0 = seg[blockIdx.x];
+    int nj = seg[blockIdx.x + 1] - j0;
+    nj = nj < 0 ? 0 : (nj > kMaxBlockJobs ? kMaxBlockJobs : nj);  // the host never lists more
+    // one round trip: the box (the same for every job of the block) and every job's arguments
+    unsigned long long *bw = reinterpret_cast<unsigned long long *>(&box);
+    unsigned long long *aw = reinterpret_cast<unsigned long long *>(args);
+    for (int i = threadIdx.x; i < kBoxWords + nj * kArgWords; i += blockDim.x) {
+        if (i < kBoxWords) {
+            bw[i] = reinterpret_cast<const unsigned long long *>(&jobs[j0].b)[i];
+        } else {
+            const int q = (i - kBoxWords) / kArgWords, w = (i - kBoxWords) - q * kArgWords;
+            aw[q * kArgWords + w] = reinterpret_cast<const unsigned long long *>(&jobs[j0 + q].a)[w];
+        }
+    }
+    if (threadIdx.x == 0) s_closed = 0;
+    __syncthreads();
+    if (nj == 0 || behind_gate(box, args[0])) return;  // every job of the block is behind the same gate
+    for (int q = 0; q < nj; ++q) {
+        // the previous step's writes (positions, lists) and its use of the
+        // shared state are complete before the next step starts
+        if (q > 0) __syncthreads();
+        if (s_closed) break;
+        const bool closed = step_body(box, args[q], 0u, 1u, true);
+        if (threadIdx.x == 0 && closed) s_closed = 1;
     }
 }
 

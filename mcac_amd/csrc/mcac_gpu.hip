@@ -1041,6 +1041,8 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
             if (l.last != b) l.seg[l.blocks++] = l.n;  // a new block: the box's first job of this launch
             l.last = b;
             l.n += 1;
+            // a block runs at most kMaxBlockJobs jobs (a box has at most that many steps in flight)
+            if (l.n - l.seg[l.blocks - 1] >= kMaxBlockJobs && (rc = group_flush(g, l)) != 0) break;
         } else {
             g->single_steps += 1;
             l.last = nullptr;
