@@ -125,6 +125,17 @@ struct SearchArgs {
     int prec;                  // 0 FP64; 1 FP32 capsule prune of cell entries; 2 as 1 plus miss count
     float margin32;            // FP32 prune margin added to the radius sum [m]
     long long pool_used, pool_cap;  // member pool state when this step runs
+    int nc_ok;                 // the host computed the move without contact (known mover):
+    int nc_cell;               //   its reference cell,
+    double nc_com[3], nc_ref[3];  //   new CoM and CoM - relative CoM (aggregat.cpp:148-161)
+};
+
+// Bounds of the visited cells of a batched step, supplied by the host from
+// its copy of the cell starts (n = 0: the kernel reads them from cstart).
+struct CellTable {
+    int n;
+    int c0[64], sz[64];  // first CSR entry and entry count of visited cell v
+    int pad;
 };
 
 struct MergeArgs {

@@ -123,7 +123,7 @@ __host__ __device__ __forceinline__ void verlet_window(const double src[3], cons
 }
 
 // Offset of cell coordinate c in a window starting at b1 (n cells per axis).
-__device__ __forceinline__ int window_offset(int c, int b1, int n) {
+__host__ __device__ __forceinline__ int window_offset(int c, int b1, int n) {
     int o = c - b1;
     if (o < 0) o += n;
     else if (o >= n) o -= n;

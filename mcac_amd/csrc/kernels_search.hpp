@@ -72,11 +72,20 @@ struct CellBounds {
     int valid, c0, sz;
 };
 
-__device__ __forceinline__ CellBounds load_cell_bounds(const DevBox &b, const SearchArgs &a) {
+__device__ __forceinline__ CellBounds load_cell_bounds(const DevBox &b, const SearchArgs &a, const CellTable *ct) {
     CellBounds cb{0, 0, 0};
     const int lj = a.len[1], lk = a.len[2];
     const int V = a.len[0] * lj * lk;
     if (!a.known || V > 64) return cb;
+    if (ct && ct->n == V) {  // the host's table (in LDS): lane v takes cell v's pair
+        cb.valid = 1;
+        const int v = static_cast<int>(threadIdx.x);
+        if (v < V) {
+            cb.c0 = ct->c0[v];
+            cb.sz = ct->sz[v];
+        }
+        return cb;
+    }
     cb.valid = 1;
     const int v = static_cast<int>(threadIdx.x), n = a.n_div;
     if (v < V) {
@@ -405,12 +414,22 @@ __device__ __forceinline__ bool finish_step(const DevBox &b, const SearchArgs &a
             rmax_src = b.armax[a.src];
         }
         double ncom[3], nref[3];
-        for (int l = 0; l < 3; ++l) {
-            ncom[l] = periodic_position(old[l] + a.dir[l] * move, a.L);
-            nref[l] = ncom[l] - rc3[l];
-            s_ref[l] = nref[l];
+        int ncell;
+        if (!contact && a.nc_ok) {  // the full move, computed by the host with the same operations
+            for (int l = 0; l < 3; ++l) {
+                ncom[l] = a.nc_com[l];
+                nref[l] = a.nc_ref[l];
+                s_ref[l] = nref[l];
+            }
+            ncell = a.nc_cell;
+        } else {
+            for (int l = 0; l < 3; ++l) {
+                ncom[l] = periodic_position(old[l] + a.dir[l] * move, a.L);
+                nref[l] = ncom[l] - rc3[l];
+                s_ref[l] = nref[l];
+            }
+            ncell = ref_cell(ncom[0], ncom[1], ncom[2], a.n_div, a.L);
         }
-        const int ncell = ref_cell(ncom[0], ncom[1], ncom[2], a.n_div, a.L);
         StepRecord *o = &srec;
         o->dist = closest;
         o->move = move;
@@ -667,19 +686,20 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
 // block should take, it leaves them in place, closes the gate for the steps
 // queued behind and publishes a pending record: the host then launches k_solve
 // for the same step.  With vg == 1 the block is the last one by construction.
+// ct: bounds of the visited cells from the host (in LDS), or null.
 // gate_open: the caller knows the gate is open for this step; otherwise the
 // gate is read with the first loads and a step behind it does no more than
 // read-only filter work (the host knows it is skipped: no record).  Returns (in
 // thread 0) whether the step closed the gate for the steps behind it.
 __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, unsigned vb, unsigned vg,
-                                          bool gate_open) {
+                                          bool gate_open, const CellTable *ct) {
     // independent loads first, so that their latencies overlap: the gate, the
     // mover's cell entry and sphere indices, the bounds of the visited cells
     const unsigned long long gate = (!gate_open && a.gated) ? b.ss->gate : 0ull;
     const int src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
     const bool mload = a.apply && a.n_mov <= kBlock;
     const int msp = (mload && static_cast<int>(threadIdx.x) < a.n_mov) ? b.members[a.start + threadIdx.x] : -1;
-    const CellBounds cb = load_cell_bounds(b, a);
+    const CellBounds cb = load_cell_bounds(b, a, ct);
     const bool skip = gate != 0ull && gate < a.seq && gate > a.released;
     MoverSphere mover{mload, msp, 0., 0., 0.};
     if (msp >= 0) {
@@ -806,7 +826,7 @@ __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
-    step_body(b, a, blockIdx.x, gridDim.x, false);
+    step_body(b, a, blockIdx.x, gridDim.x, false, nullptr);
 }
 
 // Steps of many boxes in one launch: block i runs jobs seg[i] .. seg[i+1]-1,
@@ -818,29 +838,39 @@ __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
 struct StepJob {
     DevBox b;
     SearchArgs a;
+    CellTable cells;
 };
 constexpr int kMaxBlockJobs = 8;
-static_assert(sizeof(DevBox) % sizeof(unsigned long long) == 0 && sizeof(SearchArgs) % sizeof(unsigned long long) == 0,
+static_assert(sizeof(DevBox) % sizeof(unsigned long long) == 0 && sizeof(SearchArgs) % sizeof(unsigned long long) == 0 &&
+                  sizeof(CellTable) % sizeof(unsigned long long) == 0,
               "job copied in 8-byte words");
 
 __global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, const int *seg) {
     __shared__ DevBox box;
     __shared__ SearchArgs args[kMaxBlockJobs];
+    __shared__ CellTable cells[kMaxBlockJobs];
     __shared__ int s_closed;
     constexpr int kBoxWords = static_cast<int>(sizeof(DevBox) / sizeof(unsigned long long));
     constexpr int kArgWords = static_cast<int>(sizeof(SearchArgs) / sizeof(unsigned long long));
+    constexpr int kCellWords = static_cast<int>(sizeof(CellTable) / sizeof(unsigned long long));
     const int j0 = seg[blockIdx.x];
     int nj = seg[blockIdx.x + 1] - j0;
     nj = nj < 0 ? 0 : (nj > kMaxBlockJobs ? kMaxBlockJobs : nj);  // the host never lists more
-    // one round trip: the box (the same for every job of the block) and every job's arguments
+    // one round trip: the box (the same for every job of the block), every
+    // job's arguments and cell table
     unsigned long long *bw = reinterpret_cast<unsigned long long *>(&box);
     unsigned long long *aw = reinterpret_cast<unsigned long long *>(args);
-    for (int i = threadIdx.x; i < kBoxWords + nj * kArgWords; i += blockDim.x) {
+    unsigned long long *cw = reinterpret_cast<unsigned long long *>(cells);
+    const int na = nj * kArgWords;
+    for (int i = threadIdx.x; i < kBoxWords + na + nj * kCellWords; i += blockDim.x) {
         if (i < kBoxWords) {
             bw[i] = reinterpret_cast<const unsigned long long *>(&jobs[j0].b)[i];
-        } else {
+        } else if (i < kBoxWords + na) {
             const int q = (i - kBoxWords) / kArgWords, w = (i - kBoxWords) - q * kArgWords;
             aw[q * kArgWords + w] = reinterpret_cast<const unsigned long long *>(&jobs[j0 + q].a)[w];
+        } else {
+            const int q = (i - kBoxWords - na) / kCellWords, w = (i - kBoxWords - na) - q * kCellWords;
+            cw[q * kCellWords + w] = reinterpret_cast<const unsigned long long *>(&jobs[j0 + q].cells)[w];
         }
     }
     if (threadIdx.x == 0) s_closed = 0;
@@ -851,7 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, cons
         // shared state are complete before the next step starts
         if (q > 0) __syncthreads();
         if (s_closed) break;
-        const bool closed = step_body(box, args[q], 0u, 1u, true);
+        const bool closed = step_body(box, args[q], 0u, 1u, true, &cells[q]);
         if (threadIdx.x == 0 && closed) s_closed = 1;
     }
 }

@@ -39,6 +39,8 @@ struct mcac_box {
     int64_t n_spheres = 0, n_aggs = 0;
     int64_t cand_cap = 0, part_cap = 0, pool_cap = 0;
     int n_cells = 0, n_built = 0;
+    int *h_cstart = nullptr;         // pinned copy of the cell starts of the current build (small grids)
+    bool cstart_ok = false;
     // host mirrors for allocation, launch sizing and the dirty list
     std::vector<int> count, start, cap;
     std::vector<int> alive_ids;  // ascending, may hold dead ids until rebuilt
@@ -190,6 +192,14 @@ int rebuild_cells(mcac_box *b) {
     b->dirty_epoch += 1;
     b->n_built = n_alive;
     b->n_rebuilds++;
+    b->cstart_ok = false;
+    if (b->h_cstart) {
+        // the host passes the visited cells' bounds with each step
+        HIP_TRY(hipMemcpyAsync(b->h_cstart, b->d.cstart, (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
+                               hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        b->cstart_ok = true;
+    }
     return 0;
 }
 
@@ -298,6 +308,13 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
         const double src[3] = {a.sx, a.sy, a.sz};
         const double vec[3] = {lpm * dir[0], lpm * dir[1], lpm * dir[2]};
         verlet_window(src, vec, a.srmax + maxradius, static_cast<int>(b->p.n_div), b->p.box_length, a.b1, a.len);
+        // the move without contact (finish_step: calcul.cpp:128-130, aggregat.cpp:148-153)
+        for (int l = 0; l < 3; ++l) {
+            a.nc_com[l] = periodic_position(src[l] + dir[l] * lpm, b->p.box_length);
+            a.nc_ref[l] = a.nc_com[l] - (l == 0 ? a.srx : (l == 1 ? a.sry : a.srz));
+        }
+        a.nc_cell = ref_cell(a.nc_com[0], a.nc_com[1], a.nc_com[2], static_cast<int>(b->p.n_div), b->p.box_length);
+        a.nc_ok = 1;
     }
     a.dirty_n = static_cast<int>(b->dirty_count);
     a.dirty_slot = -1;
@@ -361,6 +378,23 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     // interval does not include waiting behind the steps queued before it
     if (timed && b->inflight > 0) HIP_TRY(hipStreamSynchronize(b->stream));
     if (job_out && one_block_step(b, n_mov, timed)) {
+        // a batched step's arguments sit in LDS: the host adds the bounds of
+        // the visited cells, saving the kernel a dependent load (a launch of
+        // its own reads them faster from cstart than from its arguments)
+        const int V = a.len[0] * a.len[1] * a.len[2];
+        CellTable &ct = job_out->cells;
+        ct.n = 0;
+        if (a.known && b->cstart_ok && V <= 64) {
+            const int n = static_cast<int>(b->p.n_div), lj = a.len[1], lk = a.len[2];
+            for (int v = 0; v < V; ++v) {
+                const int oi = v / (lj * lk), rem = v - oi * (lj * lk), oj = rem / lk, ok = rem - oj * lk;
+                const int c = (window_offset(a.b1[0] + oi, 0, n) * n + window_offset(a.b1[1] + oj, 0, n)) * n +
+                              window_offset(a.b1[2] + ok, 0, n);
+                ct.c0[v] = b->h_cstart[c];
+                ct.sz[v] = b->h_cstart[c + 1] - b->h_cstart[c];
+            }
+            ct.n = V;
+        }
         job_out->b = b->d;
         job_out->a = a;
         *batched = true;
@@ -594,6 +628,10 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.row_cnt, Ns); ALLOC(d.row_j, Ns * kOverlapSlots); ALLOC(d.row_c, Ns * kOverlapSlots);
     ALLOC(d.ss, 1); ALLOC(d.ms, 1);
 #undef ALLOC
+    if (b->n_cells + 1 <= (1 << 20) &&
+        hipHostMalloc(reinterpret_cast<void **>(&b->h_cstart), (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
+                      hipHostMallocDefault) != hipSuccess)
+        return fail(MCAC_E_DEVICE);
     if (hipHostMalloc(reinterpret_cast<void **>(&b->h_step), mcac_box::kRing * sizeof(StepRecord),
                       hipHostMallocCoherent) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void **>(&b->h_merge), sizeof(MergeRecord), hipHostMallocCoherent) != hipSuccess)
@@ -618,6 +656,7 @@ int mcac_box_destroy(mcac_box *b) {
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->h_step) (void)hipHostFree(b->h_step);
     if (b->h_merge) (void)hipHostFree(b->h_merge);
+    if (b->h_cstart) (void)hipHostFree(b->h_cstart);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &row : b->sev)
