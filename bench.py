@@ -163,7 +163,7 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=4, help="host threads sharing the GPU (ensemble, serving)")
     ap.add_argument("--group-boxes", type=int, default=4,
                     help="boxes per worker stepped in lock step, one launch per round (0: one box per worker)")
-    ap.add_argument("--group-depth", type=int, default=6, help="steps of each box per grouped launch (1..8)")
+    ap.add_argument("--group-depth", type=int, default=8, help="steps of each box per grouped launch (1..8)")
     ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")
     ap.add_argument("--c3-windows", type=float, default=0.0,
                     help="seconds: run the N=1e6 box on and report MC steps/s per window of N_agg (0: skip)")

@@ -339,10 +339,12 @@ __device__ __forceinline__ long long chunk_count(long long pairs, long long chun
 // Last block of a step: orders the candidates like the reference's multimap,
 // runs its early-exit scan over the exact per-candidate distances (from LDS
 // when lc is given, else cand_exact / cand_pair), moves the aggregate,
-// publishes the record and resets the search counters.  Returns (in thread 0)
-// whether the step closed the gate for the steps queued behind it.
+// writes the record (publishing it unless the caller publishes later) and
+// resets the search counters.  Returns (in thread 0) whether the step closed
+// the gate for the steps queued behind it.
 __device__ __forceinline__ bool finish_step(const DevBox &b, const SearchArgs &a, unsigned int F, long long nsrc,
-                                            int ssrc, const StepTotals &t, const MoverSphere &m, StepCands *lc) {
+                                            int ssrc, const StepTotals &t, const MoverSphere &m, StepCands *lc,
+                                            bool publish) {
     bool closed = false;
     // std::multimap order: (bounding distance, visit rank, label)
     if (lc) {
@@ -554,7 +556,7 @@ __device__ __forceinline__ bool finish_step(const DevBox &b, const SearchArgs &a
         if (static_cast<int>(threadIdx.x) < kWords)
             reinterpret_cast<unsigned long long *>(a.out)[threadIdx.x] =
                 reinterpret_cast<const unsigned long long *>(&srec)[threadIdx.x];
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && publish) {
             __threadfence_system();
             __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
@@ -676,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     t.overflow = b.ss->overflow != 0u;
     t.src_slot = b.cslot[a.src];
     t.tpairs = tpairs;
-    finish_step(b, a, F, nsrc, ssrc, t, MoverSphere{0, -1, 0., 0., 0.}, nullptr);
+    finish_step(b, a, F, nsrc, ssrc, t, MoverSphere{0, -1, 0., 0., 0.}, nullptr, true);
 }
 
 // One MC step for a small mover, worked on by vg blocks (vb is this one):
@@ -687,12 +689,14 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
 // queued behind and publishes a pending record: the host then launches k_solve
 // for the same step.  With vg == 1 the block is the last one by construction.
 // ct: bounds of the visited cells from the host (in LDS), or null.
+// publish: publish the record (else wave 0 writes its body and the caller
+// publishes the sequence number later, after a system-scope fence).
 // gate_open: the caller knows the gate is open for this step; otherwise the
 // gate is read with the first loads and a step behind it does no more than
 // read-only filter work (the host knows it is skipped: no record).  Returns (in
 // thread 0) whether the step closed the gate for the steps behind it.
 __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, unsigned vb, unsigned vg,
-                                          bool gate_open, const CellTable *ct) {
+                                          bool gate_open, const CellTable *ct, bool publish) {
     // independent loads first, so that their latencies overlap: the gate, the
     // mover's cell entry and sphere indices, the bounds of the visited cells
     const unsigned long long gate = (!gate_open && a.gated) ? b.ss->gate : 0ull;
@@ -769,8 +773,10 @@ __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, 
                 o->status = kPendingSolve;
                 o->contact = 0;
                 o->skipped = 0;
-                __threadfence_system();
-                __hip_atomic_store(&o->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (publish) {
+                    __threadfence_system();
+                    __hip_atomic_store(&o->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
             }
             return a.gated != 0;
         }
@@ -822,11 +828,11 @@ __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, 
             }
         }
     }
-    return finish_step(b, a, F, nsrc, ssrc, t, mover, &sc);
+    return finish_step(b, a, F, nsrc, ssrc, t, mover, &sc, publish);
 }
 
 __global__ __launch_bounds__(kBlock) void k_step(DevBox b, SearchArgs a) {
-    step_body(b, a, blockIdx.x, gridDim.x, false, nullptr);
+    step_body(b, a, blockIdx.x, gridDim.x, false, nullptr, true);
 }
 
 // Steps of many boxes in one launch: block i runs jobs seg[i] .. seg[i+1]-1,
@@ -876,13 +882,22 @@ __global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, cons
     if (threadIdx.x == 0) s_closed = 0;
     __syncthreads();
     if (nj == 0 || behind_gate(box, args[0])) return;  // every job of the block is behind the same gate
+    int ran = 0;
     for (int q = 0; q < nj; ++q) {
         // the previous step's writes (positions, lists) and its use of the
         // shared state are complete before the next step starts
         if (q > 0) __syncthreads();
         if (s_closed) break;
-        const bool closed = step_body(box, args[q], 0u, 1u, true, &cells[q]);
+        const bool closed = step_body(box, args[q], 0u, 1u, true, &cells[q], false);
+        ran = q + 1;
         if (threadIdx.x == 0 && closed) s_closed = 1;
+    }
+    // the records of the steps that ran, written by wave 0: one system-scope
+    // fence, then their sequence numbers (the host reads them after the launch)
+    if (threadIdx.x == 0 && ran > 0) {
+        __threadfence_system();
+        for (int q = 0; q < ran; ++q)
+            __hip_atomic_store(&args[q].out->seq, args[q].seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

@@ -1064,8 +1064,11 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
         const bool valid = agg >= 0 && agg < b->n_aggs;
         const bool batch = valid && one_block_step(b, b->count[static_cast<size_t>(agg)], next_step_timed(b));
         // a step launched on its own, or the cell-list rebuild that the
-        // enqueue launches, runs after every job already listed
-        if ((!batch || (rebuild_due(b) && l.last == b)) && (rc = group_flush(g, l)) != 0) break;
+        // enqueue launches, runs after every job already listed; a block
+        // runs at most kMaxBlockJobs jobs (a box has at most that many steps
+        // in flight, so this never splits a box's moves in practice)
+        const bool full = batch && l.last == b && l.n - l.seg[l.blocks - 1] >= kMaxBlockJobs;
+        if ((!batch || full || (rebuild_due(b) && l.last == b)) && (rc = group_flush(g, l)) != 0) break;
         unsigned long long t = 0;
         bool batched = false;
         rc = enqueue_search(b, agg, dirs + 3 * i, lpm[i], maxradius[i], 1, 1, &t, batch ? l.jobs + l.n : nullptr,
@@ -1080,8 +1083,6 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
             if (l.last != b) l.seg[l.blocks++] = l.n;  // a new block: the box's first job of this launch
             l.last = b;
             l.n += 1;
-            // a block runs at most kMaxBlockJobs jobs (a box has at most that many steps in flight)
-            if (l.n - l.seg[l.blocks - 1] >= kMaxBlockJobs && (rc = group_flush(g, l)) != 0) break;
         } else {
             g->single_steps += 1;
             l.last = nullptr;

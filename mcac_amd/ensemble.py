@@ -73,7 +73,7 @@ def serving_requests(n_requests: int, rate_per_s: float, s0: int = 5000, rng_see
 
 
 def batch_run(inis: list[str], arrivals=None, workers: int = 8, device: int = 0, max_steps: int = 2 ** 62,
-              boxes_per_worker: int = 0, depth: int = 6) -> dict:
+              boxes_per_worker: int = 0, depth: int = 8) -> dict:
     """Runs the jobs on `workers` host threads sharing the GPU (include/mcac_sim.h):
     one box per worker at a time (mcac_batch_run), or with boxes_per_worker >= 1
     up to that many boxes per worker stepped in lock step, up to `depth` steps
