@@ -36,3 +36,47 @@ def test_group_runs_match_oracle_runs(depth):
         assert res["steps"][j] == n, (j, res["steps"][j], n)
         assert res["final_time"][j] == ora.scalar("time"), (j, res["final_time"][j], ora.scalar("time"))
     sys.stdout.flush()
+
+
+def test_group_enqueue_rejects_interleaved_moves():
+    """A box's moves must be contiguous in one group enqueue, boxes must belong
+    to the group, and n is bounded by 8 moves per box of the group's capacity."""
+    import ctypes
+    import mcac_amd
+    print("case: group enqueue argument checks", flush=True)
+    L = mcac_amd.load_gpu()
+    g = ctypes.c_void_p()
+    assert L.mcac_group_create(0, 2, ctypes.byref(g)) == 0
+    boxes = []
+    try:
+        rng = np.random.default_rng(2)
+        for k in range(3):
+            b = mcac_amd.Box(1e-6, 4, 8, 8)
+            n = 4
+            pos = rng.uniform(0, 1e-6, (n, 3))
+            b.upload(dict(x=pos[:, 0], y=pos[:, 1], z=pos[:, 2], r=np.full(n, 1e-8), vol=np.full(n, 4.2e-24),
+                          surf=np.full(n, 1.3e-15), rx=np.zeros(n), ry=np.zeros(n), rz=np.zeros(n)),
+                     dict(pos=pos, rcm=np.zeros((n, 3)), rmax=np.full(n, 1e-8), volume=np.full(n, 4.2e-24),
+                          surface=np.full(n, 1.3e-15), count=np.ones(n, np.int64), members=np.arange(n)))
+            boxes.append(b)
+        assert L.mcac_group_add(g, boxes[0].h) == 0 and L.mcac_group_add(g, boxes[1].h) == 0
+        assert L.mcac_group_add(g, boxes[0].h) == 34  # already in a group
+
+        def enqueue(bs):
+            n = len(bs)
+            arr = (ctypes.c_void_p * n)(*[b.h.value if hasattr(b.h, "value") else b.h for b in bs])
+            aggs = np.zeros(n, np.int64)
+            dirs = np.tile([1.0, 0.0, 0.0], n)
+            lpm = np.full(n, 1e-9)
+            maxr = np.full(n, 1e-8)
+            t = (ctypes.c_uint64 * max(n, 1))()
+            return L.mcac_move_enqueue_group(g, arr, mcac_amd._ip(aggs), mcac_amd._dp(dirs), mcac_amd._dp(lpm),
+                                             mcac_amd._dp(maxr), n, t)
+
+        assert enqueue([boxes[0], boxes[1], boxes[0]]) == 34   # interleaved
+        assert enqueue([boxes[2]]) == 34                        # not in the group
+        assert enqueue([boxes[0]] * 17) == 34                   # more than 8 per box of capacity 2
+    finally:
+        for b in boxes:
+            b.close()
+        L.mcac_group_destroy(g)
