@@ -164,6 +164,12 @@ int mcac_move_enqueue_group(mcac_group *group, mcac_box *const *boxes, const int
                             const double *lpm, const double *maxradius, int n, uint64_t *tickets);
 /* [group launches, moves in group launches, moves launched on their own] */
 int mcac_group_stats(mcac_group *group, int64_t out[3]);
+/* 1 when a move of the box whose sweep reaches `reach` (the mover's rmax +
+ * maxradius + lpm) around its centre would currently run in one workgroup of
+ * a group launch (its cell window holds few enough aggregates), else 0.  No
+ * counterpart in the reference; lets a driver choose between group launches
+ * and one launch per move. */
+int mcac_box_block_steps(mcac_box *box, double reach);
 
 int mcac_box_download_spheres(mcac_box *box, double *x, double *y, double *z, double *rx, double *ry,
                               double *rz);

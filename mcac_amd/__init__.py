@@ -55,7 +55,7 @@ GPU_SYMBOLS = (
     "mcac_box_download_spheres", "mcac_box_sphere_owners", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
-    "mcac_group_stats",
+    "mcac_group_stats", "mcac_box_block_steps",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
@@ -124,6 +124,7 @@ def load_gpu():
         L.mcac_move_enqueue_group.argtypes = [_P, ctypes.POINTER(_P), _I64P, _DP, _DP, _DP, _I,
                                               ctypes.POINTER(ctypes.c_uint64)]
         L.mcac_group_stats.argtypes = [_P, _I64P]
+        L.mcac_box_block_steps.argtypes = [_P, _D]
         L.mcac_version.restype = ctypes.c_char_p
         _gpu = L
     return _gpu

@@ -116,6 +116,12 @@ struct Simulation {
     int precision = 0;  // box contact-search precision (mcac_box_params)
     mcac_box *box = nullptr;
     mcac_group *group = nullptr;  // lock-step form: the box works on the group's stream
+    mcac_group *own_group = nullptr;  // one-box group for rounds of steps in one launch
+    bool rounds = true;               // issue rounds through own_group when the steps fit one workgroup
+    std::vector<int64_t> r_aggs;      // scratch of issue_round
+    std::vector<double> r_dirs, r_lpm, r_maxr;
+    std::vector<uint64_t> r_tickets;
+    std::vector<mcac_box *> r_boxes;
     // spheres (index order) — host copy of the initial state only
     size_t N = 0;
     // aggregates by stable id
@@ -142,6 +148,7 @@ struct Simulation {
 
     ~Simulation() {
         if (box) mcac_box_destroy(box);
+        if (own_group) mcac_group_destroy(own_group);  // after the box that works on its stream
     }
 
     // Monomer update() (aggregat.cpp:284-288) for a single sphere of radius r.
@@ -307,6 +314,10 @@ struct Simulation {
             rc = mcac_group_add(group, box);
             if (rc) return rc;
             depth = group_depth;  // steps picked ahead per box and group launch
+        } else if (rounds && out_dir.empty() && depth > 1) {
+            rc = mcac_group_create(device, 1, &own_group);
+            if (rc == 0) rc = mcac_group_add(own_group, box);
+            if (rc) return rc;
         }
         std::vector<double> pos3(3 * N), rcm3(3 * N, 0.0);
         std::vector<int64_t> cnt(N, 1), mem(N);
@@ -541,6 +552,67 @@ struct Simulation {
         return 0;
     }
 
+    // Steps picked ahead in rounds: when the next move's cell window is small
+    // enough for one workgroup, up to kRoundDepth moves go into one group
+    // launch (a workgroup runs them in order; those behind a contact do
+    // nothing, as behind the gate); otherwise the 3-deep one-launch pipeline.
+    // A new round is picked once at most kRoundRefill steps remain in flight.
+    static constexpr size_t kRoundDepth = 8, kRoundRefill = 3;
+
+    int issue_more(int64_t consumed, int64_t max_steps) {
+        if (!own_group) {
+            while (!finished() && can_issue(consumed, max_steps)) {
+                int rc = issue();
+                if (rc) return rc;
+            }
+            return 0;
+        }
+        if (flight.size() > kRoundRefill || finished() || !can_issue(consumed, max_steps)) return 0;
+        Pending first = pick_move();
+        const size_t U = static_cast<size_t>(first.num);
+        if (mcac_box_block_steps(box, rmax[U] + maxradius + first.full) != 1) {
+            // one launch per step for this one and up to the pipeline depth
+            auto te0 = Clock::now();
+            int rc = mcac_move_enqueue(box, first.num, first.draw.dir.data(), first.full, maxradius, &first.ticket);
+            t_enqueue += ms_since(te0);
+            if (rc) return rc;
+            flight.push_back(first);
+            while (!finished() && can_issue(consumed, max_steps)) {
+                rc = issue();
+                if (rc) return rc;
+            }
+            return 0;
+        }
+        const size_t saved = depth;
+        depth = kRoundDepth;
+        flight.push_back(first);
+        size_t k = 1;
+        while (!finished() && can_issue(consumed, max_steps)) {
+            flight.push_back(pick_move());
+            ++k;
+        }
+        depth = saved;
+        r_boxes.assign(k, box);
+        r_aggs.resize(k);
+        r_dirs.resize(3 * k);
+        r_lpm.resize(k);
+        r_maxr.assign(k, maxradius);
+        r_tickets.assign(k, 0);
+        for (size_t q = 0; q < k; ++q) {
+            const Pending &P = flight[flight.size() - k + q];
+            r_aggs[q] = P.num;
+            for (int l = 0; l < 3; ++l) r_dirs[3 * q + static_cast<size_t>(l)] = P.draw.dir[static_cast<size_t>(l)];
+            r_lpm[q] = P.full;
+        }
+        auto te0 = Clock::now();
+        int rc = mcac_move_enqueue_group(own_group, r_boxes.data(), r_aggs.data(), r_dirs.data(), r_lpm.data(),
+                                         r_maxr.data(), static_cast<int>(k), r_tickets.data());
+        t_enqueue += ms_since(te0);
+        if (rc) return rc;
+        for (size_t q = 0; q < k; ++q) flight[flight.size() - k + q].ticket = r_tickets[q];
+        return 0;
+    }
+
     // Outputs at the end of a run (calcul.cpp:282-284), once.
     int finalize() {
         if (!finished() || final_written) return 0;
@@ -717,13 +789,11 @@ struct Simulation {
         int64_t n = 0;
         while (n < max_steps) {
             if (flight.empty() && finished()) break;
-            while (!finished() && can_issue(n, max_steps)) {
-                int rc = issue();
-                if (rc) return rc;
-            }
+            int rc = issue_more(n, max_steps);
+            if (rc) return rc;
             if (flight.empty()) break;
             bool stepped = false;
-            int rc = consume(&stepped);
+            rc = consume(&stepped);
             if (rc) return rc;
             if (stepped) ++n;
             // a stop met with steps still in flight (STOPCODE) lets those
@@ -808,6 +878,9 @@ int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mc
     if (const char *e = std::getenv("MCAC_PIPELINE_DEPTH")) {  // steps kept in flight, 1..6
         const long v = std::strtol(e, nullptr, 10);
         h->s.depth = static_cast<size_t>(v < 1 ? 1 : (v > 6 ? 6 : v));
+    }
+    if (const char *e = std::getenv("MCAC_ROUNDS")) {  // 0: always one launch per step
+        h->s.rounds = std::strtol(e, nullptr, 10) != 0;
     }
     *out = h;
     return 0;

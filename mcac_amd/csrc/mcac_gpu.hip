@@ -77,6 +77,7 @@ struct mcac_box {
     int64_t slot_epoch[kRing] = {0};
     int slot_void[kRing] = {0};      // known to be skipped: its predictions were taken out
     int slot_fused[kRing] = {0};
+    int slot_solving[kRing] = {0};   // its k_solve was launched before a cell-list rebuild
     int max_count = 0;               // upper bound of the aggregate sizes
     int64_t n_handover = 0;          // k_step searches finished by k_solve
     int64_t n_fused = 0, n_split = 0;  // searches launched as k_step / k_filter + k_solve
@@ -258,11 +259,47 @@ int compact_pool(mcac_box *b, int64_t need) {
 
 int harvest_timing(mcac_box *b, int slot);
 
+// A step handed over to k_solve finishes (moves its aggregate, drops its cell
+// entry and writes its dirty-list entry at the slot predicted for the current
+// cell list) only when its k_solve runs, which the host launches when it reads
+// the pending record.  A cell-list rebuild must not run in between: before a
+// rebuild with steps in flight, every queued kernel is drained and the
+// k_solve of a pending step (at most one: it closes the gate) is launched now.
+int settle_handovers(mcac_box *b) {
+    if (b->inflight == 0) return 0;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (unsigned long long t = b->oldest; t <= b->seq; ++t) {
+        const int slot = static_cast<int>(t % mcac_box::kRing);
+        if (b->slot_void[slot] || b->slot_solving[slot]) continue;
+        StepRecord &r = b->h_step[slot];
+        if (__atomic_load_n(&r.seq, __ATOMIC_ACQUIRE) != t || r.skipped || r.status != kPendingSolve) continue;
+        __atomic_store_n(&r.seq, 0ull, __ATOMIC_RELEASE);  // the k_solve publishes the final record
+        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, b->slot_args[slot]);
+        HIP_TRY(hipGetLastError());
+        b->slot_solving[slot] = 1;
+        b->n_handover += 1;
+    }
+    return 0;
+}
+
 // A step that one block can take whole: the one-launch form (small mover, few
 // sphere pairs possible), the mover moved by the finishing block, not timed.
 bool one_block_step(const mcac_box *b, int n_mov, bool timed) {
     const bool fused = n_mov <= kStepMover && static_cast<long long>(n_mov) * b->max_count <= kStepPairs;
     return fused && n_mov <= 4 * kBlock && !timed;
+}
+
+// Entries a step may have to sweep (cells of a window for a mover of rmax at
+// most `rmax`, at the mean occupancy, plus the dirty list), and the bound
+// under which one workgroup takes a step in a group launch: with a step's
+// entries one or a few per thread, one block is faster than a grid, whose
+// cross-block ticket and cache fences cost more than the work they split.
+constexpr double kBlockItems = 4096.0;
+double window_items(const mcac_box *b, double reach) {
+    const double cell = b->p.box_length / static_cast<double>(b->p.n_div);
+    const double per_axis = std::min<double>(static_cast<double>(b->p.n_div), std::ceil(2.0 * reach / cell) + 2.0);
+    const double per_cell = b->n_cells > 0 ? static_cast<double>(b->n_built) / b->n_cells : 0.0;
+    return per_cell * per_axis * per_axis * per_axis + static_cast<double>(b->dirty_count) + 1.0;
 }
 
 // The next enqueued step rebuilds the cell list first (kernels on the stream).
@@ -272,10 +309,10 @@ bool next_step_timed(const mcac_box *b) {
     return b->timing > 0 && (b->seq + 1) % static_cast<unsigned long long>(b->timing) == 0;
 }
 
-// Picks the launch of one step and enqueues it.  With job_out, a step that
-// one block can take whole (one-launch form, mover moved by the finishing
-// block, not timed) is written there instead, for a k_step_batch launch by the
-// caller (*batched = true).
+// Picks the launch of one step and enqueues it.  With job_out (given only for
+// a step that one block can take whole: one-launch form, mover moved by the
+// finishing block, not timed, few window entries) the step is written there
+// instead, for a k_step_batch launch by the caller (*batched = true).
 int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double maxradius, int apply, int gated,
                    unsigned long long *ticket, StepJob *job_out = nullptr, bool *batched = nullptr) {
     if (batched) *batched = false;
@@ -283,7 +320,8 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     if (b->inflight >= mcac_box::kRing) return MCAC_E_CAPACITY;
     if (b->inline_pending) return MCAC_E_ARGUMENT;  // the caller merges the contact first
     if (rebuild_due(b)) {
-        int rc = rebuild_cells(b);
+        int rc = settle_handovers(b);
+        if (rc == 0) rc = rebuild_cells(b);
         if (rc) return rc;
     }
     const size_t A = static_cast<size_t>(agg);
@@ -377,7 +415,7 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     // kernel's own execution interval; the queue is drained first so that the
     // interval does not include waiting behind the steps queued before it
     if (timed && b->inflight > 0) HIP_TRY(hipStreamSynchronize(b->stream));
-    if (job_out && one_block_step(b, n_mov, timed)) {
+    if (job_out) {  // the caller checked one_block_step and window_items before the enqueue
         // a batched step's arguments sit in LDS: the host adds the bounds of
         // the visited cells, saving the kernel a dependent load (a launch of
         // its own reads them faster from cstart than from its arguments)
@@ -421,6 +459,7 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     b->slot_dirty_new[slot] = dirty_new;
     b->slot_epoch[slot] = b->dirty_epoch;
     b->slot_void[slot] = doomed ? 1 : 0;
+    b->slot_solving[slot] = 0;
     b->slot_agg[slot] = agg;
     b->slot_translate[slot] = apply && !a.translate_in_finish;
     b->inflight += 1;
@@ -475,8 +514,9 @@ int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contac
     }
     int rc = wait_seq(b, &b->h_step[slot].seq, ticket);
     if (rc) return rc;
-    bool handed_over = false;
-    if (!b->h_step[slot].skipped && b->h_step[slot].status == kPendingSolve) {
+    bool handed_over = b->slot_solving[slot] != 0;  // its k_solve already ran (settle_handovers)
+    b->slot_solving[slot] = 0;
+    if (!handed_over && !b->h_step[slot].skipped && b->h_step[slot].status == kPendingSolve) {
         handed_over = true;
         // k_step left its survivors to the multi-block solve; the gate it
         // closed keeps the queued steps behind it from running meanwhile
@@ -1062,7 +1102,9 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
         mcac_box *b = boxes[i];
         const int64_t agg = aggs[i];
         const bool valid = agg >= 0 && agg < b->n_aggs;
-        const bool batch = valid && one_block_step(b, b->count[static_cast<size_t>(agg)], next_step_timed(b));
+        const double reach = valid ? b->h_rmax[static_cast<size_t>(agg)] + maxradius[i] + lpm[i] : 0.0;
+        const bool batch = valid && one_block_step(b, b->count[static_cast<size_t>(agg)], next_step_timed(b)) &&
+                           window_items(b, reach) <= kBlockItems;
         // a step launched on its own, or the cell-list rebuild that the
         // enqueue launches, runs after every job already listed; a block
         // runs at most kMaxBlockJobs jobs (a box has at most that many steps
@@ -1091,6 +1133,11 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
     // the steps enqueued before a failing one still run: their results are read as usual
     const int frc = group_flush(g, l);
     return rc ? rc : frc;
+}
+
+int mcac_box_block_steps(mcac_box *b, double reach) {
+    if (!b) return MCAC_E_ARGUMENT;
+    return window_items(b, reach) <= kBlockItems ? 1 : 0;
 }
 
 int mcac_group_stats(mcac_group *g, int64_t out[3]) {

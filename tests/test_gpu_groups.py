@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 CFG = pathlib.Path(__file__).parent / "configs"
 
 
-@pytest.mark.parametrize("depth", [1, 6])
+@pytest.mark.parametrize("depth", [1, 6, 8])
 def test_group_runs_match_oracle_runs(depth):
     print(f"case: 8 boxes (one dense, hand-over path), 2 workers x 3 boxes, depth {depth}, staggered arrivals",
           flush=True)

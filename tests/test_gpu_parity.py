@@ -95,15 +95,15 @@ def compare_runs(cfg_text, max_steps, fast_init=True):
         if sim.finished():
             break
     assert n_gpu == n_ora
-    assert sim.finished() == ora.finished()
     a, b = ora.steps(), sim.steps()
     assert len(a) == len(b)
-    for f in a.dtype.names:
+    for f in a.dtype.names:  # the first differing step names the case
         eq = (a[f] == b[f]) | (np.isnan(a[f]) & np.isnan(b[f])) if a[f].dtype.kind == "f" else a[f] == b[f]
         if eq.ndim > 1:
             eq = eq.all(axis=1)
         bad = np.nonzero(~eq)[0]
         assert bad.size == 0, f"step field {f} differs first at step {bad[0]}: {a[bad[0]]} vs {b[bad[0]]}"
+    assert sim.finished() == ora.finished()
     ma, mb = ora.merges(), sim.merges()
     assert len(ma) == len(mb)
     for f in ma.dtype.names:
