@@ -45,10 +45,11 @@ typedef struct mcac_merge_rec {
  * Environment: MCAC_PRECISION (0, 1, 2) selects the box's contact-search
  * precision (mcac_box_params.precision; results are identical);
  * MCAC_PIPELINE_DEPTH (1..6, default 3) the MC steps kept in flight;
- * MCAC_ROUNDS (default 1; 0 off): when the next move's cell window is small
+ * MCAC_ROUNDS (default 0; 1 on): when the next move's cell window is small
  * enough for one workgroup, up to 8 moves are picked ahead and run in one
  * group launch (mcac_move_enqueue_group); depth 1 or an output directory
- * turns this off. */
+ * turns this off.  Off by default: on the N=1e6 box it is no faster than the
+ * one-launch pipeline, and slower where windows hold a few thousand entries. */
 int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mcac_sim **out, char *err,
                     int errlen);
 int mcac_sim_destroy(mcac_sim *sim);

@@ -117,7 +117,7 @@ struct Simulation {
     mcac_box *box = nullptr;
     mcac_group *group = nullptr;  // lock-step form: the box works on the group's stream
     mcac_group *own_group = nullptr;  // one-box group for rounds of steps in one launch
-    bool rounds = true;               // issue rounds through own_group when the steps fit one workgroup
+    bool rounds = false;              // issue rounds through own_group when the steps fit one workgroup
     std::vector<int64_t> r_aggs;      // scratch of issue_round
     std::vector<double> r_dirs, r_lpm, r_maxr;
     std::vector<uint64_t> r_tickets;

@@ -207,6 +207,67 @@ __device__ __forceinline__ unsigned long long filter_phase(const DevBox &b, cons
     const float fx = static_cast<float>(sx), fy = static_cast<float>(sy), fz = static_cast<float>(sz);
     const float fr = static_cast<float>(sr), fdx = static_cast<float>(a.dir[0]), fdy = static_cast<float>(a.dir[1]);
     const float fdz = static_cast<float>(a.dir[2]), flpm = static_cast<float>(a.lpm), fL = static_cast<float>(a.L);
+    if (a.prec == 0 && V <= kMaxVisit && total > gstride) {
+        // FP64 sweep with several entries per thread (a block that takes a
+        // window alone): kSweep entries per pass have their rank searches and
+        // row loads in flight together instead of one load latency per entry
+        constexpr int kSweep = 4;
+        int top = 1;
+        while (2 * top < V) top *= 2;
+        for (int w0 = gtid; w0 < total; w0 += kSweep * gstride) {
+            int rk[kSweep] = {}, idv[kSweep];
+            double xv[kSweep], yv[kSweep], zv[kSweep], rv[kSweep];
+            for (int s = top; s > 0; s >>= 1) {  // largest v with pre[v] <= w, as above
+#pragma unroll
+                for (int u = 0; u < kSweep; ++u) {
+                    const int m = rk[u] + s;
+                    if (m < V && pre[m] <= w0 + u * gstride) rk[u] = m;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kSweep; ++u) {
+                const int w = w0 + u * gstride;
+                idv[u] = a.src;
+                xv[u] = yv[u] = zv[u] = 0.0;
+                rv[u] = -1.0;
+                if (w < total) {
+                    const int pos = cbeg[rk[u]] + (w - pre[rk[u]]);
+                    idv[u] = b.citems[pos];
+                    xv[u] = b.cpx[pos];
+                    yv[u] = b.cpy[pos];
+                    zv[u] = b.cpz[pos];
+                    rv[u] = b.cpr[pos];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kSweep; ++u) {
+                if (rv[u] < 0.0 || idv[u] == a.src) continue;  // past the end, moved since the build, or the mover
+                cnt += 1;
+                const double d = swept_sphere(sx, sy, sz, sr, xv[u], yv[u], zv[u], rv[u], a.dir[0], a.dir[1],
+                                              a.dir[2], a.lpm, a.L);
+                if (d < a.lpm && !skip) {
+                    append_candidate(b, a, idv[u], rk[u], d);
+                    cnt += kAppendOne;
+                }
+            }
+        }
+        for (int i = gtid; i < nd; i += gstride) {  // the dirty list, as below
+            const int id = b.dirty_list[i];
+            const int c = b.dpcell[i];
+            if (c < 0 || id == a.src) continue;
+            int oi = window_offset(c / (n * n), b1i, n), oj = window_offset((c / n) % n, b1j, n),
+                ok = window_offset(c % n, b1k, n);
+            if (oi >= li || oj >= lj || ok >= lk) continue;
+            cnt += 1;
+            const double d = swept_sphere(sx, sy, sz, sr, b.dpx[i], b.dpy[i], b.dpz[i], b.dpr[i], a.dir[0], a.dir[1],
+                                          a.dir[2], a.lpm, a.L);
+            if (d < a.lpm && !skip) {
+                append_candidate(b, a, id, (oi * lj + oj) * lk + ok, d);
+                cnt += kAppendOne;
+            }
+        }
+        return cnt;
+    }
     for (int w = gtid; w < total + nd; w += gstride) {
         int id, rank;
         double x, y, z, r;

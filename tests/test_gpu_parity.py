@@ -131,9 +131,12 @@ def test_lognormal_run_bit_exact():
     compare_runs((CFG / "lognormal_small.ini").read_text(), 10 ** 7)
 
 
-def test_dense_run_bit_exact():
-    """N=3000 at FV=0.05 to 60 aggregates: every step and merge identical (large candidate sets)."""
-    say("case: dense N=3000 FV=0.05 -> 60 aggregates")
+@pytest.mark.parametrize("rounds", ["0", "1"])
+def test_dense_run_bit_exact(rounds, monkeypatch):
+    """N=3000 at FV=0.05 to 60 aggregates: every step and merge identical (large candidate sets),
+    with one launch per step and with rounds of up to eight steps per group launch."""
+    monkeypatch.setenv("MCAC_ROUNDS", rounds)
+    say(f"case: dense N=3000 FV=0.05 -> 60 aggregates, MCAC_ROUNDS={rounds}")
     ora, sim = compare_runs((CFG / "dense_small.ini").read_text(), 10 ** 7)
     say("stats", sim.box().stats())
 
