@@ -68,6 +68,12 @@ typedef struct mcac_box_params {
                                 /*    final solve (results identical to 0); 2: as   */
                                 /*    1, and every entry the FP32 test drops is     */
                                 /*    also tested in FP64 to count misses           */
+    int32_t volsurf;            /* overlap correction of the merge update           */
+                                /* (compute_volume_surface, aggregat.cpp:321-430):  */
+                                /* 0 none (sphere sums); 1 spherical caps           */
+                                /* (SPHERICAL_CAPS, :354-386; no merge inside the   */
+                                /* contact step, mcac_box_set_auto_merge is a no-op) */
+    int32_t reserved;           /* 0                                                */
 } mcac_box_params;
 
 /* Result of a contact search (AggregateContactInfo, tools/contact_info.hpp:73-85). */
@@ -97,6 +103,8 @@ typedef struct mcac_merge_result {
     double com[3];              /* aggregate position (periodic)                    */
     double rcm[3];              /* centre of mass relative to sphere 0              */
     double overlapping, coordination;
+    double sphere_volume;       /* sum of the spheres' own volumes in list order    */
+                                /* (vol_pp = this / n, aggregat.cpp:255-262)        */
 } mcac_merge_result;
 
 int mcac_box_create(const mcac_box_params *params, mcac_box **out);
@@ -139,8 +147,17 @@ int mcac_release(mcac_box *box);
  * spheres and its member segment fits (calcul.cpp:156-160 always merges on
  * contact); mcac_merge then returns that result without a launch.  The
  * caller must call mcac_merge for such a contact before enqueueing further
- * moves.  Off by default; set while no move is queued. */
+ * moves.  Off by default; set while no move is queued.  A box with
+ * spherical caps (volsurf 1) keeps it off. */
 int mcac_box_set_auto_merge(mcac_box *box, int enable);
+/* Host-only check of the spherical-caps walk order (no GPU call): the
+ * iteration order of one sphere's neighbour map (std::unordered_map<size_t,
+ * double>, aggregat.hpp:61-65) after Aggregate::update_distances_and_overlapping
+ * (aggregat.cpp:730-751) cleared it, reserved prev entries (its size after the
+ * previous update) and inserted the m keys (ascending, m <= 32).  out gets the
+ * m keys in that order.  No counterpart in the reference: the reference walks
+ * the map itself; the device kernels walk this order. */
+int mcac_neighbour_order(int prev, const int64_t *keys, int m, int64_t *out);
 
 /* Groups of boxes: one queued move of each of many boxes in a single launch
  * (one workgroup per box).  No counterpart in the reference, which runs one box

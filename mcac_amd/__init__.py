@@ -20,7 +20,7 @@ import pathlib
 import numpy as np
 
 __all__ = [
-    "physics_scalars", "swept_sphere_batch", "LIB_DIR", "load_gpu", "load_sim", "Simulation", "Box", "STEP_DTYPE", "MERGE_DTYPE",
+    "physics_scalars", "swept_sphere_batch", "neighbour_order", "LIB_DIR", "load_gpu", "load_sim", "Simulation", "Box", "STEP_DTYPE", "MERGE_DTYPE",
     "CONTACT_DTYPE", "MERGE_RESULT_DTYPE", "GPU_SYMBOLS", "SIM_SYMBOLS",
 ]
 
@@ -46,6 +46,7 @@ MERGE_RESULT_DTYPE = np.dtype([
     ("keeper", "<i8"), ("removed", "<i8"), ("n", "<i8"), ("merged", "<i4"), ("status", "<i4"),
     ("volume", "<f8"), ("surface", "<f8"), ("rg", "<f8"), ("rmax", "<f8"), ("radius_sum", "<f8"),
     ("com", "<f8", (3,)), ("rcm", "<f8", (3,)), ("overlapping", "<f8"), ("coordination", "<f8"),
+    ("sphere_volume", "<f8"),
 ])
 
 # Every symbol declared in include/mcac_gpu.h and include/mcac_sim.h.
@@ -55,7 +56,7 @@ GPU_SYMBOLS = (
     "mcac_box_download_spheres", "mcac_box_sphere_owners", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
-    "mcac_group_stats", "mcac_box_block_steps",
+    "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
@@ -125,6 +126,7 @@ def load_gpu():
                                               ctypes.POINTER(ctypes.c_uint64)]
         L.mcac_group_stats.argtypes = [_P, _I64P]
         L.mcac_box_block_steps.argtypes = [_P, _D]
+        L.mcac_neighbour_order.argtypes = [_I, _I64P, _I, _I64P]
         L.mcac_version.restype = ctypes.c_char_p
         _gpu = L
     return _gpu
@@ -192,6 +194,15 @@ def physics_scalars(ini_text: str) -> dict:
     return dict(zip(PHYSICS_KEYS, out.tolist()))
 
 
+def neighbour_order(prev: int, keys) -> np.ndarray:
+    """Walk order of one sphere's neighbour map after an update (host-only
+    check, mcac_neighbour_order): keys ascending, prev the map's previous size."""
+    k = np.ascontiguousarray(keys, dtype=np.int64)
+    out = np.zeros_like(k)
+    _check(load_gpu().mcac_neighbour_order(int(prev), _ip(k), len(k), _ip(out)), "mcac_neighbour_order")
+    return out
+
+
 def swept_sphere_batch(p1, r1, p2, r2, direction, dist, box: float, device: int = 0):
     """Device evaluation of distance_to_contact(Sphere, Sphere) for n inputs."""
     p1 = np.ascontiguousarray(p1, np.float64).reshape(-1, 3)
@@ -212,15 +223,17 @@ class Box:
 
     PARAMS = np.dtype([("box_length", "<f8"), ("n_div", "<i8"), ("max_spheres", "<i8"),
                        ("max_aggregates", "<i8"), ("candidate_capacity", "<i8"), ("member_pool", "<i8"),
-                       ("device", "<i4"), ("precision", "<i4")])
+                       ("device", "<i4"), ("precision", "<i4"), ("volsurf", "<i4"), ("reserved", "<i4")])
 
     def __init__(self, box_length: float, n_div: int, max_spheres: int, max_aggregates: int,
-                 device: int = 0, precision: int = 0, candidate_capacity: int = 0, member_pool: int = 0):
+                 device: int = 0, precision: int = 0, candidate_capacity: int = 0, member_pool: int = 0,
+                 volsurf: int = 0):
         L = load_gpu()
         p = np.zeros(1, self.PARAMS)
         p["box_length"], p["n_div"], p["max_spheres"], p["max_aggregates"] = box_length, n_div, max_spheres, max_aggregates
         p["candidate_capacity"], p["member_pool"], p["device"], p["precision"] = (
             candidate_capacity, member_pool, device, precision)
+        p["volsurf"] = volsurf
         h = _P()
         _check(L.mcac_box_create(p.ctypes.data, ctypes.byref(h)), "mcac_box_create")
         self.h = h

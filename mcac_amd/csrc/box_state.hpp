@@ -94,6 +94,8 @@ struct DevBox {
     double *dc;
     int *row_cnt, *row_j;
     double *row_c;
+    double *row_d;             // pair distances of the rows (spherical caps only)
+    double *cvol, *csurf;      // per-position corrected sphere volume and surface (spherical caps only)
     SearchState *ss;
     MergeState *ms;
 };
@@ -141,7 +143,8 @@ struct CellTable {
 struct MergeArgs {
     long long ms, os, magg, oagg;
     int keep, rem, n_k, n_r, new_start, start_k, start_r, n_div;
-    int cap_k, pad;            // keeper's segment capacity after the merge
+    int cap_k;                 // keeper's segment capacity after the merge
+    int volsurf;               // 0: sphere sums (method none); 1: spherical caps
     double L;
     MergeRecord *out;          // pinned host record
     unsigned long long seq;

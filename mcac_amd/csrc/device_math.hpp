@@ -12,6 +12,7 @@ namespace mcac_dev {
 constexpr double kInf = __builtin_huge_val();
 constexpr double kContactEpsilon = 1e-28;      // constants.hpp:127
 constexpr double kCoordinationEpsilon = 1e-10; // constants.hpp:128
+constexpr double kPi = 3.141592653589793;      // atan(1.0) * 4, constants.hpp:113 (the same double)
 // Loops that fold a coordinate into the box stop after this many turns; the
 // inputs of this path stay within a few box lengths, so the cap is never hit
 // for finite data and only guarantees termination on corrupted input.

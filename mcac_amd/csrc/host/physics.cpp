@@ -128,7 +128,10 @@ void PhysicalModel::load(const IniFile &ini) {
     if (flag) unsupported = "with_surface_reactions";
     ini.extract("surface_growth", "flux_surfgrowth", flux_surfgrowth);
     s.clear();
-    if (ini.get("surface_growth", "volsurf_method", s) && !s.empty() && s != "none") unsupported = "volsurf_method=" + s;
+    if (ini.get("surface_growth", "volsurf_method", s) && !s.empty()) {  // constants.hpp:103-111
+        if (s == "caps") volsurf = 1;
+        else if (s != "none") unsupported = "volsurf_method=" + s;
+    }
     flag = false;
     ini.extract("nucleation", "with_nucleation", flag);
     if (flag) unsupported = "with_nucleation";
@@ -268,10 +271,11 @@ bool PhysicalModel::time_to_write(size_t total_events) {
 }
 
 // aggregat.cpp:261-276
-AggregatePhysics aggregate_physics(const PhysicalModel &pm, size_t n, double volume, double radius_sum, double rg) {
+AggregatePhysics aggregate_physics(const PhysicalModel &pm, size_t n, double volume, double sphere_volume,
+                                   double radius_sum, double rg) {
     AggregatePhysics o{};
     o.dp = 2 * radius_sum / static_cast<double>(n);
-    double vol_pp = volume / static_cast<double>(n);  // the list-order volume sum is the aggregate volume
+    double vol_pp = sphere_volume / static_cast<double>(n);  // list-order sum of the spheres' own volumes
     o.f_agg = pm.friction_coeff(volume, vol_pp, 0.5 * o.dp);
     o.d_m = pm.mobility_diameter(volume, vol_pp, 0.5 * o.dp);
     double masse = pm.density * volume;

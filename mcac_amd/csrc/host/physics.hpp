@@ -85,6 +85,7 @@ struct PhysicalModel {
     size_t n_time_per_file = 10;
     bool with_domain_duplication = true, with_collisions = true, enforce_volume_fraction = true;
     std::string output_dir = "MCAC_output";
+    int volsurf = 0;          // volsurf_method: 0 none, 1 caps (spherical caps)
     std::string unsupported;  // first option this path does not implement
 
     void load(const IniFile &ini);
@@ -105,6 +106,7 @@ struct PhysicalModel {
 struct AggregatePhysics {
     double dp, f_agg, d_m, time_step, lpm, dg_over_dp;
 };
-AggregatePhysics aggregate_physics(const PhysicalModel &pm, size_t n, double volume, double radius_sum, double rg);
+AggregatePhysics aggregate_physics(const PhysicalModel &pm, size_t n, double volume, double sphere_volume,
+                                   double radius_sum, double rg);
 
 }  // namespace mcac

@@ -160,7 +160,7 @@ struct Simulation {
         double brg = 0. + v * (r * r);
         double arg = 0. + v * (0.0 * 0.0);
         rg[i] = std::sqrt(std::abs((arg + 3. / 5. * brg) / vol[i]));
-        AggregatePhysics ph = aggregate_physics(pm, 1, vol[i], 0. + r, rg[i]);
+        AggregatePhysics ph = aggregate_physics(pm, 1, vol[i], vol[i], 0. + r, rg[i]);
         dp[i] = ph.dp;
         f_agg[i] = ph.f_agg;
         d_m[i] = ph.d_m;
@@ -308,6 +308,7 @@ struct Simulation {
         bp.max_aggregates = static_cast<int64_t>(N);
         bp.device = device;
         bp.precision = precision;
+        bp.volsurf = pm.volsurf;
         rc = mcac_box_create(&bp, &box);
         if (rc) return rc;
         if (group) {
@@ -709,7 +710,8 @@ struct Simulation {
                 rcm[K] = {mr.rcm[0], mr.rcm[1], mr.rcm[2]};
                 overlapping[K] = mr.overlapping;
                 coord[K] = mr.coordination;
-                AggregatePhysics ph = aggregate_physics(pm, static_cast<size_t>(mr.n), mr.volume, mr.radius_sum, mr.rg);
+                AggregatePhysics ph = aggregate_physics(pm, static_cast<size_t>(mr.n), mr.volume, mr.sphere_volume,
+                                                        mr.radius_sum, mr.rg);
                 dp[K] = ph.dp;
                 f_agg[K] = ph.f_agg;
                 d_m[K] = ph.d_m;

@@ -13,6 +13,7 @@
 // the multi-kernel sequence below.  Both publish the result to pinned memory.
 #pragma once
 #include "kernels_common.hpp"
+#include "volsurf.hpp"
 
 namespace mcac_dev {
 
@@ -303,6 +304,7 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         res.status = (V <= 0 || S <= 0) ? MCAC_VOL_SURF_ERROR : 0;  // aggregat.cpp:427-429
         res.volume = V;
         res.surface = S;
+        res.sphere_volume = V;  // the sphere sum is the volume for method none
         res.radius_sum = sr;
         for (int l = 0; l < 3; ++l) {
             res.com[l] = com[l];
@@ -466,6 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_seq(DevBox b, MergeArgs a) {
     res.status = (V <= 0 || S <= 0) ? MCAC_VOL_SURF_ERROR : 0;
     res.volume = V;
     res.surface = S;
+    res.sphere_volume = V;
     res.radius_sum = sr;
     for (int l = 0; l < 3; ++l) {
         res.com[l] = com[l];
@@ -491,9 +494,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_dc(DevBox b) {
     if ((threadIdx.x & 63) == 0) atomicMax(&m->rmax_bits, static_cast<unsigned long long>(__double_as_longlong(mx)));
 }
 
-__global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b) {
+// With spherical caps the weights are the corrected volumes by list position
+// and this kernel ends the sequence (publishes).
+__global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b, MergeArgs a) {
     MergeState *m = b.ms;
-    if (!m->ok) return;
+    if (!m->ok) return;  // k_merge_prep published the failed re-check
     __shared__ double tv[kBlock], td[kBlock];
     const int k = m->keep, n = m->n_k + m->n_r, st = m->new_start;
     double arg = 0.;
@@ -501,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b) {
         const int cnt = (n - t0 < kBlock) ? n - t0 : kBlock;
         if (threadIdx.x < cnt) {
             int s = b.members[st + t0 + threadIdx.x];
-            tv[threadIdx.x] = b.svol[s];
+            tv[threadIdx.x] = a.volsurf ? b.cvol[t0 + threadIdx.x] : b.svol[s];
             td[threadIdx.x] = b.dc[t0 + threadIdx.x];
         }
         __syncthreads();
@@ -517,10 +522,13 @@ __global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b) {
     b.arg[k] = rg;
     m->res.rmax = rmax;
     m->res.rg = rg;
+    if (a.volsurf) publish_merge(a, m->res);
 }
 
 // Pair pass of the merged aggregate in kBlock x kBlock tiles, j tile in LDS.
-__global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b) {
+// With both (spherical caps) a contact goes into both spheres' rows, with
+// its distance, so that each row holds the sphere's whole neighbour map.
+__global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b, int both) {
     const MergeState *m = b.ms;
     if (!m->ok) return;
     const int n = m->n_k + m->n_r, st = m->new_start;
@@ -555,14 +563,27 @@ __global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b) {
             if (slot < kOverlapSlots) {
                 b.row_j[static_cast<long long>(i) * kOverlapSlots + slot] = j;
                 b.row_c[static_cast<long long>(i) * kOverlapSlots + slot] = c;
+                if (both) b.row_d[static_cast<long long>(i) * kOverlapSlots + slot] = dist;
             } else {
                 atomicOr(&b.ms->row_overflow, 1u);
+            }
+            if (both) {
+                slot = atomicAdd(&b.row_cnt[j], 1);
+                if (slot < kOverlapSlots) {
+                    b.row_j[static_cast<long long>(j) * kOverlapSlots + slot] = i;
+                    b.row_c[static_cast<long long>(j) * kOverlapSlots + slot] = c;
+                    b.row_d[static_cast<long long>(j) * kOverlapSlots + slot] = dist;
+                } else {
+                    atomicOr(&b.ms->row_overflow, 1u);
+                }
             }
         }
     }
 }
 
 // Ordered overlap sum in (i, j) order; clears the row counters; publishes.
+// With spherical caps the rows hold both directions (pairs with j > i are
+// summed) and stay for k_caps_volumes, which clears them; no publication.
 __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
     MergeState *m = b.ms;
     if (!m->ok) return;
@@ -580,16 +601,19 @@ __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
             for (int x = 1; x < rc; ++x) {  // insertion sort of the row by j
                 int kj = b.row_j[base + x];
                 double kc = b.row_c[base + x];
+                double kd = a.volsurf ? b.row_d[base + x] : 0.0;
                 int y = x - 1;
                 while (y >= 0 && b.row_j[base + y] > kj) {
                     b.row_j[base + y + 1] = b.row_j[base + y];
                     b.row_c[base + y + 1] = b.row_c[base + y];
+                    if (a.volsurf) b.row_d[base + y + 1] = b.row_d[base + y];
                     --y;
                 }
                 b.row_j[base + y + 1] = kj;
                 b.row_c[base + y + 1] = kc;
+                if (a.volsurf) b.row_d[base + y + 1] = kd;
             }
-            b.row_cnt[i] = 0;
+            if (!a.volsurf) b.row_cnt[i] = 0;
             tcnt[threadIdx.x] = rc;
         }
         __syncthreads();
@@ -597,6 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
             for (int x = 0; x < cnt; ++x) {
                 const long long base = static_cast<long long>(t0 + x) * kOverlapSlots;
                 for (int y = 0; y < tcnt[x]; ++y) {
+                    if (b.row_j[base + y] <= t0 + x) continue;  // the lower half of a two-way row
                     overlapping += 2.0 * b.row_c[base + y];
                     inter += 2;
                 }
@@ -613,7 +638,119 @@ __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
     m->res.overlapping = overlapping;
     m->res.coordination = coord;
     if (m->row_overflow) m->res.status = MCAC_E_CAPACITY;
-    publish_merge(a, m->res);
+    if (!a.volsurf) publish_merge(a, m->res);
+}
+
+// ------------------------------------------------------------------ spherical caps (aggregat.cpp:354-386)
+// Corrected volume and surface of every sphere of the merged aggregate, one
+// thread per list position t, from its two-way row (neighbours ascending,
+// with distances): the caps of each lower neighbour in ascending order, then
+// those of each higher neighbour in the order of the sphere's neighbour map,
+// then the clamp at zero.  The map's previous size is the count of the
+// sphere's neighbours inside the keeper for a keeper sphere (the keeper's last
+// update saw the same rigid geometry) and 0 for a sphere of the removed
+// aggregate (a new map).  Clears the row counters.
+__global__ __launch_bounds__(kBlock) void k_caps_volumes(DevBox b, MergeArgs a) {
+    const MergeState *m = b.ms;
+    if (!m->ok) return;
+    const int n = m->n_k + m->n_r, st = m->new_start, nk = m->n_k;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int rc = b.row_cnt[t];
+        if (rc > kOverlapSlots) rc = kOverlapSlots;
+        const long long base = static_cast<long long>(t) * kOverlapSlots;
+        int64_t keys[kOverlapSlots], order[kOverlapSlots];
+        int prev = 0;
+        for (int x = 0; x < rc; ++x) {
+            keys[x] = b.row_j[base + x];
+            if (t < nk && keys[x] < nk) ++prev;
+        }
+        neighbour_order(prev, keys, rc, order);
+        const int s = b.members[st + t];
+        const double rt = b.sr[s], vt = b.svol[s], at = b.ssurf[s];
+        double vol = vt, surf = at;
+        for (int x = 0; x < rc && keys[x] < t; ++x) {  // lower neighbours: this sphere is the second
+            const int sj = b.members[st + keys[x]];
+            double v1, s1, v2, s2;
+            caps_intersection(b.sr[sj], b.svol[sj], b.ssurf[sj], rt, vt, at, b.row_d[base + x], &v1, &s1, &v2, &s2);
+            vol = vol - v2;
+            surf = surf - s2;
+        }
+        for (int y = 0; y < rc; ++y) {  // higher neighbours in map order: this sphere is the first
+            const int64_t j = order[y];
+            if (j <= t) continue;
+            int x = 0;
+            while (keys[x] != j) ++x;
+            const int sj = b.members[st + static_cast<int>(j)];
+            double v1, s1, v2, s2;
+            caps_intersection(rt, vt, at, b.sr[sj], b.svol[sj], b.ssurf[sj], b.row_d[base + x], &v1, &s1, &v2, &s2);
+            vol = vol - v1;
+            surf = surf - s1;
+        }
+        b.cvol[t] = smax(vol, 0.0);
+        b.csurf[t] = smax(surf, 0.0);
+        b.row_cnt[t] = 0;
+    }
+}
+
+// Ordered sums of the merged aggregate with the corrected volumes, all
+// positions from the start (aggregat.cpp:387-390, 437-441, 257-260, 474-478):
+// one block stages tiles in LDS, thread 0 accumulates in list order.
+__global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) {
+    MergeState *m = b.ms;
+    if (!m->ok) return;
+    __shared__ double tv[kBlock], ts[kBlock], tx[kBlock], ty[kBlock], tz[kBlock], tr[kBlock], tw[kBlock];
+    const int k = m->keep, n = m->n_k + m->n_r, st = m->new_start;
+    double V = 0.0, S = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, sr = 0.0, svr2 = 0.0, raw = 0.0;
+    for (int t0 = 0; t0 < n; t0 += kBlock) {
+        const int cnt = (n - t0 < kBlock) ? n - t0 : kBlock;
+        if (threadIdx.x < cnt) {
+            const int t = t0 + threadIdx.x;
+            const int s = b.members[st + t];
+            tv[threadIdx.x] = b.cvol[t];
+            ts[threadIdx.x] = b.csurf[t];
+            tw[threadIdx.x] = b.svol[s];
+            tx[threadIdx.x] = b.srx[s];
+            ty[threadIdx.x] = b.sry[s];
+            tz[threadIdx.x] = b.srz[s];
+            tr[threadIdx.x] = b.sr[s];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < cnt; ++i) {
+                const double v = tv[i];
+                V = V + v;
+                S = S + ts[i];
+                vx += tx[i] * v;
+                vy += ty[i] * v;
+                vz += tz[i] * v;
+                sr += tr[i];
+                svr2 = svr2 + v * (tr[i] * tr[i]);
+                raw += tw[i];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double r[3] = {vx / V, vy / V, vz / V};
+    const int s0 = b.members[st];
+    double com[3] = {periodic_position(b.sx[s0] + r[0], a.L), periodic_position(b.sy[s0] + r[1], a.L),
+                     periodic_position(b.sz[s0] + r[2], a.L)};
+    store_merged(b, a, V, S, vx, vy, vz, sr, svr2, r, com);
+    for (int l = 0; l < 3; ++l) m->r[l] = r[l];
+    mcac_merge_result &res = m->res;
+    res.keeper = k;
+    res.removed = a.rem;
+    res.n = n;
+    res.merged = 1;
+    res.status = (V <= 0 || S <= 0) ? MCAC_VOL_SURF_ERROR : (m->row_overflow ? MCAC_E_CAPACITY : 0);
+    res.volume = V;
+    res.surface = S;
+    res.sphere_volume = raw;
+    res.radius_sum = sr;
+    for (int l = 0; l < 3; ++l) {
+        res.com[l] = com[l];
+        res.rcm[l] = r[l];
+    }
 }
 
 __global__ void k_gather_members(DevBox b, int agg, int *out, int n) {

@@ -626,7 +626,8 @@ const char *mcac_version(void) { return "mcac_gpu 0.2 (gfx950, FP64 collision pa
 int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (!params || !out) return MCAC_E_ARGUMENT;
     if (params->box_length <= 0 || params->n_div < 1 || params->n_div > 1290 || params->max_spheres < 1 ||
-        params->max_aggregates < 1 || params->max_spheres > INT32_MAX / 2 || params->max_aggregates > INT32_MAX / 2)
+        params->max_aggregates < 1 || params->max_spheres > INT32_MAX / 2 || params->max_aggregates > INT32_MAX / 2 ||
+        params->volsurf < 0 || params->volsurf > 1)
         return MCAC_E_ARGUMENT;
     auto *b = new (std::nothrow) mcac_box();
     if (!b) return MCAC_E_CAPACITY;
@@ -666,6 +667,9 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.part_d, b->part_cap); ALLOC(d.part_p, b->part_cap);
     ALLOC(d.dc, Ns);
     ALLOC(d.row_cnt, Ns); ALLOC(d.row_j, Ns * kOverlapSlots); ALLOC(d.row_c, Ns * kOverlapSlots);
+    if (params->volsurf == 1) {
+        ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.cvol, Ns); ALLOC(d.csurf, Ns);
+    }
     ALLOC(d.ss, 1); ALLOC(d.ms, 1);
 #undef ALLOC
     if (b->n_cells + 1 <= (1 << 20) &&
@@ -955,9 +959,23 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     a.L = b->p.box_length;
     a.out = b->d_merge;
     a.seq = ++b->merge_seq;
+    a.volsurf = b->p.volsurf;
     const int n = nk + nr;
     if (b->timing) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    if (n <= kSmallMerge) {
+    if (a.volsurf == 1) {
+        // spherical caps: the two-way contact rows first, then the corrected
+        // sphere volumes, the full ordered sums, dc, rmax and Rg (publishes)
+        const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
+        const unsigned tiles = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
+        hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_caps_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+        hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+    } else if (n <= kSmallMerge) {
         hipLaunchKernelGGL(k_merge_small, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
     } else {
         const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
@@ -965,9 +983,9 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
         hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
         hipLaunchKernelGGL(k_merge_seq, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
         hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d);
+        hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
         const unsigned tiles = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d);
+        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 0);
         hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
     }
     if (b->timing) HIP_TRY(hipEventRecord(b->ev[1], b->stream));
@@ -1163,8 +1181,15 @@ int mcac_release(mcac_box *b) {
 
 int mcac_box_set_auto_merge(mcac_box *b, int enable) {
     if (!b || b->inflight) return MCAC_E_ARGUMENT;
-    b->automerge = enable ? 1 : 0;
+    b->automerge = (enable && b->p.volsurf == 0) ? 1 : 0;  // caps merges run their own kernel sequence
     return 0;
+}
+
+int mcac_neighbour_order(int prev, const int64_t *keys, int m, int64_t *out) {
+    if (!keys || !out || prev < 0 || m < 0 || m > kMapKeys) return MCAC_E_ARGUMENT;
+    for (int c = 1; c < m; ++c)
+        if (keys[c] <= keys[c - 1]) return MCAC_E_ARGUMENT;
+    return neighbour_order(prev, keys, m, out) == 0 ? 0 : MCAC_E_ARGUMENT;
 }
 
 int mcac_box_download_spheres(mcac_box *b, double *x, double *y, double *z, double *rx, double *ry, double *rz) {

@@ -269,7 +269,7 @@ struct Physics {
     size_t n_verlet_divisions = 10;
     size_t n_monomeres = 2500;
     int init_mode = LOGNORMAL;
-    int volsurf_none = 1;
+    int volsurf = 0;  // volsurf_method: 0 none, 1 caps (constants.hpp:103-111)
     int pick_random = 1;
     size_t n_iter_without_event = 0;
     double physical_time_limit = -1;
@@ -302,7 +302,8 @@ struct Physics {
         extract(ini, "surface_growth", "flux_surfgrowth", flux_surfgrowth);
         s.clear();
         extract(ini, "surface_growth", "volsurf_method", s);
-        if (!s.empty() && s != "none") unsupported = "volsurf_method=" + s;
+        if (s == "caps") volsurf = 1;
+        else if (!s.empty() && s != "none") unsupported = "volsurf_method=" + s;
         extract(ini, "nucleation", "with_nucleation", with_nucleation);
         extract(ini, "limits", "number_of_aggregates", number_of_aggregates_limit);
         extract(ini, "limits", "n_iter_without_event", n_iter_without_event_limit);
@@ -407,6 +408,33 @@ struct Physics {
     }
 };
 
+// sphere_intersection.cpp:29-66 (spherical caps, Morán et al. 2018)
+struct Intersection {
+    double volume_1 = 0., volume_2 = 0., surface_1 = 0., surface_2 = 0.;
+};
+Intersection sphere_intersection(double radius_1, double vol_1, double surf_1, double radius_2, double vol_2,
+                                 double surf_2, double dist) {
+    Intersection x;
+    if (dist <= 0) return x;
+    if (dist < radius_1 + radius_2) {
+        if (dist >= std::fdim(radius_1, radius_2)) {
+            double h_1 = (std::pow(radius_2, 2) - std::pow((radius_1 - dist), 2)) / (2. * dist);
+            double h_2 = (std::pow(radius_1, 2) - std::pow((radius_2 - dist), 2)) / (2. * dist);
+            x.volume_1 = kPi * std::pow(h_1, 2) * (3 * radius_1 - h_1) / 3.;
+            x.volume_2 = kPi * std::pow(h_2, 2) * (3 * radius_2 - h_2) / 3.;
+            x.surface_1 = 2 * kPi * radius_1 * h_1;
+            x.surface_2 = 2 * kPi * radius_2 * h_2;
+        } else if (radius_1 < radius_2) {
+            x.volume_1 = vol_1;
+            x.surface_1 = surf_1;
+        } else {
+            x.volume_2 = vol_2;
+            x.surface_2 = surf_2;
+        }
+    }
+    return x;
+}
+
 struct Agg {
     double rg = 0, f_agg = 0, lpm = 0, time_step = 0, rmax = 0, volume = 0, surface = 0;
     V3 pos{0, 0, 0}, rcm{0, 0, 0};
@@ -414,6 +442,8 @@ struct Agg {
     double bulk_density = 0;
     std::vector<size_t> sph;           // myspheres, list order
     std::vector<double> dcenter;       // distances_center
+    std::vector<std::unordered_map<size_t, double>> distances;  // aggregat.hpp:61-65 (contacts only)
+    std::vector<double> volumes, surfaces;                       // aggregat.hpp:67-68
     std::array<size_t, 3> vidx{0, 0, 0};
     int64_t uid = 0;
 };
@@ -509,18 +539,28 @@ struct Sim {
     }
 
     // ------------------------------------------------------------ aggregate update
-    // aggregat.cpp:719-764 (volsurf none: the contact map itself is unused)
+    // aggregat.cpp:719-764, with the contact map kept as the reference keeps it
+    // (a std::unordered_map per sphere, cleared and re-reserved at its old size)
+    // because spherical caps walk it in its iteration order
     void update_distances_and_overlapping(Agg &a) {
         a.overlapping = a.coordination = 0.0;
         int intersections = 0;
         const size_t n = a.sph.size();
+        a.distances.resize(n);
         a.dcenter.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            size_t old_size = a.distances[i].size();
+            a.distances[i].clear();
+            a.distances[i].reserve(old_size);
+        }
         for (size_t i = 0; i < n; i++) {
             size_t si = a.sph[i];
             for (size_t j = i + 1; j < n; j++) {
                 size_t sj = a.sph[j];
                 double dist = std::sqrt(relative_distance_2(srel(si), srel(sj)));
                 if (dist <= (1. + kCoordinationEpsilon) * (sr[si] + sr[sj])) {
+                    a.distances[i][j] = dist;
+                    a.distances[j][i] = dist;
                     double c_ij = (sr[si] + sr[sj] - dist) / (sr[si] + sr[sj]);
                     a.overlapping += 2.0 * c_ij;
                     intersections += 2;
@@ -532,23 +572,47 @@ struct Sim {
             a.coordination = static_cast<double>(intersections) / static_cast<double>(n);
         }
     }
-    // aggregat.cpp:321-430 (method none)
+    Intersection intersection(size_t s1, size_t s2, double dist) const {
+        return sphere_intersection(sr[s1], svol[s1], ssurf[s1], sr[s2], svol[s2], ssurf[s2], dist);
+    }
+    // aggregat.cpp:321-430 (methods none and caps)
     void compute_volume_surface(Agg &a) {
         a.volume = a.surface = 0.0;
-        for (size_t s : a.sph) {
-            a.volume = a.volume + svol[s];
-            a.surface = a.surface + ssurf[s];
+        const size_t n = a.sph.size();
+        a.volumes.resize(n);
+        a.surfaces.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            a.volumes[i] = svol[a.sph[i]];
+            a.surfaces[i] = ssurf[a.sph[i]];
+        }
+        if (pm.volsurf == 1) {  // :354-386, the map walked in its own iteration order
+            for (size_t i = 0; i < n; i++) {
+                for (const auto &[j, dist] : a.distances[i]) {
+                    if (j <= i) continue;
+                    Intersection x = intersection(a.sph[i], a.sph[j], dist);
+                    a.volumes[i] = a.volumes[i] - x.volume_1;
+                    a.surfaces[i] = a.surfaces[i] - x.surface_1;
+                    a.volumes[j] = a.volumes[j] - x.volume_2;
+                    a.surfaces[j] = a.surfaces[j] - x.surface_2;
+                }
+                a.volumes[i] = std::max(a.volumes[i], 0.0);
+                a.surfaces[i] = std::max(a.surfaces[i], 0.0);
+            }
+        }
+        for (size_t i = 0; i < n; i++) {
+            a.volume = a.volume + a.volumes[i];
+            a.surface = a.surface + a.surfaces[i];
         }
         if (a.volume <= 0 || a.surface <= 0) throw std::runtime_error("VolSurfError");
     }
     // aggregat.cpp:247-282 (+ 431-483)
-    void update_partial(size_t label, bool in_verlet) {
-        Agg &a = aggs[label];
+    // compute_mass_center / max_radius / giration_radius (aggregat.cpp:431-483)
+    // without the position update; returns the centre of mass relative to sphere 0
+    V3 geometry(Agg &a) {
         const size_t n = a.sph.size();
-        // compute_mass_center :431-452
         V3 r{0., 0., 0.};
         for (size_t i = 0; i < n; i++) {
-            V3 t = srel(a.sph[i]) * svol[a.sph[i]];
+            V3 t = srel(a.sph[i]) * a.volumes[i];
             r[0] += t[0];
             r[1] += t[1];
             r[2] += t[2];
@@ -561,7 +625,6 @@ struct Sim {
             V3 diff = srel(a.sph[i]) - r;
             a.dcenter[i] = std::sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
         }
-        set_position(label, spos(a.sph[0]) + r, in_verlet);
         a.rcm = r;
         // compute_max_radius :453-464
         a.rmax = 0.0;
@@ -569,11 +632,18 @@ struct Sim {
         // compute_giration_radius :465-483
         double arg = 0., brg = 0.;
         for (size_t i = 0; i < n; i++) {
-            double v = svol[a.sph[i]];
+            double v = a.volumes[i];
             arg = arg + v * (a.dcenter[i] * a.dcenter[i]);
             brg = brg + v * (sr[a.sph[i]] * sr[a.sph[i]]);
         }
         a.rg = std::sqrt(std::abs((arg + 3. / 5. * brg) / a.volume));
+        return r;
+    }
+    void update_partial(size_t label, bool in_verlet) {
+        Agg &a = aggs[label];
+        const size_t n = a.sph.size();
+        V3 r = geometry(a);
+        set_position(label, spos(a.sph[0]) + r, in_verlet);  // :448
         // :254-276
         a.dp = 0.;
         double vol_pp = 0.0;
@@ -1172,6 +1242,57 @@ int64_t ora_verlet_query(int64_t n_div, double width, const int64_t *cells, int6
     int64_t n = std::min<int64_t>(max, static_cast<int64_t>(v.size()));
     for (int64_t i = 0; i < n; i++) out[i] = static_cast<int64_t>(v[static_cast<size_t>(i)]);
     return static_cast<int64_t>(v.size());
+}
+
+void ora_caps_intersection(double r1, double vol1, double surf1, double r2, double vol2, double surf2, double dist,
+                           double out[4]) {
+    Intersection x = sphere_intersection(r1, vol1, surf1, r2, vol2, surf2, dist);
+    out[0] = x.volume_1; out[1] = x.surface_1; out[2] = x.volume_2; out[3] = x.surface_2;
+}
+int ora_merge_geometry(int volsurf, int64_t nk, int64_t n, const double *rx, const double *ry, const double *rz,
+                       const double *r, const double *vol, const double *surf, double out[11]) {
+    if (nk < 1 || n < nk) return -1;
+    Sim s;
+    s.pm.volsurf = volsurf;
+    const size_t N = static_cast<size_t>(n);
+    s.srx.assign(rx, rx + N); s.sry.assign(ry, ry + N); s.srz.assign(rz, rz + N);
+    s.sx = s.srx; s.sy = s.sry; s.sz = s.srz;
+    s.sr.assign(r, r + N); s.svol.assign(vol, vol + N); s.ssurf.assign(surf, surf + N);
+    s.slabel.assign(N, 0);
+    Agg a;
+    try {
+        // the keeper's previous update on its own spheres, then the merged update
+        for (int64_t q = 0; q < nk; q++) a.sph.push_back(static_cast<size_t>(q));
+        s.update_distances_and_overlapping(a);
+        s.compute_volume_surface(a);
+        for (int64_t q = nk; q < n; q++) a.sph.push_back(static_cast<size_t>(q));
+        s.update_distances_and_overlapping(a);
+        s.compute_volume_surface(a);
+    } catch (const std::exception &) {
+        return -2;
+    }
+    V3 c = s.geometry(a);
+    double raw = 0.0, rs = 0.0;
+    for (size_t q = 0; q < N; q++) {
+        raw += s.svol[q];
+        rs += s.sr[q];
+    }
+    const double v[11] = {a.volume, a.surface, a.rg, a.rmax, c[0], c[1], c[2], a.overlapping, a.coordination, raw, rs};
+    for (int q = 0; q < 11; q++) out[q] = v[q];
+    return 0;
+}
+int ora_neighbour_order(int prev, const int64_t *keys, int m, int64_t *out) {
+    // aggregat.cpp:730-751 on one map: the previous update's prev entries,
+    // then clear(), reserve(old size) and the new keys in ascending order
+    std::unordered_map<size_t, double> mp;
+    for (int q = 0; q < prev; q++) mp[static_cast<size_t>(7 * q + 3)] = 0.0;
+    size_t old_size = mp.size();
+    mp.clear();
+    mp.reserve(old_size);
+    for (int c = 0; c < m; c++) mp[static_cast<size_t>(keys[c])] = 1.0;
+    int c = 0;
+    for (const auto &kv : mp) out[c++] = static_cast<int64_t>(kv.first);
+    return c;
 }
 
 }  // extern "C"

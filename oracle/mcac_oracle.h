@@ -121,6 +121,22 @@ int64_t ora_verlet_query(int64_t n_div, double width, const int64_t *cells, int6
                          const double src[3], const double vec[3], double dist,
                          int64_t *out, int64_t max);
 
+/* Spherical caps of two spheres (sphere_intersection.cpp:29-66):
+ * out = {volume_1, surface_1, volume_2, surface_2}. */
+void ora_caps_intersection(double r1, double vol1, double surf1, double r2, double vol2, double surf2, double dist,
+                           double out[4]);
+/* Geometry of Aggregate::update (aggregat.cpp:284-288, 431-483) for n spheres
+ * given by their positions relative to sphere 0, of which the first nk were
+ * last updated on their own (the keeper of a merge): volsurf 0 none, 1 caps.
+ * out = {volume, surface, rg, rmax, rcm x, y, z, overlapping, coordination,
+ * sum of sphere volumes, sum of radii}; returns 0, or < 0 (bad sizes, VolSurfError). */
+int ora_merge_geometry(int volsurf, int64_t nk, int64_t n, const double *rx, const double *ry, const double *rz,
+                       const double *r, const double *vol, const double *surf, double out[11]);
+/* Iteration order of a std::unordered_map<size_t,double> that held prev
+ * entries, was cleared, reserved at its old size and given keys[0..m) in
+ * order (aggregat.cpp:730-751); returns the number of keys written. */
+int ora_neighbour_order(int prev, const int64_t *keys, int m, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

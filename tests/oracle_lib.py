@@ -79,6 +79,10 @@ def lib():
         L.ora_physics.argtypes = [ctypes.c_char_p, _DP]
         L.ora_verlet_query.restype = _I64
         L.ora_verlet_query.argtypes = [_I64, _D, _I64P, _I64, _DP, _DP, _D, _I64P, _I64]
+        L.ora_caps_intersection.argtypes = [_D] * 7 + [_DP]
+        L.ora_caps_intersection.restype = None
+        L.ora_merge_geometry.argtypes = [ctypes.c_int, _I64, _I64] + [_DP] * 6 + [_DP]
+        L.ora_neighbour_order.argtypes = [ctypes.c_int, _I64P, ctypes.c_int, _I64P]
         _lib = L
     return _lib
 
@@ -176,6 +180,36 @@ def swept_sphere(p1, r1, p2, r2, direction, dist, box):
 
 def contact(p1, r1, p2, r2, box):
     return bool(lib().ora_contact(dp(v3(p1)), r1, dp(v3(p2)), r2, box))
+
+
+def caps_intersection(r1, vol1, surf1, r2, vol2, surf2, dist):
+    """(volume_1, surface_1, volume_2, surface_2) of sphere_intersection.cpp:29-66."""
+    out = np.zeros(4)
+    lib().ora_caps_intersection(r1, vol1, surf1, r2, vol2, surf2, dist, dp(out))
+    return tuple(out.tolist())
+
+
+def merge_geometry(volsurf, nk, rel, r, vol, surf):
+    """Aggregate::update geometry of the merged list (first nk spheres: the keeper).
+    Returns dict(volume, surface, rg, rmax, rcm, overlapping, coordination, sphere_volume, radius_sum)."""
+    rel = np.ascontiguousarray(rel, dtype=np.float64)
+    cols = [np.ascontiguousarray(rel[:, k]) for k in range(3)]
+    r, vol, surf = (np.ascontiguousarray(a, dtype=np.float64) for a in (r, vol, surf))
+    out = np.zeros(11)
+    rc = lib().ora_merge_geometry(int(volsurf), int(nk), len(r), *(dp(c) for c in cols), dp(r), dp(vol), dp(surf), dp(out))
+    if rc != 0:
+        raise RuntimeError(f"ora_merge_geometry: {rc}")
+    return dict(volume=out[0], surface=out[1], rg=out[2], rmax=out[3], rcm=out[4:7].copy(), overlapping=out[7],
+                coordination=out[8], sphere_volume=out[9], radius_sum=out[10])
+
+
+def neighbour_order(prev, keys):
+    """Iteration order of a real std::unordered_map after clear/reserve(prev)/ascending inserts."""
+    k = np.ascontiguousarray(keys, dtype=np.int64)
+    out = np.zeros_like(k)
+    n = lib().ora_neighbour_order(int(prev), ip(k), len(k), ip(out))
+    assert n == len(k)
+    return out
 
 
 class OracleRng:

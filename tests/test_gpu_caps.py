@@ -1,0 +1,120 @@
+"""Spherical caps (volsurf_method = caps) on the device, bit for bit against
+the oracle: merges of overlapping clusters (several caps per sphere, so the
+neighbour-map walk order matters) and a full run."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from test_gpu_parity import CFG, compare_runs, mcac, say
+from test_volsurf_caps import sphere_vs
+
+pytestmark = pytest.mark.gpu
+
+MAX_NEIGHBOURS = 12  # clusters are redrawn above this (the device keeps 16 per sphere)
+
+
+def neighbour_counts(rel, r):
+    d = np.sqrt(((rel[:, None, :] - rel[None, :, :]) ** 2).sum(-1))
+    touch = d <= (1 + 1e-9) * (r[:, None] + r[None, :])
+    np.fill_diagonal(touch, False)
+    return touch.sum(1)
+
+
+def cluster(rng, n, r_mean=12e-9):
+    """n overlapping spheres grown one at a time onto one of the last few, a
+    placement kept only when no sphere ends with more than MAX_NEIGHBOURS
+    neighbours (bounded tries; chains of several overlaps per sphere)."""
+    r = rng.normal(r_mean, 0.2 * r_mean, n).clip(0.4 * r_mean, 2 * r_mean)
+    rel = np.zeros((n, 3))
+    cnt = np.zeros(n, np.int64)
+    for i in range(1, n):
+        for _ in range(1000):
+            p = int(rng.integers(max(0, i - 4), i))
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            x = rel[p] + u * (r[i] + r[p]) * (1 - rng.uniform(0.05, 0.3))
+            touch = np.sqrt(((rel[:i] - x) ** 2).sum(1)) <= (1 + 1e-9) * (r[:i] + r[i])
+            if touch.sum() <= 6 and np.all(cnt[:i][touch] < MAX_NEIGHBOURS):
+                break
+        else:
+            raise RuntimeError("no placement found")
+        rel[i] = x
+        cnt[:i][touch] += 1
+        cnt[i] = touch.sum()
+    assert n == 1 or neighbour_counts(rel, r).max() <= MAX_NEIGHBOURS
+    return rel, r
+
+
+def attach(rng, anchor_pos, anchor_r, r0):
+    """Origin of a cluster whose sphere 0 (radius r0) overlaps a sphere at anchor_pos by 20 %."""
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    return anchor_pos + u * (anchor_r + r0) * 0.8
+
+
+def merge_case(seed, sizes, L=4e-6):
+    rng = np.random.default_rng(seed)
+    rels, rads, origins = [], [], []
+    centre = np.full(3, L / 2)
+    for q, n in enumerate(sizes):
+        rel, r = cluster(rng, n)
+        origin = centre
+        for _ in range(q and 1000):  # sphere 0 of cluster q overlaps the last sphere of cluster q - 1
+            origin = attach(rng, origins[-1] + rels[-1][-1], rads[-1][-1], r[0])
+            pts = np.concatenate([o + x for o, x in zip(origins, rels)] + [origin + rel])
+            if neighbour_counts(pts, np.concatenate(rads + [r])).max() <= MAX_NEIGHBOURS:
+                break
+        else:
+            assert q == 0, "no placement found"
+        rels.append(rel)
+        rads.append(r)
+        origins.append(origin)
+    r = np.concatenate(rads)
+    vol, surf = sphere_vs(r)
+    rel_all = np.concatenate(rels)
+    abs_all = np.concatenate([o + rel for o, rel in zip(origins, rels)])
+    count = np.array(sizes, np.int64)
+    first = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    aggs = dict(pos=np.array([origins[q] + rels[q].mean(0) for q in range(len(sizes))]),
+                rcm=np.array([rels[q].mean(0) for q in range(len(sizes))]),
+                rmax=np.array([np.max(np.linalg.norm(rels[q] - rels[q].mean(0), axis=1) + rads[q]) for q in range(len(sizes))]),
+                volume=np.array([vol[first[q]:first[q] + sizes[q]].sum() for q in range(len(sizes))]),
+                surface=np.array([surf[first[q]:first[q] + sizes[q]].sum() for q in range(len(sizes))]),
+                count=count, members=np.arange(r.size, dtype=np.int64))
+    spheres = dict(x=abs_all[:, 0], y=abs_all[:, 1], z=abs_all[:, 2], r=r, vol=vol, surf=surf,
+                   rx=rel_all[:, 0], ry=rel_all[:, 1], rz=rel_all[:, 2])
+    return spheres, aggs, first
+
+
+@pytest.mark.parametrize("seed,sizes", [(1, (12, 9, 7)), (2, (40, 30, 300)), (3, (1, 1, 25)), (4, (150, 200, 60))])
+def test_caps_merges_match_oracle(seed, sizes):
+    say(f"case: caps merges seed={seed} sizes={sizes}")
+    M = mcac()
+    spheres, aggs, first = merge_case(seed, sizes)
+    box = M.Box(4e-6, 10, spheres["r"].size, len(sizes), volsurf=1)
+    box.upload(spheres, aggs)
+    nk = sizes[0]
+    for q in range(1, len(sizes)):
+        ms = int(first[q])                        # sphere 0 of cluster q (moving)
+        os_ = int(first[q - 1] + sizes[q - 1] - 1)  # last sphere of cluster q - 1, now in aggregate 0
+        c = np.zeros(1, M.CONTACT_DTYPE)[0]
+        c["distance"], c["move"], c["contact"] = 0.0, 0.0, 1
+        c["moving_sphere"], c["other_sphere"], c["moving_agg"], c["other_agg"] = ms, os_, q, 0
+        res = box.merge(c)
+        assert res["merged"] == 1 and res["status"] == 0, res
+        members = box.members(0)
+        sp = box.spheres()
+        rel = np.stack([sp["rx"][members], sp["ry"][members], sp["rz"][members]], axis=1)
+        want = O.merge_geometry(1, nk, rel, spheres["r"][members], spheres["vol"][members], spheres["surf"][members])
+        for f in ("volume", "surface", "rg", "rmax", "overlapping", "coordination", "sphere_volume", "radius_sum"):
+            assert res[f] == want[f], (q, f, res[f], want[f])
+        assert np.array_equal(res["rcm"], want["rcm"]), (q, res["rcm"], want["rcm"])
+        assert want["volume"] < want["sphere_volume"]  # the caps took something off
+        nk += sizes[q]
+    assert box.aggregates()["count"][0] == sum(sizes)
+
+
+def test_caps_run_bit_exact():
+    """N=3000 at FV=0.05 with volsurf_method = caps to 60 aggregates: every step and merge identical."""
+    say("case: dense N=3000 FV=0.05 caps -> 60 aggregates")
+    compare_runs((CFG / "dense_small_caps.ini").read_text(), 10 ** 7)

@@ -35,4 +35,5 @@ def test_record_layouts():
     assert mcac_amd.STEP_DTYPE.itemsize == 112
     assert mcac_amd.MERGE_DTYPE.itemsize == 176
     assert mcac_amd.CONTACT_DTYPE.itemsize == 80
-    assert mcac_amd.MERGE_RESULT_DTYPE.itemsize == 136
+    assert mcac_amd.MERGE_RESULT_DTYPE.itemsize == 144
+    assert mcac_amd.Box.PARAMS.itemsize == 64
