@@ -71,8 +71,10 @@ typedef struct mcac_box_params {
     int32_t volsurf;            /* overlap correction of the merge update           */
                                 /* (compute_volume_surface, aggregat.cpp:321-430):  */
                                 /* 0 none (sphere sums); 1 spherical caps           */
-                                /* (SPHERICAL_CAPS, :354-386; no merge inside the   */
-                                /* contact step, mcac_box_set_auto_merge is a no-op) */
+                                /* (SPHERICAL_CAPS, :354-386); 2 alphas (ALPHAS,    */
+                                /* :392-422, correction factors on the host between */
+                                /* two device phases).  With 1 or 2 no merge runs   */
+                                /* inside the contact step (auto-merge stays off)   */
     int32_t reserved;           /* 0                                                */
 } mcac_box_params;
 

@@ -61,6 +61,8 @@ struct MergeState {
     double diffpos[3], refpos[3], r[3];
     unsigned long long rmax_bits;
     unsigned int row_overflow, pad;
+    // list-order sums of the first alphas phase: V, S, Σ rel·vol (3), Σ r, Σ vol·r², Σ vol
+    double sums[8];
     mcac_merge_result res;
 };
 
@@ -96,6 +98,9 @@ struct DevBox {
     double *row_c;
     double *row_d;             // pair distances of the rows (spherical caps only)
     double *cvol, *csurf;      // per-position corrected sphere volume and surface (spherical caps only)
+    double *sr3;               // pow(r, 3) per sphere, from the host's libm (alphas only)
+    double *walk;              // per-position neighbour-map walk rows of (c_ij, vp1 + vp2, sp1 + sp2) (alphas only)
+    int *walk_n;               // entries of each walk row (alphas only)
     SearchState *ss;
     MergeState *ms;
 };
@@ -144,7 +149,7 @@ struct MergeArgs {
     long long ms, os, magg, oagg;
     int keep, rem, n_k, n_r, new_start, start_k, start_r, n_div;
     int cap_k;                 // keeper's segment capacity after the merge
-    int volsurf;               // 0: sphere sums (method none); 1: spherical caps
+    int volsurf;               // 0: sphere sums (method none); 1: spherical caps; 2: alphas
     double L;
     MergeRecord *out;          // pinned host record
     unsigned long long seq;

@@ -130,6 +130,7 @@ void PhysicalModel::load(const IniFile &ini) {
     s.clear();
     if (ini.get("surface_growth", "volsurf_method", s) && !s.empty()) {  // constants.hpp:103-111
         if (s == "caps") volsurf = 1;
+        else if (s == "alphas") volsurf = 2;
         else if (s != "none") unsupported = "volsurf_method=" + s;
     }
     flag = false;

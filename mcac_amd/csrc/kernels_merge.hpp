@@ -506,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b, MergeArgs a) {
         const int cnt = (n - t0 < kBlock) ? n - t0 : kBlock;
         if (threadIdx.x < cnt) {
             int s = b.members[st + t0 + threadIdx.x];
-            tv[threadIdx.x] = a.volsurf ? b.cvol[t0 + threadIdx.x] : b.svol[s];
+            tv[threadIdx.x] = a.volsurf == 1 ? b.cvol[t0 + threadIdx.x] : b.svol[s];
             td[threadIdx.x] = b.dc[t0 + threadIdx.x];
         }
         __syncthreads();
@@ -692,12 +692,44 @@ __global__ __launch_bounds__(kBlock) void k_caps_volumes(DevBox b, MergeArgs a) 
     }
 }
 
-// Ordered sums of the merged aggregate with the corrected volumes, all
-// positions from the start (aggregat.cpp:387-390, 437-441, 257-260, 474-478):
-// one block stages tiles in LDS, thread 0 accumulates in list order.
+// Thread 0: centre of mass from the list-order sums and the aggregate volume V
+// (aggregat.cpp:442-451), the merged aggregate's row, and the result fields.
+__device__ __forceinline__ void merge_store_result(const DevBox &b, const MergeArgs &a, int k, int n, double V,
+                                                   double S, double vx, double vy, double vz, double sr,
+                                                   double svr2, double raw) {
+    MergeState *m = b.ms;
+    const int st = m->new_start;
+    double r[3] = {vx / V, vy / V, vz / V};
+    const int s0 = b.members[st];
+    double com[3] = {periodic_position(b.sx[s0] + r[0], a.L), periodic_position(b.sy[s0] + r[1], a.L),
+                     periodic_position(b.sz[s0] + r[2], a.L)};
+    store_merged(b, a, V, S, vx, vy, vz, sr, svr2, r, com);
+    for (int l = 0; l < 3; ++l) m->r[l] = r[l];
+    mcac_merge_result &res = m->res;
+    res.keeper = k;
+    res.removed = a.rem;
+    res.n = n;
+    res.merged = 1;
+    res.status = (V <= 0 || S <= 0) ? MCAC_VOL_SURF_ERROR : (m->row_overflow ? MCAC_E_CAPACITY : 0);
+    res.volume = V;
+    res.surface = S;
+    res.sphere_volume = raw;
+    res.radius_sum = sr;
+    for (int l = 0; l < 3; ++l) {
+        res.com[l] = com[l];
+        res.rcm[l] = r[l];
+    }
+}
+
+// Ordered sums of the merged aggregate, all positions from the start
+// (aggregat.cpp:387-390, 437-441, 257-260, 474-478), with the corrected
+// volumes (caps: stores the aggregate) or the spheres' own (alphas: the sums
+// wait in MergeState for the host's correction factors): one block stages
+// tiles in LDS, thread 0 accumulates in list order.
 __global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) {
     MergeState *m = b.ms;
     if (!m->ok) return;
+    const bool alphas = a.volsurf == 2;
     __shared__ double tv[kBlock], ts[kBlock], tx[kBlock], ty[kBlock], tz[kBlock], tr[kBlock], tw[kBlock];
     const int k = m->keep, n = m->n_k + m->n_r, st = m->new_start;
     double V = 0.0, S = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, sr = 0.0, svr2 = 0.0, raw = 0.0;
@@ -706,8 +738,8 @@ __global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) {
         if (threadIdx.x < cnt) {
             const int t = t0 + threadIdx.x;
             const int s = b.members[st + t];
-            tv[threadIdx.x] = b.cvol[t];
-            ts[threadIdx.x] = b.csurf[t];
+            tv[threadIdx.x] = alphas ? b.svol[s] : b.cvol[t];
+            ts[threadIdx.x] = alphas ? b.ssurf[s] : b.csurf[t];
             tw[threadIdx.x] = b.svol[s];
             tx[threadIdx.x] = b.srx[s];
             ty[threadIdx.x] = b.sry[s];
@@ -731,25 +763,65 @@ __global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) {
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
-    double r[3] = {vx / V, vy / V, vz / V};
-    const int s0 = b.members[st];
-    double com[3] = {periodic_position(b.sx[s0] + r[0], a.L), periodic_position(b.sy[s0] + r[1], a.L),
-                     periodic_position(b.sz[s0] + r[2], a.L)};
-    store_merged(b, a, V, S, vx, vy, vz, sr, svr2, r, com);
-    for (int l = 0; l < 3; ++l) m->r[l] = r[l];
-    mcac_merge_result &res = m->res;
-    res.keeper = k;
-    res.removed = a.rem;
-    res.n = n;
-    res.merged = 1;
-    res.status = (V <= 0 || S <= 0) ? MCAC_VOL_SURF_ERROR : (m->row_overflow ? MCAC_E_CAPACITY : 0);
-    res.volume = V;
-    res.surface = S;
-    res.sphere_volume = raw;
-    res.radius_sum = sr;
-    for (int l = 0; l < 3; ++l) {
-        res.com[l] = com[l];
-        res.rcm[l] = r[l];
+    if (alphas) {
+        const double s8[8] = {V, S, vx, vy, vz, sr, svr2, raw};
+        for (int l = 0; l < 8; ++l) m->sums[l] = s8[l];
+        return;
+    }
+    merge_store_result(b, a, k, n, V, S, vx, vy, vz, sr, svr2, raw);
+}
+
+// ------------------------------------------------------------------ alphas (aggregat.cpp:392-422)
+// Each sphere's walk of its neighbour map in the map's order (all
+// neighbours, as the reference's loop over distances[i]): per entry the
+// overlap coefficient c_ij = (r_i + r_j - d) / (r_i + r_j), pow(r_i, 3) +
+// pow(r_j, 3) (host libm values) and r_i² + r_j².  The host sums the rows in
+// order (the cube of c_ij is glibc pow) and derives the correction factors.
+// Moves the row counters to walk_n and clears them.
+__global__ __launch_bounds__(kBlock) void k_alpha_walk(DevBox b, MergeArgs a) {
+    const MergeState *m = b.ms;
+    if (!m->ok) return;
+    const int n = m->n_k + m->n_r, st = m->new_start, nk = m->n_k;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int rc = b.row_cnt[t];
+        if (rc > kOverlapSlots) rc = kOverlapSlots;
+        const long long base = static_cast<long long>(t) * kOverlapSlots;
+        int64_t keys[kOverlapSlots], order[kOverlapSlots];
+        int prev = 0;
+        for (int x = 0; x < rc; ++x) {
+            keys[x] = b.row_j[base + x];
+            if (t < nk && keys[x] < nk) ++prev;
+        }
+        neighbour_order(prev, keys, rc, order);
+        const int s = b.members[st + t];
+        const double ri = b.sr[s], vi = b.sr3[s];
+        for (int y = 0; y < rc; ++y) {
+            int x = 0;
+            while (keys[x] != order[y]) ++x;
+            const int sj = b.members[st + static_cast<int>(order[y])];
+            const double rj = b.sr[sj];
+            const double c = (ri + rj - b.row_d[base + x]) / (ri + rj);
+            double *w = b.walk + 3 * (base + y);
+            w[0] = c;
+            w[1] = vi + b.sr3[sj];
+            w[2] = ri * ri + rj * rj;
+        }
+        b.walk_n[t] = rc;
+        b.row_cnt[t] = 0;
+    }
+}
+
+// Second alphas phase (thread 0): the corrected volume and surface, and the
+// walk's overlap and coordination (aggregat.cpp:417-421) when it had entries.
+__global__ void k_alpha_store(DevBox b, MergeArgs a, double V, double S, double overlapping, double coordination,
+                              int set_overlap) {
+    MergeState *m = b.ms;
+    if (!m->ok || threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double *s8 = m->sums;
+    merge_store_result(b, a, m->keep, m->n_k + m->n_r, V, S, s8[2], s8[3], s8[4], s8[5], s8[6], s8[7]);
+    if (set_overlap) {
+        m->res.overlapping = overlapping;
+        m->res.coordination = coordination;
     }
 }
 

@@ -83,6 +83,8 @@ struct mcac_box {
     int64_t n_fused = 0, n_split = 0;  // searches launched as k_step / k_filter + k_solve
     int64_t n_compactions = 0;
     MergeRecord *h_merge = nullptr, *d_merge = nullptr;
+    std::vector<double> h_walk;      // alphas: the merged aggregate's neighbour-map walk rows
+    std::vector<int> h_walk_n;
     unsigned long long seq = 0;        // step tickets (searches)
     unsigned long long merge_seq = 0;  // merge records
     std::vector<void *> allocs;
@@ -627,7 +629,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (!params || !out) return MCAC_E_ARGUMENT;
     if (params->box_length <= 0 || params->n_div < 1 || params->n_div > 1290 || params->max_spheres < 1 ||
         params->max_aggregates < 1 || params->max_spheres > INT32_MAX / 2 || params->max_aggregates > INT32_MAX / 2 ||
-        params->volsurf < 0 || params->volsurf > 1)
+        params->volsurf < 0 || params->volsurf > 2)
         return MCAC_E_ARGUMENT;
     auto *b = new (std::nothrow) mcac_box();
     if (!b) return MCAC_E_CAPACITY;
@@ -669,6 +671,10 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.row_cnt, Ns); ALLOC(d.row_j, Ns * kOverlapSlots); ALLOC(d.row_c, Ns * kOverlapSlots);
     if (params->volsurf == 1) {
         ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.cvol, Ns); ALLOC(d.csurf, Ns);
+    }
+    if (params->volsurf == 2) {
+        ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.sr3, Ns); ALLOC(d.walk, 3 * Ns * kOverlapSlots);
+        ALLOC(d.walk_n, Ns);
     }
     ALLOC(d.ss, 1); ALLOC(d.ms, 1);
 #undef ALLOC
@@ -717,6 +723,7 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
                     const double *surface, const int64_t *count, const int64_t *members) {
     if (!b || ns < 0 || na < 0 || ns > b->p.max_spheres || na > b->p.max_aggregates) return MCAC_E_ARGUMENT;
     const size_t S = static_cast<size_t>(ns), A = static_cast<size_t>(na);
+    int rc = 0;
     auto up = [&](double *dst, const double *src, size_t n) -> int {
         if (n) HIP_TRY(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
         return 0;
@@ -771,7 +778,11 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
         }
     }
     if (static_cast<int64_t>(pool.size()) > b->pool_cap) return MCAC_E_CAPACITY;
-    int rc = 0;
+    if (b->p.volsurf == 2) {  // pow(r, 3) of the alphas sums (aggregat.cpp:402-403), host libm as the reference
+        std::vector<double> r3(S);
+        for (size_t s = 0; s < S; ++s) r3[s] = std::pow(r[s], 3.0);
+        if ((rc = up(b->d.sr3, r3.data(), S))) return rc;
+    }
     if ((rc = up(b->d.sx, x, S)) || (rc = up(b->d.sy, y, S)) || (rc = up(b->d.sz, z, S)) || (rc = up(b->d.sr, r, S)) ||
         (rc = up(b->d.svol, vol, S)) || (rc = up(b->d.ssurf, surf, S)) || (rc = up(b->d.srx, rx, S)) ||
         (rc = up(b->d.sry, ry, S)) || (rc = up(b->d.srz, rz, S)))
@@ -901,6 +912,57 @@ static int merge_done(mcac_box *b, size_t K, size_t R, int new_start, int new_ca
     return out->status;
 }
 
+// Alphas correction of a merged aggregate (aggregat.cpp:289-319, 392-422) from
+// its neighbour-map walk rows (c_ij, vp1 + vp2, sp1 + sp2) in walk order and
+// its list-order volume V and surface S.  Host FP64 with the reference's
+// libm calls: pow(x, 2) is x * x in the reference's build, pow(x, 3) and
+// pow(x, 1.5) are glibc's.
+struct AlphaFactors {
+    double volume, surface, overlapping, coordination;
+    bool any;
+};
+AlphaFactors alpha_factors(int n, const int *walk_n, const double *walk, double V, double S) {
+    double overlapping = 0.0, c_v30 = 0.0, c_v20 = 0.0, c_s10 = 0.0, vp_sum = 0.0, sp_sum = 0.0;
+    size_t intersections = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int y = 0; y < walk_n[i]; ++y) {
+            const double *w = walk + 3 * (static_cast<size_t>(i) * kOverlapSlots + static_cast<size_t>(y));
+            const double c_ij = w[0], vp = w[1], sp = w[2];
+            vp_sum += vp;
+            sp_sum += sp;
+            overlapping += c_ij;
+            c_s10 += c_ij * sp;
+            c_v20 += (c_ij * c_ij) * vp;
+            c_v30 += std::pow(c_ij, 3.0) * vp;
+        }
+        intersections += static_cast<size_t>(walk_n[i]);
+    }
+    AlphaFactors f{V, S, overlapping, 0.0, intersections > 0};
+    if (intersections == 0) return f;
+    c_s10 /= sp_sum;
+    c_v20 /= vp_sum;
+    c_v30 /= vp_sum;
+    const double min_coordination_number = 2 * (1.0 - 1.0 / static_cast<double>(n));
+    const double alpha_vs_extreme = 1.0 / static_cast<double>(n);  // aggregat.cpp:539
+    f.overlapping = overlapping / static_cast<double>(intersections);
+    f.coordination = static_cast<double>(intersections) / static_cast<double>(n);
+    const double diff = std::abs(f.coordination - min_coordination_number);
+    // volume_alpha_correction :289-304
+    double corr = 0.25 * (3.0 * c_v20 - c_v30) * f.coordination - c_v30 * diff * 0.62741833 -
+                  std::pow(diff, 1.5) * 0.00332425;
+    if (corr < 0.0) corr = 1.0;
+    corr = smin(corr, 1.0);
+    const double alpha_v = smax(1.0 - corr, alpha_vs_extreme);
+    // surface_alpha_correction :305-319
+    double corr_s = 0.5 * c_s10 * f.coordination - (c_s10 * c_s10) * diff * 0.70132500 - (diff * diff) * 0.00450000;
+    if (corr_s < 0.0) corr_s = 1.0;
+    corr_s = smin(corr_s, 1.0);
+    const double alpha_s = smax(1.0 - corr_s, alpha_vs_extreme);
+    f.volume = V * alpha_v;
+    f.surface = S * alpha_s;
+    return f;
+}
+
 int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     if (!b || !c || !out) return MCAC_E_ARGUMENT;
     std::memset(out, 0, sizeof(*out));
@@ -975,6 +1037,34 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
         hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
         hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
         hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+    } else if (a.volsurf == 2) {
+        // alphas: contact rows, the neighbour-map walks and the list-order sums
+        // on the device; the correction factors on the host (glibc pow, as
+        // the reference); then the centre of mass, dc, rmax and Rg (publishes)
+        const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
+        const unsigned tiles = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
+        hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_alpha_walk, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        MergeState ms{};
+        HIP_TRY(hipMemcpy(&ms, b->d.ms, sizeof(MergeState), hipMemcpyDeviceToHost));
+        if (ms.ok) {
+            b->h_walk_n.resize(static_cast<size_t>(n));
+            b->h_walk.resize(3 * static_cast<size_t>(n) * kOverlapSlots);
+            HIP_TRY(hipMemcpy(b->h_walk_n.data(), b->d.walk_n, static_cast<size_t>(n) * sizeof(int),
+                              hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(b->h_walk.data(), b->d.walk, b->h_walk.size() * sizeof(double), hipMemcpyDeviceToHost));
+            const AlphaFactors f = alpha_factors(n, b->h_walk_n.data(), b->h_walk.data(), ms.sums[0], ms.sums[1]);
+            hipLaunchKernelGGL(k_alpha_store, dim3(1), dim3(64), 0, b->stream, b->d, a, f.volume, f.surface,
+                               f.overlapping, f.coordination, f.any ? 1 : 0);
+            hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+            hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        }
     } else if (n <= kSmallMerge) {
         hipLaunchKernelGGL(k_merge_small, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
     } else {

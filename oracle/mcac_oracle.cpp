@@ -269,7 +269,7 @@ struct Physics {
     size_t n_verlet_divisions = 10;
     size_t n_monomeres = 2500;
     int init_mode = LOGNORMAL;
-    int volsurf = 0;  // volsurf_method: 0 none, 1 caps (constants.hpp:103-111)
+    int volsurf = 0;  // volsurf_method: 0 none, 1 caps, 2 alphas (constants.hpp:103-111)
     int pick_random = 1;
     size_t n_iter_without_event = 0;
     double physical_time_limit = -1;
@@ -303,6 +303,7 @@ struct Physics {
         s.clear();
         extract(ini, "surface_growth", "volsurf_method", s);
         if (s == "caps") volsurf = 1;
+        else if (s == "alphas") volsurf = 2;
         else if (!s.empty() && s != "none") unsupported = "volsurf_method=" + s;
         extract(ini, "nucleation", "with_nucleation", with_nucleation);
         extract(ini, "limits", "number_of_aggregates", number_of_aggregates_limit);
@@ -407,6 +408,32 @@ struct Physics {
         volume_fraction = tv / box_volume;
     }
 };
+
+// aggregat.cpp:289-304
+double volume_alpha_correction(const double coordination_number, const double c_20, const double c_30,
+                               const double min_coordination_number, const double alpha_vs_extreme) {
+    double difference_coordination = std::abs(coordination_number - min_coordination_number);
+    double correction = 0.25 * (3.0 * c_20 - c_30) * coordination_number -
+                        c_30 * difference_coordination * 0.62741833 -
+                        pow(difference_coordination, 1.5) * 0.00332425;
+    if (correction < 0.0) correction = 1.0;
+    correction = std::min(correction, 1.0);
+    double alpha_v = 1.0 - correction;
+    alpha_v = std::max(alpha_v, alpha_vs_extreme);
+    return alpha_v;
+}
+// aggregat.cpp:305-319
+double surface_alpha_correction(const double coordination_number, const double c_10,
+                                const double min_coordination_number, const double alpha_vs_extreme) {
+    double difference_coordination = std::abs(coordination_number - min_coordination_number);
+    double correction = 0.5 * c_10 * coordination_number - std::pow(c_10, 2) * difference_coordination * 0.70132500 -
+                        std::pow(difference_coordination, 2) * 0.00450000;
+    if (correction < 0.0) correction = 1.0;
+    correction = std::min(correction, 1.0);
+    double alpha_s = 1.0 - correction;
+    alpha_s = std::max(alpha_s, alpha_vs_extreme);
+    return alpha_s;
+}
 
 // sphere_intersection.cpp:29-66 (spherical caps, Morán et al. 2018)
 struct Intersection {
@@ -602,6 +629,43 @@ struct Sim {
         for (size_t i = 0; i < n; i++) {
             a.volume = a.volume + a.volumes[i];
             a.surface = a.surface + a.surfaces[i];
+        }
+        if (pm.volsurf == 2) {  // :391-426
+            a.overlapping = 0.0;
+            double c_v30(0.0), c_v20(0.0), c_s10(0.0);
+            double vp_sum(0.0), sp_sum(0.0);
+            size_t intersections(0);
+            for (size_t i = 0; i < n; i++) {
+                for (const auto &it : a.distances[i]) {
+                    double radius_1 = sr[a.sph[i]];
+                    double radius_2 = sr[a.sph[it.first]];
+                    double c_ij = (radius_1 + radius_2 - it.second) / (radius_1 + radius_2);
+                    double vp1 = std::pow(radius_1, 3);
+                    double vp2 = std::pow(radius_2, 3);
+                    double sp1 = std::pow(radius_1, 2);
+                    double sp2 = std::pow(radius_2, 2);
+                    vp_sum += (vp1 + vp2);
+                    sp_sum += (sp1 + sp2);
+                    a.overlapping += c_ij;
+                    c_s10 += c_ij * (sp1 + sp2);
+                    c_v20 += std::pow(c_ij, 2) * (vp1 + vp2);
+                    c_v30 += std::pow(c_ij, 3) * (vp1 + vp2);
+                }
+                intersections += a.distances[i].size();
+            }
+            if (intersections > 0) {
+                c_s10 /= sp_sum;
+                c_v20 /= vp_sum;
+                c_v30 /= vp_sum;
+                double alpha_vs_extreme = 1.0 / static_cast<double>(n);  // aggregat.cpp:539
+                double min_coordination_number = 2 * (1.0 - 1.0 / static_cast<double>(n));
+                a.overlapping /= static_cast<double>(intersections);
+                a.coordination = static_cast<double>(intersections) / static_cast<double>(n);
+                a.volume *= volume_alpha_correction(a.coordination, c_v20, c_v30, min_coordination_number,
+                                                    alpha_vs_extreme);
+                a.surface *= surface_alpha_correction(a.coordination, c_s10, min_coordination_number,
+                                                      alpha_vs_extreme);
+            }
         }
         if (a.volume <= 0 || a.surface <= 0) throw std::runtime_error("VolSurfError");
     }

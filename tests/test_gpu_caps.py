@@ -1,6 +1,7 @@
-"""Spherical caps (volsurf_method = caps) on the device, bit for bit against
-the oracle: merges of overlapping clusters (several caps per sphere, so the
-neighbour-map walk order matters) and a full run."""
+"""Overlap corrections of the merge update on the device (volsurf_method =
+caps and alphas), bit for bit against the oracle: merges of overlapping
+clusters (several contacts per sphere, so the neighbour-map walk order
+matters) and full runs."""
 import numpy as np
 import pytest
 
@@ -86,12 +87,13 @@ def merge_case(seed, sizes, L=4e-6):
     return spheres, aggs, first
 
 
+@pytest.mark.parametrize("volsurf", [1, 2])
 @pytest.mark.parametrize("seed,sizes", [(1, (12, 9, 7)), (2, (40, 30, 300)), (3, (1, 1, 25)), (4, (150, 200, 60))])
-def test_caps_merges_match_oracle(seed, sizes):
-    say(f"case: caps merges seed={seed} sizes={sizes}")
+def test_overlap_merges_match_oracle(seed, sizes, volsurf):
+    say(f"case: overlap merges volsurf={volsurf} seed={seed} sizes={sizes}")
     M = mcac()
     spheres, aggs, first = merge_case(seed, sizes)
-    box = M.Box(4e-6, 10, spheres["r"].size, len(sizes), volsurf=1)
+    box = M.Box(4e-6, 10, spheres["r"].size, len(sizes), volsurf=volsurf)
     box.upload(spheres, aggs)
     nk = sizes[0]
     for q in range(1, len(sizes)):
@@ -105,16 +107,17 @@ def test_caps_merges_match_oracle(seed, sizes):
         members = box.members(0)
         sp = box.spheres()
         rel = np.stack([sp["rx"][members], sp["ry"][members], sp["rz"][members]], axis=1)
-        want = O.merge_geometry(1, nk, rel, spheres["r"][members], spheres["vol"][members], spheres["surf"][members])
+        want = O.merge_geometry(volsurf, nk, rel, spheres["r"][members], spheres["vol"][members], spheres["surf"][members])
         for f in ("volume", "surface", "rg", "rmax", "overlapping", "coordination", "sphere_volume", "radius_sum"):
             assert res[f] == want[f], (q, f, res[f], want[f])
         assert np.array_equal(res["rcm"], want["rcm"]), (q, res["rcm"], want["rcm"])
-        assert want["volume"] < want["sphere_volume"]  # the caps took something off
+        assert want["volume"] < want["sphere_volume"]  # the correction took something off
         nk += sizes[q]
     assert box.aggregates()["count"][0] == sum(sizes)
 
 
-def test_caps_run_bit_exact():
-    """N=3000 at FV=0.05 with volsurf_method = caps to 60 aggregates: every step and merge identical."""
-    say("case: dense N=3000 FV=0.05 caps -> 60 aggregates")
-    compare_runs((CFG / "dense_small_caps.ini").read_text(), 10 ** 7)
+@pytest.mark.parametrize("method", ["caps", "alphas"])
+def test_overlap_run_bit_exact(method):
+    """N=3000 at FV=0.05 with an overlap correction to 60 aggregates: every step and merge identical."""
+    say(f"case: dense N=3000 FV=0.05 {method} -> 60 aggregates")
+    compare_runs((CFG / f"dense_small_{method}.ini").read_text(), 10 ** 7)

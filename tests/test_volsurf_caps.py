@@ -1,6 +1,7 @@
-"""Spherical caps (volsurf_method = caps, aggregat.cpp:354-386): the oracle
-against closed forms, and the order in which the device walks a sphere's
-neighbour map against libstdc++'s own std::unordered_map (CPU only)."""
+"""Overlap corrections of the merge update (volsurf_method = caps,
+aggregat.cpp:354-386, and alphas, :391-426): the oracle against closed forms,
+and the order in which the device walks a sphere's neighbour map against
+libstdc++'s own std::unordered_map (CPU only)."""
 import math
 
 import numpy as np
@@ -68,6 +69,38 @@ def test_two_overlapping_spheres_closed_forms():
     g0 = ora.merge_geometry(0, 1, rel, [r, r], [v, v], [s, s])
     assert g0["volume"] == 0.0 + v + v and g0["surface"] == 0.0 + s + s
     assert g0["rg"] == pytest.approx(math.sqrt(d * d / 4 + 0.6 * r * r), rel=1e-12)
+
+
+def test_alphas_two_spheres_match_exact_caps():
+    """For two equal spheres the alphas fits reduce to the exact cap fractions:
+    volume 1 - c²(3 - c)/4 and surface 1 - c/2 of the sphere sums, c the overlap
+    coefficient (aggregat.cpp:289-319 with coordination 1 = its minimum)."""
+    for r, d in ((15e-9, 24e-9), (10e-9, 19.5e-9), (20e-9, 30e-9)):
+        v, s = sphere_vs(r)
+        rel = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0]])
+        ga = ora.merge_geometry(2, 1, rel, [r, r], [v, v], [s, s])
+        gc = ora.merge_geometry(1, 1, rel, [r, r], [v, v], [s, s])
+        c = (2 * r - d) / (2 * r)
+        assert ga["volume"] == pytest.approx(2 * v * (1 - c * c * (3 - c) / 4), rel=1e-12)
+        assert ga["surface"] == pytest.approx(2 * s * (1 - c / 2), rel=1e-12)
+        assert ga["volume"] == pytest.approx(gc["volume"], rel=1e-12)
+        assert ga["surface"] == pytest.approx(gc["surface"], rel=1e-12)
+        assert ga["coordination"] == 1.0 and ga["overlapping"] == pytest.approx(c, rel=1e-12)
+        # update_partial weights by the spheres' own volumes but divides by the
+        # corrected aggregate volume (aggregat.cpp:431-483 after :417-421)
+        assert ga["rcm"] == pytest.approx([d / 2 * (2 * v) / ga["volume"], 0, 0], rel=1e-12)
+        rc = ga["rcm"][0]
+        rg2 = (v * rc * rc + v * (d - rc) ** 2 + 0.6 * 2 * v * r * r) / ga["volume"]
+        assert ga["rg"] == pytest.approx(math.sqrt(rg2), rel=1e-9)
+
+
+def test_alphas_without_contact_keeps_sphere_values():
+    # no contact: no correction, overlap and coordination stay 0 (aggregat.cpp:393, 415)
+    r = 10e-9
+    v, s = sphere_vs(r)
+    g = ora.merge_geometry(2, 1, np.zeros((1, 3)), [r], [v], [s])
+    assert g["volume"] == 0.0 + v and g["surface"] == 0.0 + s
+    assert g["coordination"] == 0.0 and g["overlapping"] == 0.0
 
 
 def test_chain_of_three_closed_form():
@@ -164,11 +197,12 @@ def test_caps_walk_order_changes_bits():
     assert n_diff > 0
 
 
-def test_oracle_run_with_caps():
-    """A dense box with volsurf_method = caps runs to its stop criterion; every
+@pytest.mark.parametrize("method", ["caps", "alphas"])
+def test_oracle_run_with_overlap_correction(method):
+    """A dense box with an overlap correction runs to its stop criterion; every
     aggregate's volume is at most the sum of its spheres' volumes."""
     import pathlib
-    text = (pathlib.Path(__file__).parent / "configs" / "dense_small_caps.ini").read_text()
+    text = (pathlib.Path(__file__).parent / "configs" / f"dense_small_{method}.ini").read_text()
     sim = ora.OracleSim(text, fast_init=True)
     sim.run(10 ** 7)
     assert sim.finished()
