@@ -73,8 +73,11 @@ typedef struct mcac_box_params {
                                 /* 0 none (sphere sums); 1 spherical caps           */
                                 /* (SPHERICAL_CAPS, :354-386); 2 alphas (ALPHAS,    */
                                 /* :392-422, correction factors on the host between */
-                                /* two device phases).  With 1 or 2 no merge runs   */
-                                /* inside the contact step (auto-merge stays off)   */
+                                /* two device phases); 3 arvo (EXACT_ARVO, :337-347,*/
+                                /* arvo.cpp restated with fixes, per-sphere parts   */
+                                /* within 1e-10 of the oracle).  With 1 to 3 no     */
+                                /* merge runs inside the contact step (auto-merge   */
+                                /* stays off)                                       */
     int32_t reserved;           /* 0                                                */
 } mcac_box_params;
 
@@ -160,6 +163,12 @@ int mcac_box_set_auto_merge(mcac_box *box, int enable);
  * m keys in that order.  No counterpart in the reference: the reference walks
  * the map itself; the device kernels walk this order. */
 int mcac_neighbour_order(int prev, const int64_t *keys, int m, int64_t *out);
+/* Host-only check of the ARVO part (no GPU call): ball 0 of the k + 1 balls
+ * (x, y, z, r in nm, k <= 16 neighbours, each overlapping ball 0 and not
+ * inside it) carries out[0] of the union's volume and out[1] of its area
+ * (arvo.cpp:728-788 restated).  No counterpart in the reference: it is the
+ * per-sphere step the device kernel runs, exposed for host checks. */
+int mcac_arvo_part(int k, const double *x, const double *y, const double *z, const double *r, double out[2]);
 
 /* Groups of boxes: one queued move of each of many boxes in a single launch
  * (one workgroup per box).  No counterpart in the reference, which runs one box

@@ -20,7 +20,7 @@ import pathlib
 import numpy as np
 
 __all__ = [
-    "physics_scalars", "swept_sphere_batch", "neighbour_order", "LIB_DIR", "load_gpu", "load_sim", "Simulation", "Box", "STEP_DTYPE", "MERGE_DTYPE",
+    "physics_scalars", "swept_sphere_batch", "neighbour_order", "arvo_part", "LIB_DIR", "load_gpu", "load_sim", "Simulation", "Box", "STEP_DTYPE", "MERGE_DTYPE",
     "CONTACT_DTYPE", "MERGE_RESULT_DTYPE", "GPU_SYMBOLS", "SIM_SYMBOLS",
 ]
 
@@ -56,7 +56,7 @@ GPU_SYMBOLS = (
     "mcac_box_download_spheres", "mcac_box_sphere_owners", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
-    "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order",
+    "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
@@ -127,6 +127,7 @@ def load_gpu():
         L.mcac_group_stats.argtypes = [_P, _I64P]
         L.mcac_box_block_steps.argtypes = [_P, _D]
         L.mcac_neighbour_order.argtypes = [_I, _I64P, _I, _I64P]
+        L.mcac_arvo_part.argtypes = [_I, _DP, _DP, _DP, _DP, _DP]
         L.mcac_version.restype = ctypes.c_char_p
         _gpu = L
     return _gpu
@@ -201,6 +202,15 @@ def neighbour_order(prev: int, keys) -> np.ndarray:
     out = np.zeros_like(k)
     _check(load_gpu().mcac_neighbour_order(int(prev), _ip(k), len(k), _ip(out)), "mcac_neighbour_order")
     return out
+
+
+def arvo_part(balls) -> tuple:
+    """(volume, area) part of ball 0 among balls (rows x, y, z, r in nm): host-only check (mcac_arvo_part)."""
+    b = np.ascontiguousarray(balls, dtype=np.float64).reshape(-1, 4)
+    cols = [np.ascontiguousarray(b[:, q]) for q in range(4)]
+    out = np.zeros(2)
+    _check(load_gpu().mcac_arvo_part(len(b) - 1, *(_dp(c) for c in cols), _dp(out)), "mcac_arvo_part")
+    return float(out[0]), float(out[1])
 
 
 def swept_sphere_batch(p1, r1, p2, r2, direction, dist, box: float, device: int = 0):

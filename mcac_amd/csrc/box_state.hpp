@@ -96,8 +96,8 @@ struct DevBox {
     double *dc;
     int *row_cnt, *row_j;
     double *row_c;
-    double *row_d;             // pair distances of the rows (spherical caps only)
-    double *cvol, *csurf;      // per-position corrected sphere volume and surface (spherical caps only)
+    double *row_d;             // pair distances of the rows (overlap corrections only)
+    double *cvol, *csurf;      // per-position corrected sphere volume and surface (spherical caps, arvo)
     double *sr3;               // pow(r, 3) per sphere, from the host's libm (alphas only)
     double *walk;              // per-position neighbour-map walk rows of (c_ij, vp1 + vp2, sp1 + sp2) (alphas only)
     int *walk_n;               // entries of each walk row (alphas only)
@@ -149,7 +149,7 @@ struct MergeArgs {
     long long ms, os, magg, oagg;
     int keep, rem, n_k, n_r, new_start, start_k, start_r, n_div;
     int cap_k;                 // keeper's segment capacity after the merge
-    int volsurf;               // 0: sphere sums (method none); 1: spherical caps; 2: alphas
+    int volsurf;               // 0: sphere sums (method none); 1: spherical caps; 2: alphas; 3: arvo
     double L;
     MergeRecord *out;          // pinned host record
     unsigned long long seq;

@@ -131,6 +131,7 @@ void PhysicalModel::load(const IniFile &ini) {
     if (ini.get("surface_growth", "volsurf_method", s) && !s.empty()) {  // constants.hpp:103-111
         if (s == "caps") volsurf = 1;
         else if (s == "alphas") volsurf = 2;
+        else if (s == "arvo") volsurf = 3;
         else if (s != "none") unsupported = "volsurf_method=" + s;
     }
     flag = false;

@@ -85,7 +85,7 @@ struct PhysicalModel {
     size_t n_time_per_file = 10;
     bool with_domain_duplication = true, with_collisions = true, enforce_volume_fraction = true;
     std::string output_dir = "MCAC_output";
-    int volsurf = 0;          // volsurf_method: 0 none, 1 caps (spherical caps), 2 alphas
+    int volsurf = 0;          // volsurf_method: 0 none, 1 caps (spherical caps), 2 alphas, 3 arvo
     std::string unsupported;  // first option this path does not implement
 
     void load(const IniFile &ini);

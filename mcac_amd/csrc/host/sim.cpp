@@ -151,16 +151,33 @@ struct Simulation {
         if (own_group) mcac_group_destroy(own_group);  // after the box that works on its stream
     }
 
-    // Monomer update() (aggregat.cpp:284-288) for a single sphere of radius r.
+    // Volume and surface a one-sphere aggregate carries (compute_volume_surface,
+    // aggregat.cpp:321-430): the sphere's own, or with ARVO (method 3) the
+    // closed form arvo.cpp computes for a ball without neighbours, in
+    // nanometres (VOLUME_FACTOR r^3, SURFACE_FACTOR r^2) and back to SI.
+    double single_volume(double r, double v) const {
+        if (pm.volsurf != 3) return v;
+        const double rn = r * 1e+09;
+        return kVolumeFactor * rn * rn * rn * 1e-27;
+    }
+    double single_surface(double r, double s) const {
+        if (pm.volsurf != 3) return s;
+        const double rn = r * 1e+09;
+        return kSurfaceFactor * rn * rn * 1e-18;
+    }
+
+    // Monomer update() (aggregat.cpp:284-288) for a single sphere of radius r
+    // (own volume v and surface s).
     void monomer_update(size_t i, double r, double v, double s) {
-        vol[i] = 0.0 + v;
-        surf[i] = 0.0 + s;
+        const double part = single_volume(r, v);
+        vol[i] = 0.0 + part;
+        surf[i] = 0.0 + single_surface(r, s);
         rcm[i] = {0., 0., 0.};
         rmax[i] = smax(0.0, r + 0.0);
-        double brg = 0. + v * (r * r);
-        double arg = 0. + v * (0.0 * 0.0);
+        double brg = 0. + part * (r * r);
+        double arg = 0. + part * (0.0 * 0.0);
         rg[i] = std::sqrt(std::abs((arg + 3. / 5. * brg) / vol[i]));
-        AggregatePhysics ph = aggregate_physics(pm, 1, vol[i], vol[i], 0. + r, rg[i]);
+        AggregatePhysics ph = aggregate_physics(pm, 1, vol[i], 0.0 + v, 0. + r, rg[i]);
         dp[i] = ph.dp;
         f_agg[i] = ph.f_agg;
         d_m[i] = ph.d_m;
@@ -294,9 +311,9 @@ struct Simulation {
                 svol[s] = kVolumeFactor * std::pow(sr[s], 3);
                 ssurf[s] = kSurfaceFactor * std::pow(sr[s], 2);
             }
-            for (size_t i = 0; i < N; ++i) {
-                vol[i] = 0.0 + svol[i];
-                surf[i] = 0.0 + ssurf[i];
+            for (size_t i = 0; i < N; ++i) {  // compute_volume_surface only (aggregat_list_storage.cpp:84-86)
+                vol[i] = 0.0 + single_volume(sr[i], svol[i]);
+                surf[i] = 0.0 + single_surface(sr[i], ssurf[i]);
             }
         }
         live.reset(N);

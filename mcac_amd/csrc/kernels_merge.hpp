@@ -13,6 +13,7 @@
 // the multi-kernel sequence below.  Both publish the result to pinned memory.
 #pragma once
 #include "kernels_common.hpp"
+#include "arvo.hpp"
 #include "volsurf.hpp"
 
 namespace mcac_dev {
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b, MergeArgs a) {
         const int cnt = (n - t0 < kBlock) ? n - t0 : kBlock;
         if (threadIdx.x < cnt) {
             int s = b.members[st + t0 + threadIdx.x];
-            tv[threadIdx.x] = a.volsurf == 1 ? b.cvol[t0 + threadIdx.x] : b.svol[s];
+            tv[threadIdx.x] = (a.volsurf == 1 || a.volsurf == 3) ? b.cvol[t0 + threadIdx.x] : b.svol[s];
             td[threadIdx.x] = b.dc[t0 + threadIdx.x];
         }
         __syncthreads();
@@ -718,6 +719,60 @@ __device__ __forceinline__ void merge_store_result(const DevBox &b, const MergeA
     for (int l = 0; l < 3; ++l) {
         res.com[l] = com[l];
         res.rcm[l] = r[l];
+    }
+}
+
+// ARVO parts of every sphere of the merged aggregate (aggregat.cpp:337-347
+// with the restated arvo.cpp), one thread per list position t: the
+// neighbours are the sphere's map entries in the map's order that overlap it
+// and are not inside it; a sphere inside a neighbour carries nothing.  Writes
+// the parts (m^3, m^2) for the ordered sums; clears the row counters.
+__global__ __launch_bounds__(kBlock) void k_arvo_volumes(DevBox b, MergeArgs a) {
+    const MergeState *m = b.ms;
+    if (!m->ok) return;
+    const int n = m->n_k + m->n_r, st = m->new_start, nk = m->n_k;
+    const double nm = 1e+09;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int rc = b.row_cnt[t];
+        if (rc > kOverlapSlots) rc = kOverlapSlots;
+        const long long base = static_cast<long long>(t) * kOverlapSlots;
+        int64_t keys[kOverlapSlots], order[kOverlapSlots];
+        int prev = 0;
+        for (int x = 0; x < rc; ++x) {
+            keys[x] = b.row_j[base + x];
+            if (t < nk && keys[x] < nk) ++prev;
+        }
+        neighbour_order(prev, keys, rc, order);
+        const int s = b.members[st + t];
+        const double ri = b.sr[s];
+        double x[kArvoBalls + 1], y[kArvoBalls + 1], z[kArvoBalls + 1], r[kArvoBalls + 1];
+        x[0] = b.srx[s] * nm;
+        y[0] = b.sry[s] * nm;
+        z[0] = b.srz[s] * nm;
+        r[0] = ri * nm;
+        int k = 0;
+        bool inside = false;
+        for (int q = 0; q < rc; ++q) {
+            const int j = static_cast<int>(order[q]);
+            int e = 0;
+            while (keys[e] != order[q]) ++e;
+            const double d = b.row_d[base + e];
+            const int sj = b.members[st + j];
+            const double rj = b.sr[sj];
+            if (d >= (1. - kCoordinationEpsilon) * (ri + rj)) continue;       // touching within the tolerance
+            if (d + ri < rj || (d + ri == rj && j < t)) inside = true;        // inside the neighbour
+            if (d + rj < ri || (d + rj == ri && t < j)) continue;             // the neighbour is inside
+            ++k;
+            x[k] = b.srx[sj] * nm;
+            y[k] = b.sry[sj] * nm;
+            z[k] = b.srz[sj] * nm;
+            r[k] = rj * nm;
+        }
+        double vol = 0.0, area = 0.0;
+        if (!inside) arvo_ball(x, y, z, r, k, &vol, &area);
+        b.cvol[t] = vol * 1e-27;
+        b.csurf[t] = area * 1e-18;
+        b.row_cnt[t] = 0;
     }
 }
 

@@ -629,7 +629,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (!params || !out) return MCAC_E_ARGUMENT;
     if (params->box_length <= 0 || params->n_div < 1 || params->n_div > 1290 || params->max_spheres < 1 ||
         params->max_aggregates < 1 || params->max_spheres > INT32_MAX / 2 || params->max_aggregates > INT32_MAX / 2 ||
-        params->volsurf < 0 || params->volsurf > 2)
+        params->volsurf < 0 || params->volsurf > 3)
         return MCAC_E_ARGUMENT;
     auto *b = new (std::nothrow) mcac_box();
     if (!b) return MCAC_E_CAPACITY;
@@ -669,7 +669,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     ALLOC(d.part_d, b->part_cap); ALLOC(d.part_p, b->part_cap);
     ALLOC(d.dc, Ns);
     ALLOC(d.row_cnt, Ns); ALLOC(d.row_j, Ns * kOverlapSlots); ALLOC(d.row_c, Ns * kOverlapSlots);
-    if (params->volsurf == 1) {
+    if (params->volsurf == 1 || params->volsurf == 3) {
         ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.cvol, Ns); ALLOC(d.csurf, Ns);
     }
     if (params->volsurf == 2) {
@@ -1024,16 +1024,17 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     a.volsurf = b->p.volsurf;
     const int n = nk + nr;
     if (b->timing) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    if (a.volsurf == 1) {
-        // spherical caps: the two-way contact rows first, then the corrected
-        // sphere volumes, the full ordered sums, dc, rmax and Rg (publishes)
+    if (a.volsurf == 1 || a.volsurf == 3) {
+        // spherical caps or arvo: the two-way contact rows first, then the
+        // per-sphere parts, the full ordered sums, dc, rmax and Rg (publishes)
         const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
         const unsigned tiles = static_cast<unsigned>((n + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
         hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
         hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
         hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_caps_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
+        if (a.volsurf == 1) hipLaunchKernelGGL(k_caps_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
+        else hipLaunchKernelGGL(k_arvo_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
         hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
         hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
         hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
@@ -1272,6 +1273,19 @@ int mcac_release(mcac_box *b) {
 int mcac_box_set_auto_merge(mcac_box *b, int enable) {
     if (!b || b->inflight) return MCAC_E_ARGUMENT;
     b->automerge = (enable && b->p.volsurf == 0) ? 1 : 0;  // caps merges run their own kernel sequence
+    return 0;
+}
+
+int mcac_arvo_part(int k, const double *x, const double *y, const double *z, const double *r, double out[2]) {
+    if (!x || !y || !z || !r || !out || k < 0 || k > kArvoBalls) return MCAC_E_ARGUMENT;
+    double bx[kArvoBalls + 1], by[kArvoBalls + 1], bz[kArvoBalls + 1], br[kArvoBalls + 1];
+    for (int j = 0; j <= k; ++j) {
+        bx[j] = x[j];
+        by[j] = y[j];
+        bz[j] = z[j];
+        br[j] = r[j];
+    }
+    arvo_ball(bx, by, bz, br, k, &out[0], &out[1]);
     return 0;
 }
 

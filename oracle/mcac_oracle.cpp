@@ -33,7 +33,9 @@
 #include <array>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <list>
 #include <map>
@@ -269,7 +271,7 @@ struct Physics {
     size_t n_verlet_divisions = 10;
     size_t n_monomeres = 2500;
     int init_mode = LOGNORMAL;
-    int volsurf = 0;  // volsurf_method: 0 none, 1 caps, 2 alphas (constants.hpp:103-111)
+    int volsurf = 0;  // volsurf_method: 0 none, 1 caps, 2 alphas, 3 arvo (constants.hpp:103-111)
     int pick_random = 1;
     size_t n_iter_without_event = 0;
     double physical_time_limit = -1;
@@ -304,6 +306,7 @@ struct Physics {
         extract(ini, "surface_growth", "volsurf_method", s);
         if (s == "caps") volsurf = 1;
         else if (s == "alphas") volsurf = 2;
+        else if (s == "arvo") volsurf = 3;
         else if (!s.empty() && s != "none") unsupported = "volsurf_method=" + s;
         extract(ini, "nucleation", "with_nucleation", with_nucleation);
         extract(ini, "limits", "number_of_aggregates", number_of_aggregates_limit);
@@ -462,6 +465,226 @@ Intersection sphere_intersection(double radius_1, double vol_1, double surf_1, d
     return x;
 }
 
+// ------------------------------------------------------------------ ARVO (a12)
+// Restatement of the ARVO method the reference carries in arvo.cpp (Busa et
+// al., Comput. Phys. Commun. 165 (2005) 59-96, C version 2.0): the part of the
+// volume and area of a union of balls carried by one ball, from the
+// stereographic images (north pole -> plane tangent at the south pole) of its
+// neighbours' intersection circles, the boundary arcs of its exposed region,
+// and closed-form arc integrals (Green / divergence theorem with the field
+// (0, 0, z)).  The parts sum to the union's volume and area.  Fixes to the
+// reference (whose copy is never compiled, src/CMakeLists.txt:108-111):
+//  F1 positions relative to the aggregate's sphere 0, not wrapped absolute
+//     ones (arvo_mcac_call.cpp:28: the z-field split depends on the origin);
+//  F2 results in m^3 and m^2 (arvo_mcac_call.cpp:29-32 scales the input to nm
+//     and returns nm units that aggregat.cpp then sums as SI);
+//  F3 the sphere count is known before the spheres are loaded (arvo.cpp:903-904
+//     sets it after LoadProtein_external read it);
+//  F4 a ball inside a neighbour carries nothing and a neighbour inside the
+//     ball is not a neighbour (MakeNeighbors, arvo.cpp:858-876, flags neither);
+//     of two coincident equal balls the lower index keeps its part;
+//  F5 map entries that do not overlap by more than the contact tolerance
+//     (distance >= (1 - 1e-10)(r_i + r_j); the map keeps pairs up to
+//     (1 + 1e-10)(r_i + r_j)) are not neighbours: their image circle
+//     degenerates (a square root of a rounding-level negative, NaN) and their
+//     caps are far below one ulp of the ball's volume.
+// As in the reference the computation runs in nanometres, so that its
+// absolute tolerances (north-pole distance, angle merging) keep their scale.
+namespace arvo {
+const double kPiA = 3.14159265358979323846264;  // arvo.cpp:37
+const double kEpsNorthPole = 0.0001, kNorthPoleReduce = 0.9999;  // arvo.cpp:38-39
+const double kEpsAngle = 1e-12, kEpsTwoPi = 1e-12;               // arvo.cpp:41-42
+const double kVolumeFactor = 4 * kPiA / 3, kSurfaceFactor = 4 * kPiA;
+
+struct Ball {
+    double x, y, z, r;
+};
+struct ImageCircle {
+    double t, s, r, o;  // centre, radius, orientation (+1: the north pole is inside the neighbour)
+};
+
+// A neighbour whose surface passes within kEpsNorthPole of the ball's north
+// pole is shrunk until it does not (arvo.cpp:690-707).
+void north_pole_fix(std::vector<Ball> &b) {
+    const double nx = b[0].x, ny = b[0].y, nz = b[0].z + b[0].r;
+    for (size_t j = 1; j < b.size(); j++) {
+        for (;;) {
+            const double d = std::fabs(std::sqrt((nx - b[j].x) * (nx - b[j].x) + (ny - b[j].y) * (ny - b[j].y) +
+                                                 (nz - b[j].z) * (nz - b[j].z)) - b[j].r);
+            if (d >= kEpsNorthPole) break;
+            b[j].r = b[j].r * kNorthPoleReduce;
+        }
+    }
+}
+
+// Image of neighbour j's intersection circle (arvo.cpp:651-678), in plane
+// coordinates centred under the ball.
+ImageCircle image_circle(const Ball &c, const Ball &n) {
+    const double r1 = c.r, dx = c.x - n.x, dy = c.y - n.y;
+    const double zn = c.z + r1 - n.z, zs = c.z - r1 - n.z;
+    const double a = dx * dx + dy * dy + zn * zn - n.r * n.r;
+    const double b = 8.0 * r1 * r1 * dx, cc = 8.0 * r1 * r1 * dy;
+    const double d = 4.0 * r1 * r1 * (dx * dx + dy * dy + zs * zs - n.r * n.r);
+    return {-b / (2.0 * a), -cc / (2.0 * a), std::sqrt((b * b + cc * cc - 4.0 * a * d) / (4.0 * a * a)),
+            a > 0 ? -1.0 : 1.0};
+}
+
+// Is plane point (t, s) exposed with respect to circle m: inside a positive
+// circle or outside a negative one (arvo.cpp:298-315)?
+bool exposed(double t, double s, const ImageCircle &m) {
+    const double d = std::sqrt((t - m.t) * (t - m.t) + (s - m.s) * (s - m.s));
+    return d < m.r ? m.o > 0 : m.o < 0;
+}
+
+// Angles on circle a of its two intersection points with circle b, in [0, 2 pi).
+void crossing_angles(const ImageCircle &a, const ImageCircle &b, double &p, double &q) {
+    const double dx = b.t - a.t, dy = b.s - a.s;
+    const double d = std::sqrt(dx * dx + dy * dy);
+    const double x = (a.r * a.r - b.r * b.r + d * d) / (2.0 * d);  // along the centre line
+    const double h = std::sqrt(std::max(a.r * a.r - x * x, 0.0));
+    const double base = std::atan2(dy, dx), w = std::atan2(h, x);
+    p = base + w;
+    q = base - w;
+    while (p < 0) p += 2.0 * kPiA;
+    while (q < 0) q += 2.0 * kPiA;
+    while (p >= 2.0 * kPiA) p -= 2.0 * kPiA;
+    while (q >= 2.0 * kPiA) q -= 2.0 * kPiA;
+}
+
+// Closed-form arc integrals (arvo.cpp:149-217): volume and area parts of one
+// arc of circle c from angle `start` over the oriented angle `delta`.
+void arc_integral(const ImageCircle &c, double start, double delta, double r1, double z1, double &dv, double &da) {
+    const double A = (4.0 * r1 * r1 + c.t * c.t + c.s * c.s + c.r * c.r) / 2.0;
+    const double B = c.t * c.r, C = c.s * c.r;
+    const double S = std::sqrt(A * A - B * B - C * C);
+    const double rr = c.r * c.r - A;
+    double I1, I2, I3, J1;
+    if (std::fabs(std::fabs(delta) - 2.0 * kPiA) < kEpsTwoPi) {
+        I1 = 2.0 * kPiA / S;
+        I2 = 2.0 * kPiA * A / std::pow(S, 3.);
+        I3 = kPiA * (2.0 * A * A + B * B + C * C) / std::pow(S, 5.);
+        J1 = kPiA + rr / 2.0 * I1;
+    } else {
+        const double al = delta < 0 ? start + delta : start;
+        const double be = delta < 0 ? start : start + delta;
+        auto f = [&](double ang, double k) {
+            return (-B * std::sin(ang) + C * std::cos(ang)) / std::pow(A + B * std::cos(ang) + C * std::sin(ang), k);
+        };
+        I1 = 2.0 * (kPiA / 2.0 - std::atan((A * std::cos((be - al) / 2.0) + B * std::cos((al + be) / 2.0) +
+                                            C * std::sin((al + be) / 2.0)) / (S * std::sin((be - al) / 2.0)))) / S;
+        I2 = (f(be, 1) - f(al, 1) + A * I1) / (S * S);
+        I3 = (f(be, 2) - f(al, 2) + (f(be, 1) - f(al, 1)) / A + (2.0 * A * A + B * B + C * C) * I2 / A) / (2.0 * S * S);
+        J1 = ((be - al) + rr * I1) / 2.0;
+    }
+    const double J2 = (I1 + rr * I2) / 4.0, J3 = (I2 + rr * I3) / 8.0;
+    dv = (128.0 * J3 * std::pow(r1, 7.) + 8.0 * J2 * std::pow(r1, 5.) + 2.0 * J1 * std::pow(r1, 3.)) / 3.0 -
+         8.0 * std::pow(r1, 4.) * J2 * (z1 + r1);
+    da = 2.0 * J1 * r1 * r1;
+    if (delta < 0) {
+        dv = -dv;
+        da = -da;
+    }
+}
+
+// Part of ball b[0] given its neighbours b[1..] (nm; arvo.cpp:728-788 with
+// CirclesToArcs / NewArcs, :497-641, summing each arc as it is found).
+void ball_part(std::vector<Ball> b, double &vol, double &area) {
+    const double r1 = b[0].r;
+    vol = 0.0;
+    area = 0.0;
+    if (b.size() == 1) {
+        vol = kVolumeFactor * r1 * r1 * r1;
+        area = kSurfaceFactor * r1 * r1;
+        return;
+    }
+    north_pole_fix(b);
+    const size_t k = b.size() - 1;
+    std::vector<ImageCircle> c(k);
+    for (size_t j = 0; j < k; j++) c[j] = image_circle(b[0], b[j + 1]);
+    const double z1 = b[0].z;
+    double pv = 0.0, pa = 0.0, dv, da;
+    auto add = [&](size_t i, double start, double delta) {
+        arc_integral(c[i], start, delta, r1, z1, dv, da);
+        pv = pv + dv;
+        pa = pa + da;
+    };
+    for (size_t i = 0; i < k; i++) {
+        std::vector<double> ang;
+        for (size_t m = 0; m < k; m++) {
+            if (m == i) continue;
+            const double d = std::sqrt((c[i].t - c[m].t) * (c[i].t - c[m].t) + (c[i].s - c[m].s) * (c[i].s - c[m].s));
+            if (d < c[m].r + c[i].r && std::fabs(c[m].r - c[i].r) < d) {
+                double p, q;
+                crossing_angles(c[i], c[m], p, q);
+                ang.push_back(p);
+                ang.push_back(q);
+            }
+        }
+        if (ang.empty()) {  // the whole circle, if it lies in every other exposed region (arvo.cpp:526-548)
+            bool all = true;
+            for (size_t m = 0; m < k && all; m++) {
+                if (m == i) continue;
+                double pt = c[i].t + c[i].r, ps = c[i].s;
+                const double d = std::sqrt((pt - c[m].t) * (pt - c[m].t) + (ps - c[m].s) * (ps - c[m].s));
+                if (d == c[m].r) pt = c[i].t;  // that point lies on circle m: test the centre
+                all = exposed(pt, ps, c[m]);
+            }
+            if (all) add(i, 0.0, 2.0 * kPiA * c[i].o);
+            continue;
+        }
+        if (c[i].o > 0) std::sort(ang.begin(), ang.end());
+        else std::sort(ang.begin(), ang.end(), std::greater<double>());
+        std::vector<double> u{ang[0]};  // merge angles closer than kEpsAngle (arvo.cpp:219-242)
+        for (size_t q = 1; q < ang.size(); q++)
+            if (std::fabs(ang[q] - u.back()) > kEpsAngle) u.push_back(ang[q]);
+        auto mid_exposed = [&](double mid) {
+            const double pt = c[i].t + c[i].r * std::cos(mid), ps = c[i].s + c[i].r * std::sin(mid);
+            for (size_t m = 0; m < k; m++)
+                if (m != i && !exposed(pt, ps, c[m])) return false;
+            return true;
+        };
+        for (size_t q = 0; q + 1 < u.size(); q++)
+            if (mid_exposed((u[q] + u[q + 1]) / 2.0)) add(i, u[q], u[q + 1] - u[q]);
+        if (mid_exposed((u[0] + 2.0 * kPiA + u.back()) / 2.0))
+            add(i, u.back(), u[0] + c[i].o * 2.0 * kPiA - u.back());
+    }
+    bool positive = false;
+    for (const auto &ci : c) positive = positive || ci.o > 0;
+    vol = pv;
+    area = pa;
+    if (!positive) {  // every circle negative: the arcs bound the covered part
+        vol = vol + kVolumeFactor * r1 * r1 * r1;
+        area = area + kSurfaceFactor * r1 * r1;
+    }
+}
+
+// Parts of every ball of an aggregate (positions relative to its sphere 0,
+// metres) from its contact map, neighbours in the map's order (F1-F5).
+void aggregate_parts(const std::vector<V3> &rel, const std::vector<double> &rad,
+                     const std::vector<std::unordered_map<size_t, double>> &dist, std::vector<double> &vol,
+                     std::vector<double> &area) {
+    const size_t n = rel.size();
+    vol.assign(n, 0.0);
+    area.assign(n, 0.0);
+    const double nm = 1e+09;
+    for (size_t i = 0; i < n; i++) {
+        std::vector<Ball> b{{rel[i][0] * nm, rel[i][1] * nm, rel[i][2] * nm, rad[i] * nm}};
+        bool inside = false;
+        for (const auto &[j, d] : dist[i]) {
+            if (d >= (1. - kCoordinationEpsilon) * (rad[i] + rad[j])) continue;          // F5
+            if (d + rad[i] < rad[j] || (d + rad[i] == rad[j] && j < i)) inside = true;  // F4
+            if (d + rad[j] < rad[i] || (d + rad[j] == rad[i] && i < j)) continue;
+            b.push_back({rel[j][0] * nm, rel[j][1] * nm, rel[j][2] * nm, rad[j] * nm});
+        }
+        if (inside) continue;
+        double v, a;
+        ball_part(b, v, a);
+        vol[i] = v * 1e-27;  // F2
+        area[i] = a * 1e-18;
+    }
+}
+}  // namespace arvo
+
 struct Agg {
     double rg = 0, f_agg = 0, lpm = 0, time_step = 0, rmax = 0, volume = 0, surface = 0;
     V3 pos{0, 0, 0}, rcm{0, 0, 0};
@@ -608,9 +831,19 @@ struct Sim {
         const size_t n = a.sph.size();
         a.volumes.resize(n);
         a.surfaces.resize(n);
-        for (size_t i = 0; i < n; i++) {
-            a.volumes[i] = svol[a.sph[i]];
-            a.surfaces[i] = ssurf[a.sph[i]];
+        if (pm.volsurf == 3) {  // :337-347, the restated ARVO (fixes F1-F5 above)
+            std::vector<V3> rel(n);
+            std::vector<double> rad(n);
+            for (size_t i = 0; i < n; i++) {
+                rel[i] = srel(a.sph[i]);
+                rad[i] = sr[a.sph[i]];
+            }
+            arvo::aggregate_parts(rel, rad, a.distances, a.volumes, a.surfaces);
+        } else {
+            for (size_t i = 0; i < n; i++) {
+                a.volumes[i] = svol[a.sph[i]];
+                a.surfaces[i] = ssurf[a.sph[i]];
+            }
         }
         if (pm.volsurf == 1) {  // :354-386, the map walked in its own iteration order
             for (size_t i = 0; i < n; i++) {
@@ -1171,7 +1404,8 @@ int64_t ora_run(void *sim, int64_t max_steps) {
             done++;
         }
         if (s->finished()) s->finished_flag = true;
-    } catch (const std::exception &) {
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle: %s at step %lld\n", e.what(), static_cast<long long>(s->step_count));
         return -1;
     }
     return done;
@@ -1344,6 +1578,11 @@ int ora_merge_geometry(int volsurf, int64_t nk, int64_t n, const double *rx, con
     const double v[11] = {a.volume, a.surface, a.rg, a.rmax, c[0], c[1], c[2], a.overlapping, a.coordination, raw, rs};
     for (int q = 0; q < 11; q++) out[q] = v[q];
     return 0;
+}
+void ora_arvo_part(int k, const double *x, const double *y, const double *z, const double *r, double out[2]) {
+    std::vector<arvo::Ball> b(static_cast<size_t>(k) + 1);
+    for (int j = 0; j <= k; j++) b[static_cast<size_t>(j)] = {x[j], y[j], z[j], r[j]};
+    arvo::ball_part(b, out[0], out[1]);
 }
 int ora_neighbour_order(int prev, const int64_t *keys, int m, int64_t *out) {
     // aggregat.cpp:730-751 on one map: the previous update's prev entries,

@@ -127,11 +127,15 @@ void ora_caps_intersection(double r1, double vol1, double surf1, double r2, doub
                            double out[4]);
 /* Geometry of Aggregate::update (aggregat.cpp:284-288, 431-483) for n spheres
  * given by their positions relative to sphere 0, of which the first nk were
- * last updated on their own (the keeper of a merge): volsurf 0 none, 1 caps.
+ * last updated on their own (the keeper of a merge): volsurf 0 none, 1 caps,
+ * 2 alphas, 3 arvo.
  * out = {volume, surface, rg, rmax, rcm x, y, z, overlapping, coordination,
  * sum of sphere volumes, sum of radii}; returns 0, or < 0 (bad sizes, VolSurfError). */
 int ora_merge_geometry(int volsurf, int64_t nk, int64_t n, const double *rx, const double *ry, const double *rz,
                        const double *r, const double *vol, const double *surf, double out[11]);
+/* ARVO part (arvo.cpp:728-788 restated) of ball 0 among k + 1 balls (nm):
+ * out = {volume, area}. */
+void ora_arvo_part(int k, const double *x, const double *y, const double *z, const double *r, double out[2]);
 /* Iteration order of a std::unordered_map<size_t,double> that held prev
  * entries, was cleared, reserved at its old size and given keys[0..m) in
  * order (aggregat.cpp:730-751); returns the number of keys written. */

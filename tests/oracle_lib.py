@@ -83,6 +83,8 @@ def lib():
         L.ora_caps_intersection.restype = None
         L.ora_merge_geometry.argtypes = [ctypes.c_int, _I64, _I64] + [_DP] * 6 + [_DP]
         L.ora_neighbour_order.argtypes = [ctypes.c_int, _I64P, ctypes.c_int, _I64P]
+        L.ora_arvo_part.argtypes = [ctypes.c_int, _DP, _DP, _DP, _DP, _DP]
+        L.ora_arvo_part.restype = None
         _lib = L
     return _lib
 
@@ -201,6 +203,15 @@ def merge_geometry(volsurf, nk, rel, r, vol, surf):
         raise RuntimeError(f"ora_merge_geometry: {rc}")
     return dict(volume=out[0], surface=out[1], rg=out[2], rmax=out[3], rcm=out[4:7].copy(), overlapping=out[7],
                 coordination=out[8], sphere_volume=out[9], radius_sum=out[10])
+
+
+def arvo_part(balls):
+    """(volume, area) part of ball 0 among balls (rows x, y, z, r in nm), restated ARVO."""
+    b = np.ascontiguousarray(balls, dtype=np.float64).reshape(-1, 4)
+    cols = [np.ascontiguousarray(b[:, q]) for q in range(4)]
+    out = np.zeros(2)
+    lib().ora_arvo_part(len(b) - 1, *(dp(c) for c in cols), dp(out))
+    return float(out[0]), float(out[1])
 
 
 def neighbour_order(prev, keys):
