@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
@@ -9,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -47,7 +49,13 @@ inline double half_spacing(double x) {
 }
 
 // Up to 8 host threads kept for the re-sum after a change of the largest time
-// step; the calling thread takes part.  Jobs are claimed by an atomic counter.
+// step and for the partitions of a tie replay; the calling thread takes part.
+// Each call is a task whose jobs are claimed by an atomic counter; the call
+// returns once every job has completed, without waiting for idle threads to
+// check in (a worker that wakes late finds the counter exhausted, and its
+// reference to the task keeps the task alive).  Workers spin for kSpin after
+// a task before they sleep, and so does the caller before it sleeps on the
+// completion count.
 class Workers {
   public:
     static Workers &get() {
@@ -62,58 +70,77 @@ class Workers {
             for (size_t k = 0; k < n; ++k) fn(k);
             return;
         }
-        std::unique_lock<std::mutex> lk(m_);
-        job_ = [&fn](size_t k) { fn(k); };
-        n_ = n;
-        next_.store(0);
-        busy_ = threads_.size();
-        ++gen_;
-        lk.unlock();
+        auto task = std::make_shared<Task>();
+        task->job = [&fn](size_t k) { fn(k); };
+        task->n = n;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            task_ = task;
+            gen_.fetch_add(1, std::memory_order_release);
+        }
         cv_.notify_all();
-        drain();
-        lk.lock();
-        done_.wait(lk, [this] { return busy_ == 0; });
-        job_ = nullptr;
+        work(*task);
+        const auto t0 = std::chrono::steady_clock::now();
+        while (task->done.load(std::memory_order_acquire) != n && std::chrono::steady_clock::now() - t0 < kSpin) {
+        }
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return task->done.load() == n; });
+        task_.reset();
     }
     ~Workers() {
         {
             std::lock_guard<std::mutex> lk(m_);
             stop_ = true;
-            ++gen_;
+            gen_.fetch_add(1, std::memory_order_release);
         }
         cv_.notify_all();
         for (auto &t : threads_) t.join();
     }
 
   private:
+    struct Task {
+        std::function<void(size_t)> job;
+        size_t n = 0;
+        std::atomic<size_t> next{0}, done{0};
+    };
+    static constexpr std::chrono::microseconds kSpin{300};
     Workers() {
         const size_t T = std::min<size_t>(8, std::max(1u, std::thread::hardware_concurrency()));
         for (size_t t = 1; t < T; ++t) threads_.emplace_back([this] { loop(); });
     }
-    void drain() {
-        for (size_t k = next_.fetch_add(1); k < n_; k = next_.fetch_add(1)) job_(k);
+    // claims and runs jobs of the task; the thread that completes the last one wakes the caller
+    void work(Task &task) {
+        for (size_t k = task.next.fetch_add(1); k < task.n; k = task.next.fetch_add(1)) {
+            task.job(k);
+            if (task.done.fetch_add(1) + 1 == task.n) {
+                std::lock_guard<std::mutex> lk(m_);
+                done_.notify_all();
+            }
+        }
     }
     void loop() {
         uint64_t seen = 0;
-        std::unique_lock<std::mutex> lk(m_);
         while (true) {
-            cv_.wait(lk, [&] { return gen_ != seen; });
-            seen = gen_;
-            if (stop_) return;
-            lk.unlock();
-            drain();
-            lk.lock();
-            if (--busy_ == 0) done_.notify_one();
+            const auto t0 = std::chrono::steady_clock::now();
+            while (gen_.load(std::memory_order_acquire) == seen && std::chrono::steady_clock::now() - t0 < kSpin) {
+            }
+            std::shared_ptr<Task> task;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return gen_.load() != seen; });
+                seen = gen_.load();
+                if (stop_) return;
+                task = task_;
+            }
+            if (task) work(*task);
         }
     }
     std::vector<std::thread> threads_;
     std::mutex user_;  // held by the thread whose job the pool runs
     std::mutex m_;
     std::condition_variable cv_, done_;
-    std::function<void(size_t)> job_;
-    size_t n_ = 0, busy_ = 0;
-    std::atomic<size_t> next_{0};
-    uint64_t gen_ = 0;
+    std::shared_ptr<Task> task_;
+    std::atomic<uint64_t> gen_{0};
     bool stop_ = false;
 };
 
@@ -489,55 +516,58 @@ inline size_t partition_pivot(KeyLabel *a, size_t f, size_t l) {
 // L_t with R_t for every t before the first t* with L_t* >= R_t*, and returns
 // min(L_t*, R_{t*-1}) (R_0 = l, L_t = +inf past the last L): while L_t < R_t
 // the scans meet only positions no swap has touched yet, and the swapped-in
-// elements stop them exactly where the sequential scans stop.
+// elements stop them exactly where the sequential scans stop.  The L and R
+// positions are not listed: one pass counts them per sub-block of kSub
+// entries, L_t and R_t are found from those prefix counts and a scan of one
+// sub-block, and each chunk of swaps starts from its first L and R position
+// (found before any swap) and walks on by the keys, which no other chunk
+// touches (every L_t with t < t* lies before every such R_t), gathering its
+// positions first and swapping them after.
 constexpr size_t kParallelPartition = size_t(1) << 16;
 constexpr size_t kPartitionBlock = size_t(1) << 14;
+constexpr size_t kSub = 256;  // entries per counted sub-block (kPartitionBlock is a multiple)
 
-size_t partition_pivot_parallel(KeyLabel *a, size_t f, size_t l, std::vector<uint32_t> &Lp,
-                                std::vector<uint32_t> &Rp) {
+size_t partition_pivot_parallel(KeyLabel *a, size_t f, size_t l, std::vector<uint32_t> &cl,
+                                std::vector<uint32_t> &cr) {
     move_median_to_first(a, f, f + 1, f + (l - f) / 2, l - 1);
     const double p = a[f].k;
     const size_t n = l - f;
-    const size_t nb = (n + kPartitionBlock - 1) / kPartitionBlock;
-    std::vector<size_t> cl(nb + 1, 0), cr(nb + 1, 0);
+    const size_t ns = (n + kSub - 1) / kSub;
+    cl.assign(ns + 1, 0);  // cl[s + 1]: L positions in sub-block s, then prefix sums
+    cr.assign(ns + 1, 0);
     auto count = [&](size_t b) {
-        const size_t s = f + b * kPartitionBlock, e = std::min(l, s + kPartitionBlock);
-        size_t x = 0, y = 0;
-        for (size_t j = s; j < e; ++j) {
-            x += (a[j].k < p) ? 0 : 1;
-            y += (p < a[j].k) ? 0 : 1;
+        const size_t s0 = b * (kPartitionBlock / kSub), s1 = std::min(ns, s0 + kPartitionBlock / kSub);
+        for (size_t s = s0; s < s1; ++s) {
+            const size_t j0 = f + s * kSub, j1 = std::min(l, j0 + kSub);
+            uint32_t x = 0, y = 0;
+            for (size_t j = j0; j < j1; ++j) {
+                x += (a[j].k < p) ? 0u : 1u;
+                y += (p < a[j].k) ? 0u : 1u;
+            }
+            if (s == 0) x -= 1;  // the pivot's own position is not an L position
+            cl[s + 1] = x;
+            cr[s + 1] = y;
         }
-        if (b == 0) x -= 1;  // the pivot's own position is not an L position
-        cl[b + 1] = x;
-        cr[b + 1] = y;
     };
-    Workers::get().run(nb, count);
-    for (size_t b = 0; b < nb; ++b) {
-        cl[b + 1] += cl[b];
-        cr[b + 1] += cr[b];
+    Workers::get().run((n + kPartitionBlock - 1) / kPartitionBlock, count);
+    for (size_t s = 0; s < ns; ++s) {
+        cl[s + 1] += cl[s];
+        cr[s + 1] += cr[s];
     }
-    const size_t nl = cl[nb], nr = cr[nb];
-    Lp.resize(nl);
-    Rp.resize(nr);
-    auto list = [&](size_t b) {  // ascending positions relative to f, branch-free through a block buffer
-        thread_local std::vector<uint32_t> bl, br;
-        bl.resize(kPartitionBlock + 1);
-        br.resize(kPartitionBlock + 1);
-        const size_t s = f + b * kPartitionBlock, e = std::min(l, s + kPartitionBlock);
-        size_t x = 0, y = 0;
-        for (size_t j = s; j < e; ++j) {
-            const uint32_t r = static_cast<uint32_t>(j - f);
-            bl[x] = r;
-            x += (a[j].k < p || j == f) ? 0 : 1;
-            br[y] = r;
-            y += (p < a[j].k) ? 0 : 1;
+    const size_t nl = cl[ns], nr = cr[ns];
+    // u-th (1-based, from the left) L or R position, relative to f
+    auto nth = [&](const std::vector<uint32_t> &c, size_t u, bool left) {
+        const size_t s = static_cast<size_t>(std::upper_bound(c.begin(), c.end(), static_cast<uint32_t>(u - 1)) -
+                                             c.begin()) - 1;
+        size_t k = c[s];
+        for (size_t j = s * kSub;; ++j) {
+            const bool hit = left ? (j != 0 && !(a[f + j].k < p)) : !(p < a[f + j].k);
+            if (hit && ++k == u) return j;
         }
-        std::copy(bl.begin(), bl.begin() + static_cast<std::ptrdiff_t>(x), Lp.begin() + static_cast<std::ptrdiff_t>(cl[b]));
-        std::copy(br.begin(), br.begin() + static_cast<std::ptrdiff_t>(y), Rp.begin() + static_cast<std::ptrdiff_t>(cr[b]));
     };
-    Workers::get().run(nb, list);
-    // L_t = Lp[t-1], R_t = Rp[nr-t]; first t with L_t >= R_t (nr >= 1: f qualifies)
-    auto crossed = [&](size_t t) { return t > nl || Lp[t - 1] >= Rp[nr - t]; };
+    auto L = [&](size_t t) { return nth(cl, t, true); };
+    auto R = [&](size_t t) { return nth(cr, nr - t + 1, false); };
+    auto crossed = [&](size_t t) { return t > nl || L(t) >= R(t); };
     size_t lo = 1, hi = nr;  // crossed(nr) holds: R_nr = f < any L
     while (lo < hi) {
         const size_t mid = lo + (hi - lo) / 2;
@@ -546,14 +576,31 @@ size_t partition_pivot_parallel(KeyLabel *a, size_t f, size_t l, std::vector<uin
     }
     const size_t ts = lo;  // t*
     const size_t nsw = ts - 1;
-    auto swaps = [&](size_t b) {
-        const size_t s = b * kPartitionBlock, e = std::min(nsw, s + kPartitionBlock);
-        for (size_t t = s + 1; t <= e; ++t) std::swap(a[f + Lp[t - 1]], a[f + Rp[nr - t]]);
+    const size_t cut = std::min(ts > nl ? l : f + L(ts), ts == 1 ? l : f + R(ts - 1));
+    const size_t nc = (nsw + kPartitionBlock - 1) / kPartitionBlock;
+    std::vector<size_t> sl(nc), sr(nc);
+    for (size_t c = 0; c < nc; ++c) {  // first L and R position of every chunk, before any swap
+        sl[c] = L(c * kPartitionBlock + 1);
+        sr[c] = R(c * kPartitionBlock + 1);
+    }
+    auto swaps = [&](size_t c) {  // the chunk's L and R positions gathered branch-free, then swapped
+        thread_local std::vector<size_t> bl, br;
+        const size_t m = std::min(nsw, (c + 1) * kPartitionBlock) - c * kPartitionBlock;
+        bl.resize(m + 1);
+        br.resize(m + 1);
+        size_t x = 0, y = 0;
+        for (size_t j = f + sl[c]; x < m; ++j) {
+            bl[x] = j;
+            x += (a[j].k < p) ? 0 : 1;
+        }
+        for (size_t j = f + sr[c]; y < m; --j) {
+            br[y] = j;
+            y += (p < a[j].k) ? 0 : 1;
+        }
+        for (size_t q = 0; q < m; ++q) std::swap(a[bl[q]], a[br[q]]);
     };
-    Workers::get().run((nsw + kPartitionBlock - 1) / kPartitionBlock, swaps);
-    const size_t Lts = ts > nl ? l : f + Lp[ts - 1];
-    const size_t Rprev = ts == 1 ? l : f + Rp[nr - (ts - 1)];
-    return std::min(Lts, Rprev);
+    Workers::get().run(nc, swaps);
+    return cut;
 }
 
 int floor_log2(size_t n) {
