@@ -583,12 +583,16 @@ __global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b, int both) {
 }
 
 // Ordered overlap sum in (i, j) order; clears the row counters; publishes.
-// With spherical caps the rows hold both directions (pairs with j > i are
-// summed) and stay for k_caps_volumes, which clears them; no publication.
+// Each thread sorts its row by j in registers and stages its coefficients
+// toward higher j in LDS, from which thread 0 adds them in order.  With an
+// overlap correction (volsurf != 0) the rows hold both directions (pairs with
+// j > i are summed) and stay, sorted, for the per-sphere kernel, which clears
+// them; no publication.
 __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
     MergeState *m = b.ms;
     if (!m->ok) return;
     const int n = m->n_k + m->n_r;
+    __shared__ double tc[kBlock * kOverlapSlots];  // each row's coefficients toward higher j, in order
     __shared__ int tcnt[kBlock];
     double overlapping = 0.0;
     long long inter = 0;
@@ -599,31 +603,54 @@ __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
             int rc = b.row_cnt[i];
             if (rc > kOverlapSlots) rc = kOverlapSlots;
             const long long base = static_cast<long long>(i) * kOverlapSlots;
-            for (int x = 1; x < rc; ++x) {  // insertion sort of the row by j
-                int kj = b.row_j[base + x];
-                double kc = b.row_c[base + x];
-                double kd = a.volsurf ? b.row_d[base + x] : 0.0;
-                int y = x - 1;
-                while (y >= 0 && b.row_j[base + y] > kj) {
-                    b.row_j[base + y + 1] = b.row_j[base + y];
-                    b.row_c[base + y + 1] = b.row_c[base + y];
-                    if (a.volsurf) b.row_d[base + y + 1] = b.row_d[base + y];
-                    --y;
+            // the row in registers (unused slots sort last), a bitonic network by j
+            int kj[kOverlapSlots];
+            double kc[kOverlapSlots], kd[kOverlapSlots];
+#pragma unroll
+            for (int x = 0; x < kOverlapSlots; ++x) {
+                kj[x] = x < rc ? b.row_j[base + x] : 0x7fffffff;
+                kc[x] = x < rc ? b.row_c[base + x] : 0.0;
+                kd[x] = (x < rc && a.volsurf) ? b.row_d[base + x] : 0.0;
+            }
+#pragma unroll
+            for (int k = 2; k <= kOverlapSlots; k <<= 1) {
+#pragma unroll
+                for (int s = k >> 1; s > 0; s >>= 1) {
+#pragma unroll
+                    for (int x = 0; x < kOverlapSlots; ++x) {
+                        const int y = x ^ s;
+                        if (y > x && ((kj[x] > kj[y]) == ((x & k) == 0))) {
+                            const int tj = kj[x];
+                            kj[x] = kj[y];
+                            kj[y] = tj;
+                            const double tcv = kc[x];
+                            kc[x] = kc[y];
+                            kc[y] = tcv;
+                            const double td = kd[x];
+                            kd[x] = kd[y];
+                            kd[y] = td;
+                        }
+                    }
                 }
-                b.row_j[base + y + 1] = kj;
-                b.row_c[base + y + 1] = kc;
-                if (a.volsurf) b.row_d[base + y + 1] = kd;
+            }
+            int hi = 0;
+#pragma unroll
+            for (int x = 0; x < kOverlapSlots; ++x) {
+                if (x < rc && a.volsurf) {  // the sorted two-way row for the per-sphere kernels
+                    b.row_j[base + x] = kj[x];
+                    b.row_c[base + x] = kc[x];
+                    b.row_d[base + x] = kd[x];
+                }
+                if (x < rc && kj[x] > i) tc[threadIdx.x * kOverlapSlots + hi++] = kc[x];  // j > i only
             }
             if (!a.volsurf) b.row_cnt[i] = 0;
-            tcnt[threadIdx.x] = rc;
+            tcnt[threadIdx.x] = hi;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int x = 0; x < cnt; ++x) {
-                const long long base = static_cast<long long>(t0 + x) * kOverlapSlots;
                 for (int y = 0; y < tcnt[x]; ++y) {
-                    if (b.row_j[base + y] <= t0 + x) continue;  // the lower half of a two-way row
-                    overlapping += 2.0 * b.row_c[base + y];
+                    overlapping += 2.0 * tc[x * kOverlapSlots + y];
                     inter += 2;
                 }
             }
