@@ -70,6 +70,7 @@ def _pct(x, q):
 
 def run_ensemble(args) -> dict:
     """Batched form: the first K boxes of the 512-box list, W boxes at a time on the GPU."""
+    import numpy as np
     from mcac_amd import ensemble
     boxes = ensemble.ensemble_configs(args.ensemble_boxes)
     print(f"bench: ensemble of {len(boxes)} boxes, {args.workers} workers", file=sys.stderr, flush=True)
@@ -79,12 +80,23 @@ def run_ensemble(args) -> dict:
     wall = time.perf_counter() - t0
     done = [j for j in range(len(boxes)) if r["steps"][j] < args.max_job_steps]
     dur = (r["end"] - r["start"])[done]
-    return {"boxes": len(boxes), "of": 512, "completed": len(done), "workers": args.workers,
-            "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
-            "runs_per_hour": 3600.0 * len(done) / wall, "mc_steps_per_s": float(r["steps"].sum()) / wall,
-            "box_time_p50_s": _pct(dur, 50), "box_time_p99_s": _pct(dur, 99),
-            "sample": f"boxes 0..{len(boxes) - 1} of the 512-box list (N log-uniform 500..5e4, normal 30/3 nm, "
-                      f"FV U[1e-4,5e-3], T U[1000,2000] K, until N_agg <= N/10), all started at t=0"}
+    out = {"boxes": len(boxes), "of": 512, "completed": len(done), "workers": args.workers,
+           "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
+           "runs_per_hour": 3600.0 * len(done) / wall, "mc_steps_per_s": float(r["steps"].sum()) / wall,
+           "box_time_p50_s": _pct(dur, 50), "box_time_p99_s": _pct(dur, 99)}
+    if args.plain_ensemble:
+        # the plain GPU path one run at a time: one worker, one box, the single-box pipeline
+        print("bench: the same boxes one at a time", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        p = ensemble.batch_run([b["ini"] for b in boxes], workers=1, max_steps=args.max_job_steps)
+        pw = time.perf_counter() - t0
+        same = bool(np.array_equal(p["steps"], r["steps"]))
+        out["plain_one_at_a_time"] = {"wall_s": pw, "runs_per_hour": 3600.0 * len(done) / pw,
+                                      "same_step_counts": same}
+        out["vs_plain_one_at_a_time"] = pw / wall
+    return out | {
+        "sample": f"boxes 0..{len(boxes) - 1} of the 512-box list (N log-uniform 500..5e4, normal 30/3 nm, "
+                  f"FV U[1e-4,5e-3], T U[1000,2000] K, until N_agg <= N/10), all started at t=0"}
 
 
 def run_ensemble_compare(args) -> dict:
@@ -135,12 +147,30 @@ def run_serving(args) -> dict:
     wall = time.perf_counter() - t0
     arrival = np.array([q["arrival"] for q in reqs])
     ttr = r["end"] - arrival
-    return {"requests": len(reqs), "rate_per_s": args.serving_rate, "workers": args.workers,
-            "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
-            "time_to_result_p50_s": _pct(ttr, 50), "time_to_result_p99_s": _pct(ttr, 99),
-            "runs_per_hour": 3600.0 * len(reqs) / wall,
-            "sample": f"N log-uniform 1e3..{args.serving_nmax:g} (full form 2e5), lognormal 30/1.25, FV U[1e-4,5e-3], "
-                      f"T U[1200,2000] K, until N_agg <= N/10, no surface reactions"}
+    out = {"requests": len(reqs), "rate_per_s": args.serving_rate, "workers": args.workers,
+           "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
+           "time_to_result_p50_s": _pct(ttr, 50), "time_to_result_p99_s": _pct(ttr, 99),
+           "runs_per_hour": 3600.0 * len(reqs) / wall}
+    if args.plain_ensemble:
+        # the plain GPU path as a queue: each request's own run time (measured one at a time),
+        # served first come first served on the same arrival times
+        t1 = time.perf_counter()
+        p = ensemble.batch_run([q["ini"] for q in reqs], workers=1, max_steps=args.max_job_steps)
+        pw = time.perf_counter() - t1
+        service = p["end"] - p["start"]
+        finish, free = np.zeros(len(reqs)), 0.0
+        for j in np.argsort(arrival, kind="stable"):
+            free = max(free, arrival[j]) + service[j]
+            finish[j] = free
+        pt = finish - arrival
+        out["plain_queue"] = {"service_s_total": float(service.sum()), "wall_s": pw,
+                              "time_to_result_p50_s": _pct(pt, 50), "time_to_result_p99_s": _pct(pt, 99),
+                              "runs_per_hour": 3600.0 * len(reqs) / max(float(finish.max()), 1e-9),
+                              "same_step_counts": bool(np.array_equal(p["steps"], r["steps"])),
+                              "note": "first-come-first-served over the measured one-at-a-time run times"}
+    return out | {
+        "sample": f"N log-uniform 1e3..{args.serving_nmax:g} (full form 2e5), lognormal 30/1.25, FV U[1e-4,5e-3], "
+                  f"T U[1200,2000] K, until N_agg <= N/10, no surface reactions"}
 
 
 def main() -> int:
@@ -154,6 +184,8 @@ def main() -> int:
                     help="only run a plain 256 MiB device copy 20 times (HBM counter calibration)")
     ap.add_argument("--ensemble-boxes", type=int, default=16,
                     help="boxes of the 512-box ensemble list run to completion (0: skip)")
+    ap.add_argument("--no-plain-ensemble", dest="plain_ensemble", action="store_false",
+                    help="skip running the ensemble sample again one box at a time")
     ap.add_argument("--compare-nmax", type=float, default=8000,
                     help="largest box of the ensemble sample also run on the CPU for the runs/hour comparison")
     ap.add_argument("--serving-requests", type=int, default=16, help="serving requests (0: skip)")
