@@ -38,6 +38,24 @@ def test_group_runs_match_oracle_runs(depth):
     sys.stdout.flush()
 
 
+def test_group_runs_with_overlap_corrections():
+    """Boxes with each overlap correction (caps, alphas, arvo) stepped in groups
+    beside a plain box: every run as the oracle's (the corrections merge through
+    their own kernel sequences, alphas with a host phase, on the group stream)."""
+    print("case: 4 boxes (none, caps, alphas, arvo), 2 workers x 2 boxes, depth 8", flush=True)
+    inis = [ensemble.box_ini(800, 3e-3, 1500.0, 31)]
+    inis += [(CFG / f"dense_small_{m}.ini").read_text() for m in ("caps", "alphas", "arvo")]
+    res = ensemble.batch_run(inis, workers=2, boxes_per_worker=2, depth=8)
+    assert np.all(res["status"] == 0)
+    for j, ini in enumerate(inis):
+        ora = O.OracleSim(ini, fast_init=True, trace=False)
+        n = ora.run(10 ** 8)
+        assert ora.finished()
+        print(f"  job {j}: {n} steps", flush=True)
+        assert res["steps"][j] == n, (j, res["steps"][j], n)
+        assert res["final_time"][j] == ora.scalar("time"), (j, res["final_time"][j], ora.scalar("time"))
+
+
 def test_group_enqueue_rejects_interleaved_moves():
     """A box's moves must be contiguous in one group enqueue, boxes must belong
     to the group, and n is bounded by 8 moves per box of the group's capacity."""
