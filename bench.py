@@ -192,6 +192,8 @@ def main() -> int:
     ap.add_argument("--serving-rate", type=float, default=4.0, help="serving arrival rate, requests/s")
     ap.add_argument("--serving-nmax", type=float, default=2e4,
                     help="largest request size in the serving sample (the full form allows 2e5)")
+    ap.add_argument("--tuned", default="",
+                    help="shape file of python3 -m mcac_amd.autotune: its workers, boxes per worker and depth")
     ap.add_argument("--workers", type=int, default=4, help="host threads sharing the GPU (ensemble, serving)")
     ap.add_argument("--group-boxes", type=int, default=4,
                     help="boxes per worker stepped in lock step, one launch per round (0: one box per worker)")
@@ -204,6 +206,10 @@ def main() -> int:
     ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
     args = ap.parse_args()
+    if args.tuned:
+        from mcac_amd.autotune import best_shape
+        shape = best_shape(args.tuned)
+        args.workers, args.group_boxes, args.group_depth = shape["workers"], shape["boxes_per_worker"], shape["depth"]
     if args.gpus != 1:
         print("bench.py measures one GPU: this path runs one aggregation box per GPU "
               "(independent boxes shard across GPUs with no exchange); use --gpus 1", file=sys.stderr)

@@ -56,6 +56,17 @@ def test_group_runs_with_overlap_corrections():
         assert res["final_time"][j] == ora.scalar("time"), (j, res["final_time"][j], ora.scalar("time"))
 
 
+def test_autotune_shapes_keep_results():
+    """The launch-shape tuner runs each candidate on the same boxes; every shape
+    gives every box the same MC step count, and the choice is one of them."""
+    from mcac_amd import autotune
+    print("case: autotune on 3 small boxes, 4 shapes", flush=True)
+    inis = [ensemble.box_ini(n, 2e-3, 1500.0, 40 + n) for n in (500, 800, 1200)]
+    res = autotune.tune_groups(inis, workers=(1, 2), boxes_per_worker=(1, 2), depth=(8,))
+    assert len(res["candidates"]) == 4 and all(r["same_step_counts"] for r in res["candidates"])
+    assert res["best"] in [{k: r[k] for k in ("workers", "boxes_per_worker", "depth")} for r in res["candidates"]]
+
+
 def test_group_enqueue_rejects_interleaved_moves():
     """A box's moves must be contiguous in one group enqueue, boxes must belong
     to the group, and n is bounded by 8 moves per box of the group's capacity."""
