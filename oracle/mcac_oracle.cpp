@@ -21,6 +21,12 @@
  *   aggregate-pair argmin (a8), ordered scan (a7) -> brute force on tiny inputs.
  *   merge / update (a10, a11)                     -> closed forms (two touching
  *       spheres: CoM at midpoint, rmax = 2r, Rg^2 = r^2 + 3/5 r^2, V = 2 v).
+ *   overlap corrections (a11 caps / alphas)       -> lens volume and cap areas;
+ *       the alphas fits against the exact cap fractions of two equal spheres
+ *       (its loop-only coefficients: transcription, partly unpinned); the
+ *       contact maps are real std::unordered_map objects.
+ *   ARVO (a12, restated with fixes F1-F5)         -> lens / caps closed forms,
+ *       contained and disjoint spheres, Monte Carlo union volume and areas.
  *   full loop (a1)                                -> invariants (sphere count,
  *       total volume, merge count) and the product's bit-exact agreement.
  * The parts of the reference that need inipp.h / <gsl/gsl> / libunwind.h
