@@ -142,15 +142,18 @@ def run_serving(args) -> dict:
     reqs = ensemble.serving_requests(args.serving_requests, args.serving_rate, n_max=args.serving_nmax)
     print(f"bench: serving {len(reqs)} requests at {args.serving_rate}/s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
+    budget = int(args.serving_hbm_mib) << 20
     r = ensemble.batch_run([q["ini"] for q in reqs], arrivals=[q["arrival"] for q in reqs], workers=args.workers,
-                           max_steps=args.max_job_steps, boxes_per_worker=args.group_boxes, depth=args.group_depth)
+                           max_steps=args.max_job_steps, boxes_per_worker=args.group_boxes, depth=args.group_depth,
+                           hbm_budget=budget)
     wall = time.perf_counter() - t0
     arrival = np.array([q["arrival"] for q in reqs])
     ttr = r["end"] - arrival
     out = {"requests": len(reqs), "rate_per_s": args.serving_rate, "workers": args.workers,
            "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
            "time_to_result_p50_s": _pct(ttr, 50), "time_to_result_p99_s": _pct(ttr, 99),
-           "runs_per_hour": 3600.0 * len(reqs) / wall}
+           "runs_per_hour": 3600.0 * len(reqs) / wall, "hbm_budget_bytes": budget,
+           "hbm_peak_bytes": r["peak_bytes"]}
     if args.plain_ensemble:
         # the plain GPU path as a queue: each request's own run time (measured one at a time),
         # served first come first served on the same arrival times
@@ -190,6 +193,8 @@ def main() -> int:
                     help="largest box of the ensemble sample also run on the CPU for the runs/hour comparison")
     ap.add_argument("--serving-requests", type=int, default=16, help="serving requests (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=4.0, help="serving arrival rate, requests/s")
+    ap.add_argument("--serving-hbm-mib", type=int, default=4096,
+                    help="device memory the admitted serving boxes may hold at once (0: no limit)")
     ap.add_argument("--serving-nmax", type=float, default=2e4,
                     help="largest request size in the serving sample (the full form allows 2e5)")
     ap.add_argument("--tuned", default="",

@@ -113,6 +113,11 @@ typedef struct mcac_merge_result {
 } mcac_merge_result;
 
 int mcac_box_create(const mcac_box_params *params, mcac_box **out);
+/* Device memory (bytes) mcac_box_create allocates for these parameters: the
+ * sum of its buffer sizes, before the allocator's rounding (host computation,
+ * no GPU call); -1 for invalid parameters.  No reference counterpart: it lets
+ * a driver admit boxes by an HBM budget. */
+int64_t mcac_box_device_bytes(const mcac_box_params *params);
 int mcac_box_destroy(mcac_box *box);
 
 /* Uploads a full state.  Spheres in index order (positions, radius, volume,

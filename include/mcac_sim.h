@@ -95,25 +95,37 @@ int64_t mcac_pick_table_selftest_ties(void);
  * position k of grid g = ((g + k) / 8, 2k / 8, 3k / 8), Radius k / 4, Label k. */
 int mcac_write_test_snapshots(const char *prefix, int64_t n, int64_t grids, int64_t per_file);
 
+/* Device memory (bytes) the box of this parameter file takes
+ * (mcac_box_device_bytes of the parameters Simulation uses: one sphere and
+ * one aggregate id per monomer); -1 when the text does not parse or asks for
+ * an option this path does not run.  No reference counterpart: admission of
+ * runs by an HBM budget. */
+int64_t mcac_sim_device_bytes(const char *ini_text);
+
 /* Ensemble / serving runs (no reference counterpart: the reference runs one
  * box per process, src/main.cpp): n_jobs parameter files run to completion
  * (or max_steps MC steps) on `workers` host threads sharing one GPU, each
  * worker with its own box and stream.  Job j starts in list order, not before
- * arrival_s[j] seconds after the call (NULL: all at once).  Per job: start and
- * end time (s since the call), MC steps, status.  Returns the first non-zero
- * job status, else 0. */
+ * arrival_s[j] seconds after the call (NULL: all at once), and, with
+ * hbm_budget > 0, not before its box (mcac_sim_device_bytes) fits in what the
+ * running boxes leave of hbm_budget bytes (a job larger than the budget runs
+ * alone).  Per job: start and end time (s since the call), MC steps, status;
+ * *peak_bytes (may be NULL) the most device memory the admitted boxes held
+ * at once (0 without a budget).  Returns the first non-zero job status, else 0. */
 int mcac_batch_run(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers, int device,
-                   int64_t max_steps, double *start_s, double *end_s, int64_t *steps, int *status);
+                   int64_t max_steps, int64_t hbm_budget, double *start_s, double *end_s, int64_t *steps, int *status,
+                   int64_t *peak_bytes);
 /* The same runs in lock step: each worker drives up to boxes_per_worker boxes
  * that share one stream (mcac_group); per round it picks up to `depth` (1..8)
  * steps ahead for every box, as the pipelined loop does, and enqueues them all
  * in one launch, one workgroup running each box's steps in order.  Same
- * outputs and job order as mcac_batch_run, plus each job's final physical time
- * (final_time may be NULL); each box's steps and results are those of a run on
- * its own. */
+ * outputs, job order and admission as mcac_batch_run, plus each job's final
+ * physical time (final_time may be NULL); each box's steps and results are
+ * those of a run on its own. */
 int mcac_batch_run_groups(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers,
-                          int boxes_per_worker, int depth, int device, int64_t max_steps, double *start_s,
-                          double *end_s, int64_t *steps, int *status, double *final_time);
+                          int boxes_per_worker, int depth, int device, int64_t max_steps, int64_t hbm_budget,
+                          double *start_s, double *end_s, int64_t *steps, int *status, double *final_time,
+                          int64_t *peak_bytes);
 
 #ifdef __cplusplus
 }

@@ -56,14 +56,14 @@ GPU_SYMBOLS = (
     "mcac_box_download_spheres", "mcac_box_sphere_owners", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
-    "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part",
+    "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
     "mcac_sim_set_trace", "mcac_sim_n_steps", "mcac_sim_get_steps", "mcac_sim_n_merges",
     "mcac_sim_get_merges", "mcac_sim_scalar", "mcac_sim_n_aggs", "mcac_sim_export_aggs", "mcac_sim_box",
     "mcac_physics_scalars", "mcac_pick_table_selftest", "mcac_pick_table_selftest_ties", "mcac_batch_run",
-    "mcac_write_test_snapshots", "mcac_batch_run_groups",
+    "mcac_write_test_snapshots", "mcac_batch_run_groups", "mcac_sim_device_bytes",
 )
 
 _P = ctypes.c_void_p
@@ -100,6 +100,8 @@ def load_gpu():
         _require(path)
         L = ctypes.CDLL(str(path))
         L.mcac_box_create.argtypes = [_P, ctypes.POINTER(_P)]
+        L.mcac_box_device_bytes.restype = _I64
+        L.mcac_box_device_bytes.argtypes = [_P]
         L.mcac_box_destroy.argtypes = [_P]
         L.mcac_box_upload.argtypes = [_P, _I64] + [_DP] * 9 + [_I64] + [_DP] * 5 + [_I64P, _I64P]
         L.mcac_next_contact.argtypes = [_P, _I64, _DP, _D, _D, _P]
@@ -164,10 +166,12 @@ def load_sim():
         L.mcac_pick_table_selftest.argtypes = [_I64, _I64, _I64, ctypes.c_uint64, _I]
         L.mcac_pick_table_selftest_ties.restype = _I64
         L.mcac_write_test_snapshots.argtypes = [ctypes.c_char_p, _I64, _I64, _I64]
-        L.mcac_batch_run.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I64, _DP, _DP, _I64P,
-                                     ctypes.POINTER(_I)]
-        L.mcac_batch_run_groups.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I, _I, _I64, _DP,
-                                            _DP, _I64P, ctypes.POINTER(_I), _DP]
+        L.mcac_batch_run.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I64, _I64, _DP, _DP,
+                                     _I64P, ctypes.POINTER(_I), _I64P]
+        L.mcac_batch_run_groups.argtypes = [ctypes.POINTER(ctypes.c_char_p), _DP, _I64, _I, _I, _I, _I, _I64, _I64,
+                                            _DP, _DP, _I64P, ctypes.POINTER(_I), _DP, _I64P]
+        L.mcac_sim_device_bytes.restype = _I64
+        L.mcac_sim_device_bytes.argtypes = [ctypes.c_char_p]
         _sim = L
     return _sim
 

@@ -3,6 +3,7 @@
 // jobs to completion; the GPU interleaves the boxes' launches.  Jobs start in
 // list order, not before their arrival time.  No counterpart in the reference,
 // which runs one box per process (src/main.cpp).
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -14,10 +15,48 @@
 #include "../../../include/mcac_sim.h"
 #include "sim_group.hpp"
 
+namespace {
+
+// Admission by device memory: a job starts only when its box fits in what the
+// running boxes leave of the budget (a job larger than the budget runs alone).
+class HbmBudget {
+  public:
+    HbmBudget(int64_t cap, const char *const *ini_texts, int64_t n) : cap_(cap), bytes_(static_cast<size_t>(n), 0) {
+        if (cap_ > 0)
+            for (int64_t j = 0; j < n; ++j) bytes_[static_cast<size_t>(j)] = std::max<int64_t>(0, mcac_sim_device_bytes(ini_texts[j]));
+    }
+    bool try_take(int64_t job) {
+        if (cap_ <= 0) return true;
+        const int64_t b = bytes_[static_cast<size_t>(job)];
+        int64_t u = used_.load();
+        while (true) {
+            if (u != 0 && u + b > cap_) return false;
+            if (used_.compare_exchange_weak(u, u + b)) {
+                int64_t pk = peak_.load();
+                while (u + b > pk && !peak_.compare_exchange_weak(pk, u + b)) {
+                }
+                return true;
+            }
+        }
+    }
+    void give(int64_t job) {
+        if (cap_ > 0) used_.fetch_sub(bytes_[static_cast<size_t>(job)]);
+    }
+    int64_t peak() const { return peak_.load(); }
+
+  private:
+    int64_t cap_;
+    std::vector<int64_t> bytes_;
+    std::atomic<int64_t> used_{0}, peak_{0};
+};
+
+}  // namespace
+
 extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers,
-                              int device, int64_t max_steps, double *start_s, double *end_s, int64_t *steps,
-                              int *status) {
+                              int device, int64_t max_steps, int64_t hbm_budget, double *start_s, double *end_s,
+                              int64_t *steps, int *status, int64_t *peak_bytes) {
     if (!ini_texts || n_jobs < 0 || workers < 1 || !start_s || !end_s || !steps || !status) return MCAC_E_ARGUMENT;
+    HbmBudget budget(hbm_budget, ini_texts, n_jobs);
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
     auto now_s = [t0] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
@@ -25,7 +64,7 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
     auto worker = [&] {
         for (int64_t j = next.fetch_add(1); j < n_jobs; j = next.fetch_add(1)) {
             const double at = arrival_s ? arrival_s[j] : 0.0;
-            while (now_s() < at) std::this_thread::sleep_for(std::chrono::microseconds(200));
+            while (now_s() < at || !budget.try_take(j)) std::this_thread::sleep_for(std::chrono::microseconds(200));
             start_s[j] = now_s();
             steps[j] = 0;
             char err[256] = {0};
@@ -39,6 +78,7 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
                 if (done == 0) break;
             }
             if (sim) mcac_sim_destroy(sim);
+            budget.give(j);
             end_s[j] = now_s();
             status[j] = rc;
             if (rc) std::fprintf(stderr, "mcac_batch_run: job %lld status %d %s\n", static_cast<long long>(j), rc, err);
@@ -47,6 +87,7 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
     std::vector<std::thread> pool;
     for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
     for (auto &t : pool) t.join();
+    if (peak_bytes) *peak_bytes = budget.peak();
     for (int64_t j = 0; j < n_jobs; ++j)
         if (status[j]) return status[j];
     return 0;
@@ -59,10 +100,12 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
 // the round and the worker admits the next job that has arrived.
 extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs,
                                      int workers, int boxes_per_worker, int depth, int device, int64_t max_steps,
-                                     double *start_s, double *end_s, int64_t *steps, int *status, double *final_time) {
+                                     int64_t hbm_budget, double *start_s, double *end_s, int64_t *steps, int *status,
+                                     double *final_time, int64_t *peak_bytes) {
     if (!ini_texts || n_jobs < 0 || workers < 1 || boxes_per_worker < 1 || boxes_per_worker > 4096 || depth < 1 ||
         depth > 8 || !start_s || !end_s || !steps || !status)
         return MCAC_E_ARGUMENT;
+    HbmBudget budget(hbm_budget, ini_texts, n_jobs);
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
     auto now_s = [t0] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
@@ -86,6 +129,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
             if (rc == 0) rc = mcac::sim_finalize(act[k].sim);
             if (final_time) final_time[j] = mcac_sim_scalar(act[k].sim, "time");
             mcac_sim_destroy(act[k].sim);
+            budget.give(j);
             end_s[j] = now_s();
             status[j] = rc;
             if (rc) std::fprintf(stderr, "mcac_batch_run_groups: job %lld status %d\n", static_cast<long long>(j), rc);
@@ -111,7 +155,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                 }
                 if (pending < 0) break;
                 const double at = arrival_s ? arrival_s[pending] : 0.0;
-                if (now_s() < at) {
+                if (now_s() < at || !budget.try_take(pending)) {  // not arrived, or no memory for it yet
                     if (!act.empty()) break;
                     std::this_thread::sleep_for(std::chrono::microseconds(200));
                     continue;
@@ -127,6 +171,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                 if (rc == 0) rc = mcac_sim_init(sim);
                 if (rc) {
                     if (sim) mcac_sim_destroy(sim);
+                    budget.give(j);
                     if (final_time) final_time[j] = 0.0;
                     end_s[j] = now_s();
                     status[j] = rc;
@@ -190,6 +235,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
     std::vector<std::thread> pool;
     for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
     for (auto &t : pool) t.join();
+    if (peak_bytes) *peak_bytes = budget.peak();
     if (worker_rc.load()) return worker_rc.load();
     for (int64_t j = 0; j < n_jobs; ++j)
         if (status[j]) return status[j];

@@ -910,6 +910,24 @@ int mcac_sim_destroy(mcac_sim *sim) {
     return 0;
 }
 
+int64_t mcac_sim_device_bytes(const char *ini_text) {
+    if (!ini_text) return -1;
+    try {
+        mcac::PhysicalModel pm;
+        pm.load(mcac::IniFile(ini_text));
+        if (!pm.unsupported.empty()) return -1;
+        mcac_box_params bp{};  // the box Simulation::init creates (one sphere and one id per monomer)
+        bp.box_length = pm.box_length > 0 ? pm.box_length : 1.0;
+        bp.n_div = static_cast<int64_t>(pm.n_verlet_divisions);
+        bp.max_spheres = static_cast<int64_t>(pm.n_monomeres);
+        bp.max_aggregates = static_cast<int64_t>(pm.n_monomeres);
+        bp.volsurf = pm.volsurf;
+        return mcac_box_device_bytes(&bp);
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
 int mcac_sim_init(mcac_sim *sim) {
     if (!sim) return MCAC_E_ARGUMENT;
     try {

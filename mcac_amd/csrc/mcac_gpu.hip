@@ -590,6 +590,59 @@ int search(mcac_box *b, int64_t agg, const double dir[3], double lpm, double max
 
 }  // namespace
 
+// Buffer sizes of a box (entries), from its parameters.
+struct BoxSizes {
+    int64_t Ns, Na, n_cells, cand_cap, part_cap, pool_cap;
+};
+BoxSizes box_sizes(const mcac_box_params &p) {
+    BoxSizes z{};
+    z.Ns = p.max_spheres;
+    z.Na = p.max_aggregates;
+    z.n_cells = p.n_div * p.n_div * p.n_div;
+    z.cand_cap = p.candidate_capacity > 0 ? p.candidate_capacity : std::max<int64_t>(4096, std::min<int64_t>(z.Na, 1 << 20));
+    z.part_cap = std::max<int64_t>(z.cand_cap + 1024, 1 << 20);
+    z.pool_cap = p.member_pool > 0 ? p.member_pool : 4 * z.Ns + 4096;
+    return z;
+}
+
+// Every device buffer of a box, in allocation order: f(pointer, entries) is
+// called once per buffer (mcac_box_create allocates through it,
+// mcac_box_device_bytes adds the sizes up).  Returns the first non-zero f.
+template <class F> int box_buffers(DevBox &d, const BoxSizes &z, int volsurf, F &&f) {
+    const int64_t Ns = z.Ns, Na = z.Na;
+    int rc = 0;
+#define ALLOC(ptr, n) \
+    if ((rc = f(ptr, (n))) != 0) return rc
+    ALLOC(d.sx, Ns); ALLOC(d.sy, Ns); ALLOC(d.sz, Ns); ALLOC(d.sr, Ns); ALLOC(d.svol, Ns); ALLOC(d.ssurf, Ns);
+    ALLOC(d.srx, Ns); ALLOC(d.sry, Ns); ALLOC(d.srz, Ns);
+    ALLOC(d.ax, Na); ALLOC(d.ay, Na); ALLOC(d.az, Na); ALLOC(d.arx, Na); ALLOC(d.ary, Na); ALLOC(d.arz, Na);
+    ALLOC(d.armax, Na); ALLOC(d.avol, Na); ALLOC(d.asurf, Na); ALLOC(d.arg, Na);
+    ALLOC(d.csvx, Na); ALLOC(d.csvy, Na); ALLOC(d.csvz, Na); ALLOC(d.csr, Na); ALLOC(d.csvr2, Na);
+    ALLOC(d.astart, Na); ALLOC(d.acount, Na); ALLOC(d.acell, Na); ALLOC(d.acap, Na);
+    ALLOC(d.members, z.pool_cap);
+    ALLOC(d.alive, Na);
+    ALLOC(d.cstart, z.n_cells + 1); ALLOC(d.citems, Na); ALLOC(d.ccursor, z.n_cells + 1);
+    ALLOC(d.cpx, Na); ALLOC(d.cpy, Na); ALLOC(d.cpz, Na); ALLOC(d.cpr, Na); ALLOC(d.cslot, Na); ALLOC(d.cf, Na);
+    ALLOC(d.dirty, Na); ALLOC(d.dirty_list, Na + 1);
+    ALLOC(d.dpx, Na + 1); ALLOC(d.dpy, Na + 1); ALLOC(d.dpz, Na + 1); ALLOC(d.dpr, Na + 1);
+    ALLOC(d.dpcell, Na + 1); ALLOC(d.dslot, Na);
+    ALLOC(d.cand_db, z.cand_cap); ALLOC(d.cand_exact, z.cand_cap); ALLOC(d.cand_rank, z.cand_cap);
+    ALLOC(d.cand_id, z.cand_cap); ALLOC(d.order, z.cand_cap); ALLOC(d.cand_pair, z.cand_cap);
+    ALLOC(d.part_d, z.part_cap); ALLOC(d.part_p, z.part_cap);
+    ALLOC(d.dc, Ns);
+    ALLOC(d.row_cnt, Ns); ALLOC(d.row_j, Ns * kOverlapSlots); ALLOC(d.row_c, Ns * kOverlapSlots);
+    if (volsurf == 1 || volsurf == 3) {
+        ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.cvol, Ns); ALLOC(d.csurf, Ns);
+    }
+    if (volsurf == 2) {
+        ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.sr3, Ns); ALLOC(d.walk, 3 * Ns * kOverlapSlots);
+        ALLOC(d.walk_n, Ns);
+    }
+    ALLOC(d.ss, 1); ALLOC(d.ms, 1);
+#undef ALLOC
+    return 0;
+}
+
 extern "C" {
 
 int mcac_swept_sphere_batch(int64_t n, const double *p1, const double *r1, const double *p2, const double *r2,
@@ -625,6 +678,19 @@ int mcac_swept_sphere_batch(int64_t n, const double *p1, const double *r1, const
 
 const char *mcac_version(void) { return "mcac_gpu 0.2 (gfx950, FP64 collision path)"; }
 
+int64_t mcac_box_device_bytes(const mcac_box_params *params) {
+    if (!params || params->max_spheres < 1 || params->max_aggregates < 1 || params->n_div < 1 ||
+        params->n_div > 1290 || params->volsurf < 0 || params->volsurf > 3)
+        return -1;
+    DevBox d{};
+    int64_t bytes = 0;
+    box_buffers(d, box_sizes(*params), params->volsurf, [&](auto *&ptr, int64_t n) {
+        bytes += (n < 1 ? 1 : n) * static_cast<int64_t>(sizeof(*ptr));
+        return 0;
+    });
+    return bytes;
+}
+
 int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (!params || !out) return MCAC_E_ARGUMENT;
     if (params->box_length <= 0 || params->n_div < 1 || params->n_div > 1290 || params->max_spheres < 1 ||
@@ -641,43 +707,13 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     };
     if (hipSetDevice(params->device) != hipSuccess) return fail(MCAC_E_DEVICE);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(MCAC_E_DEVICE);
-    const int64_t Ns = params->max_spheres, Na = params->max_aggregates;
-    const int64_t nd = params->n_div;
-    b->n_cells = static_cast<int>(nd * nd * nd);
-    b->cand_cap = params->candidate_capacity > 0 ? params->candidate_capacity
-                                                 : std::max<int64_t>(4096, std::min<int64_t>(Na, 1 << 20));
-    b->part_cap = std::max<int64_t>(b->cand_cap + 1024, 1 << 20);
-    b->pool_cap = params->member_pool > 0 ? params->member_pool : 4 * Ns + 4096;
-    DevBox &d = b->d;
-#define ALLOC(ptr, n) \
-    if ((rc = dalloc(b, &(ptr), (n))) != 0) return fail(rc)
-    ALLOC(d.sx, Ns); ALLOC(d.sy, Ns); ALLOC(d.sz, Ns); ALLOC(d.sr, Ns); ALLOC(d.svol, Ns); ALLOC(d.ssurf, Ns);
-    ALLOC(d.srx, Ns); ALLOC(d.sry, Ns); ALLOC(d.srz, Ns);
-    ALLOC(d.ax, Na); ALLOC(d.ay, Na); ALLOC(d.az, Na); ALLOC(d.arx, Na); ALLOC(d.ary, Na); ALLOC(d.arz, Na);
-    ALLOC(d.armax, Na); ALLOC(d.avol, Na); ALLOC(d.asurf, Na); ALLOC(d.arg, Na);
-    ALLOC(d.csvx, Na); ALLOC(d.csvy, Na); ALLOC(d.csvz, Na); ALLOC(d.csr, Na); ALLOC(d.csvr2, Na);
-    ALLOC(d.astart, Na); ALLOC(d.acount, Na); ALLOC(d.acell, Na); ALLOC(d.acap, Na);
-    ALLOC(d.members, b->pool_cap);
-    ALLOC(d.alive, Na);
-    ALLOC(d.cstart, b->n_cells + 1); ALLOC(d.citems, Na); ALLOC(d.ccursor, b->n_cells + 1);
-    ALLOC(d.cpx, Na); ALLOC(d.cpy, Na); ALLOC(d.cpz, Na); ALLOC(d.cpr, Na); ALLOC(d.cslot, Na); ALLOC(d.cf, Na);
-    ALLOC(d.dirty, Na); ALLOC(d.dirty_list, Na + 1);
-    ALLOC(d.dpx, Na + 1); ALLOC(d.dpy, Na + 1); ALLOC(d.dpz, Na + 1); ALLOC(d.dpr, Na + 1);
-    ALLOC(d.dpcell, Na + 1); ALLOC(d.dslot, Na);
-    ALLOC(d.cand_db, b->cand_cap); ALLOC(d.cand_exact, b->cand_cap); ALLOC(d.cand_rank, b->cand_cap);
-    ALLOC(d.cand_id, b->cand_cap); ALLOC(d.order, b->cand_cap); ALLOC(d.cand_pair, b->cand_cap);
-    ALLOC(d.part_d, b->part_cap); ALLOC(d.part_p, b->part_cap);
-    ALLOC(d.dc, Ns);
-    ALLOC(d.row_cnt, Ns); ALLOC(d.row_j, Ns * kOverlapSlots); ALLOC(d.row_c, Ns * kOverlapSlots);
-    if (params->volsurf == 1 || params->volsurf == 3) {
-        ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.cvol, Ns); ALLOC(d.csurf, Ns);
-    }
-    if (params->volsurf == 2) {
-        ALLOC(d.row_d, Ns * kOverlapSlots); ALLOC(d.sr3, Ns); ALLOC(d.walk, 3 * Ns * kOverlapSlots);
-        ALLOC(d.walk_n, Ns);
-    }
-    ALLOC(d.ss, 1); ALLOC(d.ms, 1);
-#undef ALLOC
+    const BoxSizes z = box_sizes(*params);
+    b->n_cells = static_cast<int>(z.n_cells);
+    b->cand_cap = z.cand_cap;
+    b->part_cap = z.part_cap;
+    b->pool_cap = z.pool_cap;
+    rc = box_buffers(b->d, z, params->volsurf, [&](auto *&ptr, int64_t n) { return dalloc(b, &ptr, n); });
+    if (rc) return fail(rc);
     if (b->n_cells + 1 <= (1 << 20) &&
         hipHostMalloc(reinterpret_cast<void **>(&b->h_cstart), (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
                       hipHostMallocDefault) != hipSuccess)

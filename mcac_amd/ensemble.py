@@ -72,12 +72,19 @@ def serving_requests(n_requests: int, rate_per_s: float, s0: int = 5000, rng_see
     return out
 
 
+def device_bytes(ini: str) -> int:
+    """Device memory the box of a parameter file takes (mcac_sim_device_bytes)."""
+    return int(load_sim().mcac_sim_device_bytes(ini.encode()))
+
+
 def batch_run(inis: list[str], arrivals=None, workers: int = 8, device: int = 0, max_steps: int = 2 ** 62,
-              boxes_per_worker: int = 0, depth: int = 8) -> dict:
+              boxes_per_worker: int = 0, depth: int = 8, hbm_budget: int = 0) -> dict:
     """Runs the jobs on `workers` host threads sharing the GPU (include/mcac_sim.h):
     one box per worker at a time (mcac_batch_run), or with boxes_per_worker >= 1
     up to that many boxes per worker stepped in lock step, up to `depth` steps
-    of each box per launch (mcac_batch_run_groups)."""
+    of each box per launch (mcac_batch_run_groups).  With hbm_budget > 0 (bytes)
+    a job starts only when its box fits in what the running boxes leave of it;
+    out["peak_bytes"] is the most the admitted boxes held at once."""
     n = len(inis)
     texts = (ctypes.c_char_p * n)(*[s.encode() for s in inis])
     arr = None if arrivals is None else np.ascontiguousarray(arrivals, np.float64)
@@ -86,15 +93,18 @@ def batch_run(inis: list[str], arrivals=None, workers: int = 8, device: int = 0,
     status = np.zeros(n, np.int32)
     st = status.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
     L = load_sim()
+    peak = np.zeros(1, np.int64)
     out = dict(start=start, end=end, steps=steps, status=status)
     if boxes_per_worker >= 1:
         final_time = np.zeros(n)
         rc = L.mcac_batch_run_groups(texts, None if arr is None else _dp(arr), n, workers, boxes_per_worker, depth,
-                                     device, max_steps, _dp(start), _dp(end), _ip(steps), st, _dp(final_time))
+                                     device, max_steps, int(hbm_budget), _dp(start), _dp(end), _ip(steps), st,
+                                     _dp(final_time), _ip(peak))
         _check(rc, "mcac_batch_run_groups")
         out["final_time"] = final_time
     else:
-        rc = L.mcac_batch_run(texts, None if arr is None else _dp(arr), n, workers, device, max_steps, _dp(start),
-                              _dp(end), _ip(steps), st)
+        rc = L.mcac_batch_run(texts, None if arr is None else _dp(arr), n, workers, device, max_steps,
+                              int(hbm_budget), _dp(start), _dp(end), _ip(steps), st, _ip(peak))
         _check(rc, "mcac_batch_run")
+    out["peak_bytes"] = int(peak[0])
     return out
