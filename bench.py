@@ -63,6 +63,57 @@ def c3_windows(args) -> int:
     return 0
 
 
+OVERLAP_INI = """[monomers]
+number = {n}
+density = 1800
+mean_diameter = 30
+dispersion_diameter = 1.25
+initialisation_mode = lognormal
+[environment]
+volume_fraction = {fv}
+temperature = 1700
+pressure = 101300
+fractal_prefactor = 1.30
+fractal_dimension = 1.78
+[limits]
+number_of_aggregates = {stop}
+[numerics]
+random_seed = 11
+with_domain_duplication = false
+[surface_growth]
+volsurf_method = {method}
+"""
+
+
+def overlap_runs() -> int:
+    """Whole runs with each volsurf_method, overlap-correction merges in one
+    workgroup (default) and as the kernel sequence (MCAC_FUSED_MERGE=0, read
+    at box creation); both forms must give the same steps, time and merges."""
+    import mcac_amd
+    rows = []
+    for n, fv, stop in ((20000, 0.01, 2000), (3000, 0.05, 60)):
+        for method in ("none", "caps", "alphas", "arvo"):
+            ref = None
+            for fused in ((1,) if method == "none" else (1, 0)):
+                os.environ["MCAC_FUSED_MERGE"] = str(fused)
+                sim = mcac_amd.Simulation(OVERLAP_INI.format(n=n, fv=fv, stop=stop, method=method), device=0)
+                t0 = time.perf_counter()
+                steps = sim.run()
+                wall = time.perf_counter() - t0
+                key = (steps, sim.scalar("time"), sim.scalar("merges"))
+                ref = ref or key
+                rows.append(dict(n=n, fv=fv, stop=stop, method=method, fused=fused, wall_s=wall, steps=steps,
+                                 merges=key[2], steps_per_s=steps / wall, same_run=key == ref))
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+                sim.close()
+                if key != ref:
+                    print("bench: the two merge forms gave different runs", file=sys.stderr)
+                    return 1
+    os.environ.pop("MCAC_FUSED_MERGE", None)
+    print(json.dumps({"overlap_runs": rows}), flush=True)
+    return 0
+
+
 def _pct(x, q):
     import numpy as np
     return float(np.percentile(np.asarray(x, float), q)) if len(x) else None
@@ -206,6 +257,8 @@ def main() -> int:
     ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")
     ap.add_argument("--c3-windows", type=float, default=0.0,
                     help="seconds: run the N=1e6 box on and report MC steps/s per window of N_agg (0: skip)")
+    ap.add_argument("--overlap-runs", action="store_true",
+                    help="time whole runs per volsurf_method in both merge launch forms, then exit")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
                     help="contact search: 0 FP64, 1 FP32 broad phase + FP64 solve, 2 as 1 with miss counting")
     ap.add_argument("--timing-stride", type=int, default=64,
@@ -234,6 +287,8 @@ def main() -> int:
     import mcac_amd
     if args.c3_windows > 0:
         return c3_windows(args)
+    if args.overlap_runs:
+        return overlap_runs()
 
     text = WORKLOAD_INI.read_text()
     print("bench: placing N=1e6 monomers", file=sys.stderr, flush=True)

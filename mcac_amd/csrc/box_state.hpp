@@ -15,6 +15,7 @@ constexpr int kMaxVisit = 2048;        // visited cells handled with an LDS pref
 constexpr int kMaxPlan = 4096;         // filtered candidates planned in LDS
 constexpr int kOverlapSlots = 16;      // contacts kept per sphere row (multi-kernel merge)
 constexpr int kSmallMerge = 256;       // merged aggregates up to this size use one fused kernel
+constexpr int kFusedMerge = 512;       // overlap-correction merges up to this size run in one workgroup
 constexpr int kStepCands = 32;         // k_step solves up to this many survivors itself
 constexpr long long kStepPairs = 16384;  // ... holding up to this many sphere pairs
 constexpr int kStepMover = 1024;       // movers up to this size may take k_step

@@ -376,8 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_small(DevBox b, MergeArgs a) {
 }
 
 // ------------------------------------------------------------------ multi-kernel merge
-__global__ void k_merge_prep(DevBox b, MergeArgs a) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void merge_prep_one(const DevBox &b, const MergeArgs &a) {
+    if (threadIdx.x != 0) return;
     MergeState *m = b.ms;
     m->ok = 0;
     m->rmax_bits = 0ull;
@@ -404,11 +404,17 @@ __global__ void k_merge_prep(DevBox b, MergeArgs a) {
     m->ok = 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_merge_lists(DevBox b) {
+__global__ void k_merge_prep(DevBox b, MergeArgs a) {
+    if (blockIdx.x == 0) merge_prep_one(b, a);
+}
+
+// Grid-stride phases take their first position and stride, so that one
+// workgroup can run them (first = threadIdx.x, stride = blockDim.x).
+__device__ __forceinline__ void merge_lists_part(const DevBox &b, int first, int stride) {
     const MergeState *m = b.ms;
     if (!m->ok) return;
     const int nk = m->n_k, nr = m->n_r;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nk + nr; t += gridDim.x * blockDim.x) {
+    for (int t = first; t < nk + nr; t += stride) {
         if (t < nk) {
             if (m->new_start != m->start_k) b.members[m->new_start + t] = b.members[m->start_k + t];
         } else {
@@ -417,6 +423,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_lists(DevBox b) {
             remap_sphere(b, s, m->diffpos, m->refpos);
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_lists(DevBox b) {
+    merge_lists_part(b, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Ordered sums over the removed spheres: one block stages tiles in LDS,
@@ -477,13 +487,13 @@ __global__ __launch_bounds__(kBlock) void k_merge_seq(DevBox b, MergeArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_merge_dc(DevBox b) {
+__device__ __forceinline__ void merge_dc_part(const DevBox &b, int first, int stride) {
     MergeState *m = b.ms;
     if (!m->ok) return;
     const int n = m->n_k + m->n_r, st = m->new_start;
     const double r0 = m->r[0], r1 = m->r[1], r2 = m->r[2];
     double mx = 0.0;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    for (int t = first; t < n; t += stride) {
         int s = b.members[st + t];
         double d0 = b.srx[s] - r0, d1 = b.sry[s] - r1, d2 = b.srz[s] - r2;
         double dc = __builtin_sqrt(d0 * d0 + d1 * d1 + d2 * d2);
@@ -495,11 +505,15 @@ __global__ __launch_bounds__(kBlock) void k_merge_dc(DevBox b) {
     if ((threadIdx.x & 63) == 0) atomicMax(&m->rmax_bits, static_cast<unsigned long long>(__double_as_longlong(mx)));
 }
 
+__global__ __launch_bounds__(kBlock) void k_merge_dc(DevBox b) {
+    merge_dc_part(b, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // With spherical caps the weights are the corrected volumes by list position
 // and this kernel ends the sequence (publishes).
-__global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b, MergeArgs a) {
+__device__ __forceinline__ void merge_rg_block(const DevBox &b, const MergeArgs &a) {
     MergeState *m = b.ms;
-    if (!m->ok) return;  // k_merge_prep published the failed re-check
+    if (!m->ok) return;  // the prep step published the failed re-check
     __shared__ double tv[kBlock], td[kBlock];
     const int k = m->keep, n = m->n_k + m->n_r, st = m->new_start;
     double arg = 0.;
@@ -526,15 +540,16 @@ __global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b, MergeArgs a) {
     if (a.volsurf) publish_merge(a, m->res);
 }
 
+__global__ __launch_bounds__(kBlock) void k_merge_rg(DevBox b, MergeArgs a) { merge_rg_block(b, a); }
+
 // Pair pass of the merged aggregate in kBlock x kBlock tiles, j tile in LDS.
 // With both (spherical caps) a contact goes into both spheres' rows, with
 // its distance, so that each row holds the sphere's whole neighbour map.
-__global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b, int both) {
+// One (i tile, j tile) pair with tj >= ti; every thread of the block calls it
+// (one barrier inside).
+__device__ __forceinline__ void overlap_pairs_tile(const DevBox &b, int ti, int tj, int both) {
     const MergeState *m = b.ms;
-    if (!m->ok) return;
     const int n = m->n_k + m->n_r, st = m->new_start;
-    const int ti = blockIdx.y, tj = blockIdx.x;
-    if (tj < ti) return;
     __shared__ double jx[kBlock], jy[kBlock], jz[kBlock], jr[kBlock];
     const int jbase = tj * kBlock;
     const int jt = jbase + threadIdx.x;
@@ -547,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b, int both) {
     }
     __syncthreads();
     const int i = ti * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n) return;  // after the block's only barrier
     const int si = b.members[st + i];
     const double ix = b.srx[si], iy = b.sry[si], iz = b.srz[si], ir = b.sr[si];
     const int jend = (jbase + kBlock < n) ? jbase + kBlock : n;
@@ -582,13 +597,18 @@ __global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b, int both) {
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b, int both) {
+    if (!b.ms->ok || static_cast<int>(blockIdx.x) < static_cast<int>(blockIdx.y)) return;
+    overlap_pairs_tile(b, blockIdx.y, blockIdx.x, both);
+}
+
 // Ordered overlap sum in (i, j) order; clears the row counters; publishes.
 // Each thread sorts its row by j in registers and stages its coefficients
 // toward higher j in LDS, from which thread 0 adds them in order.  With an
 // overlap correction (volsurf != 0) the rows hold both directions (pairs with
 // j > i are summed) and stay, sorted, for the per-sphere kernel, which clears
 // them; no publication.
-__global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
+__device__ __forceinline__ void overlap_sum_block(const DevBox &b, const MergeArgs &a) {
     MergeState *m = b.ms;
     if (!m->ok) return;
     const int n = m->n_k + m->n_r;
@@ -669,6 +689,8 @@ __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
     if (!a.volsurf) publish_merge(a, m->res);
 }
 
+__global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) { overlap_sum_block(b, a); }
+
 // ------------------------------------------------------------------ spherical caps (aggregat.cpp:354-386)
 // Corrected volume and surface of every sphere of the merged aggregate, one
 // thread per list position t, from its two-way row (neighbours ascending,
@@ -678,11 +700,11 @@ __global__ __launch_bounds__(kBlock) void k_overlap_sum(DevBox b, MergeArgs a) {
 // sphere's neighbours inside the keeper for a keeper sphere (the keeper's last
 // update saw the same rigid geometry) and 0 for a sphere of the removed
 // aggregate (a new map).  Clears the row counters.
-__global__ __launch_bounds__(kBlock) void k_caps_volumes(DevBox b, MergeArgs a) {
+__device__ __forceinline__ void caps_volumes_part(const DevBox &b, int first, int stride) {
     const MergeState *m = b.ms;
     if (!m->ok) return;
     const int n = m->n_k + m->n_r, st = m->new_start, nk = m->n_k;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    for (int t = first; t < n; t += stride) {
         int rc = b.row_cnt[t];
         if (rc > kOverlapSlots) rc = kOverlapSlots;
         const long long base = static_cast<long long>(t) * kOverlapSlots;
@@ -720,6 +742,10 @@ __global__ __launch_bounds__(kBlock) void k_caps_volumes(DevBox b, MergeArgs a) 
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_caps_volumes(DevBox b) {
+    caps_volumes_part(b, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // Thread 0: centre of mass from the list-order sums and the aggregate volume V
 // (aggregat.cpp:442-451), the merged aggregate's row, and the result fields.
 __device__ __forceinline__ void merge_store_result(const DevBox &b, const MergeArgs &a, int k, int n, double V,
@@ -754,12 +780,12 @@ __device__ __forceinline__ void merge_store_result(const DevBox &b, const MergeA
 // neighbours are the sphere's map entries in the map's order that overlap it
 // and are not inside it; a sphere inside a neighbour carries nothing.  Writes
 // the parts (m^3, m^2) for the ordered sums; clears the row counters.
-__global__ __launch_bounds__(kBlock) void k_arvo_volumes(DevBox b, MergeArgs a) {
+__device__ __forceinline__ void arvo_volumes_part(const DevBox &b, int first, int stride) {
     const MergeState *m = b.ms;
     if (!m->ok) return;
     const int n = m->n_k + m->n_r, st = m->new_start, nk = m->n_k;
     const double nm = 1e+09;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    for (int t = first; t < n; t += stride) {
         int rc = b.row_cnt[t];
         if (rc > kOverlapSlots) rc = kOverlapSlots;
         const long long base = static_cast<long long>(t) * kOverlapSlots;
@@ -803,12 +829,16 @@ __global__ __launch_bounds__(kBlock) void k_arvo_volumes(DevBox b, MergeArgs a) 
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_arvo_volumes(DevBox b) {
+    arvo_volumes_part(b, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // Ordered sums of the merged aggregate, all positions from the start
 // (aggregat.cpp:387-390, 437-441, 257-260, 474-478), with the corrected
 // volumes (caps: stores the aggregate) or the spheres' own (alphas: the sums
 // wait in MergeState for the host's correction factors): one block stages
 // tiles in LDS, thread 0 accumulates in list order.
-__global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) {
+__device__ __forceinline__ void caps_sums_block(const DevBox &b, const MergeArgs &a) {
     MergeState *m = b.ms;
     if (!m->ok) return;
     const bool alphas = a.volsurf == 2;
@@ -853,6 +883,8 @@ __global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) {
     merge_store_result(b, a, k, n, V, S, vx, vy, vz, sr, svr2, raw);
 }
 
+__global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) { caps_sums_block(b, a); }
+
 // ------------------------------------------------------------------ alphas (aggregat.cpp:392-422)
 // Each sphere's walk of its neighbour map in the map's order (all
 // neighbours, as the reference's loop over distances[i]): per entry the
@@ -860,11 +892,11 @@ __global__ __launch_bounds__(kBlock) void k_caps_sums(DevBox b, MergeArgs a) {
 // pow(r_j, 3) (host libm values) and r_i² + r_j².  The host sums the rows in
 // order (the cube of c_ij is glibc pow) and derives the correction factors.
 // Moves the row counters to walk_n and clears them.
-__global__ __launch_bounds__(kBlock) void k_alpha_walk(DevBox b, MergeArgs a) {
+__device__ __forceinline__ void alpha_walk_part(const DevBox &b, int first, int stride) {
     const MergeState *m = b.ms;
     if (!m->ok) return;
     const int n = m->n_k + m->n_r, st = m->new_start, nk = m->n_k;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    for (int t = first; t < n; t += stride) {
         int rc = b.row_cnt[t];
         if (rc > kOverlapSlots) rc = kOverlapSlots;
         const long long base = static_cast<long long>(t) * kOverlapSlots;
@@ -893,18 +925,74 @@ __global__ __launch_bounds__(kBlock) void k_alpha_walk(DevBox b, MergeArgs a) {
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_alpha_walk(DevBox b) {
+    alpha_walk_part(b, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // Second alphas phase (thread 0): the corrected volume and surface, and the
 // walk's overlap and coordination (aggregat.cpp:417-421) when it had entries.
-__global__ void k_alpha_store(DevBox b, MergeArgs a, double V, double S, double overlapping, double coordination,
-                              int set_overlap) {
+__device__ __forceinline__ void alpha_store_one(const DevBox &b, const MergeArgs &a, double V, double S,
+                                                double overlapping, double coordination, int set_overlap) {
     MergeState *m = b.ms;
-    if (!m->ok || threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!m->ok || threadIdx.x != 0) return;
     const double *s8 = m->sums;
     merge_store_result(b, a, m->keep, m->n_k + m->n_r, V, S, s8[2], s8[3], s8[4], s8[5], s8[6], s8[7]);
     if (set_overlap) {
         m->res.overlapping = overlapping;
         m->res.coordination = coordination;
     }
+}
+
+__global__ void k_alpha_store(DevBox b, MergeArgs a, double V, double S, double overlapping, double coordination,
+                              int set_overlap) {
+    if (blockIdx.x == 0) alpha_store_one(b, a, V, S, overlapping, coordination, set_overlap);
+}
+
+// ------------------------------------------------------------------ one-workgroup overlap-correction merge
+// The phases of the multi-kernel sequence above, in the same order and with
+// the same per-position work, run by one workgroup with a barrier between
+// phases (after a barrier the block sees every global write and atomic its
+// threads made before it), for merged aggregates of at most kFusedMerge
+// spheres: one launch per merge instead of six to eight.  Caps and arvo end
+// with the publication; alphas stops after the list-order sums, which the
+// host turns into correction factors for k_alpha_finish.
+__global__ __launch_bounds__(kBlock) void k_merge_fused(DevBox b, MergeArgs a) {
+    merge_prep_one(b, a);
+    __syncthreads();
+    if (!b.ms->ok) return;  // the prep step published the failed re-check
+    const int n = a.n_k + a.n_r;
+    merge_lists_part(b, threadIdx.x, kBlock);
+    __syncthreads();
+    const int tiles = (n + kBlock - 1) / kBlock;
+    for (int ti = 0; ti < tiles; ++ti) {
+        for (int tj = ti; tj < tiles; ++tj) {
+            overlap_pairs_tile(b, ti, tj, 1);
+            __syncthreads();  // the j tile in LDS is reloaded next
+        }
+    }
+    overlap_sum_block(b, a);
+    __syncthreads();
+    if (a.volsurf == 1) caps_volumes_part(b, threadIdx.x, kBlock);
+    else if (a.volsurf == 3) arvo_volumes_part(b, threadIdx.x, kBlock);
+    else alpha_walk_part(b, threadIdx.x, kBlock);
+    __syncthreads();
+    caps_sums_block(b, a);
+    if (a.volsurf == 2) return;
+    __syncthreads();
+    merge_dc_part(b, threadIdx.x, kBlock);
+    __syncthreads();
+    merge_rg_block(b, a);
+}
+
+// Second alphas phase in one workgroup: the corrected sums stored, then dc,
+// rmax and Rg (publishes).
+__global__ __launch_bounds__(kBlock) void k_alpha_finish(DevBox b, MergeArgs a, double V, double S,
+                                                         double overlapping, double coordination, int set_overlap) {
+    alpha_store_one(b, a, V, S, overlapping, coordination, set_overlap);
+    __syncthreads();
+    merge_dc_part(b, threadIdx.x, kBlock);
+    __syncthreads();
+    merge_rg_block(b, a);
 }
 
 __global__ void k_gather_members(DevBox b, int agg, int *out, int n) {

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -58,6 +59,7 @@ struct mcac_box {
     unsigned long long released = 0; // gate values the steps enqueued from now on ignore
     unsigned long long last_gate = 0;  // seq of the last consumed step that closed the gate
     int automerge = 0;               // queued moves merge in the same launch on contact
+    int fused_merge = 1;             // overlap-correction merges of <= kFusedMerge spheres in one launch
     bool inline_pending = false;     // a contact merged in its launch; mcac_merge not called yet
     StepRecord inline_rec{};
     int64_t inline_src = -1;
@@ -83,8 +85,10 @@ struct mcac_box {
     int64_t n_fused = 0, n_split = 0;  // searches launched as k_step / k_filter + k_solve
     int64_t n_compactions = 0;
     MergeRecord *h_merge = nullptr, *d_merge = nullptr;
-    std::vector<double> h_walk;      // alphas: the merged aggregate's neighbour-map walk rows
-    std::vector<int> h_walk_n;
+    // alphas: the merge state and the merged aggregate's neighbour-map walk
+    // rows, copied back into pinned memory (grown to the largest merge seen)
+    unsigned char *h_alpha = nullptr;
+    size_t h_alpha_cap = 0;
     unsigned long long seq = 0;        // step tickets (searches)
     unsigned long long merge_seq = 0;  // merge records
     std::vector<void *> allocs;
@@ -700,6 +704,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     auto *b = new (std::nothrow) mcac_box();
     if (!b) return MCAC_E_CAPACITY;
     b->p = *params;
+    if (const char *e = std::getenv("MCAC_FUSED_MERGE")) b->fused_merge = std::atoi(e) != 0;  // 0: the kernel sequence
     int rc = 0;
     auto fail = [&](int code) {
         mcac_box_destroy(b);
@@ -742,6 +747,7 @@ int mcac_box_destroy(mcac_box *b) {
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->h_step) (void)hipHostFree(b->h_step);
     if (b->h_merge) (void)hipHostFree(b->h_merge);
+    if (b->h_alpha) (void)hipHostFree(b->h_alpha);
     if (b->h_cstart) (void)hipHostFree(b->h_cstart);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1060,7 +1066,10 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     a.volsurf = b->p.volsurf;
     const int n = nk + nr;
     if (b->timing) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    if (a.volsurf == 1 || a.volsurf == 3) {
+    const bool fused = b->fused_merge && a.volsurf != 0 && n <= kFusedMerge;
+    if (fused && a.volsurf != 2) {
+        hipLaunchKernelGGL(k_merge_fused, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+    } else if (a.volsurf == 1 || a.volsurf == 3) {
         // spherical caps or arvo: the two-way contact rows first, then the
         // per-sphere parts, the full ordered sums, dc, rmax and Rg (publishes)
         const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
@@ -1069,8 +1078,8 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
         hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
         hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
         hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
-        if (a.volsurf == 1) hipLaunchKernelGGL(k_caps_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
-        else hipLaunchKernelGGL(k_arvo_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
+        if (a.volsurf == 1) hipLaunchKernelGGL(k_caps_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+        else hipLaunchKernelGGL(k_arvo_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
         hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
         hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
         hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
@@ -1080,27 +1089,49 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
         // the reference); then the centre of mass, dc, rmax and Rg (publishes)
         const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
         const unsigned tiles = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
-        hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_alpha_walk, dim3(nb), dim3(kBlock), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        const size_t off_n = (sizeof(MergeState) + 63) & ~static_cast<size_t>(63);
+        const size_t off_w = off_n + ((static_cast<size_t>(n) * sizeof(int) + 63) & ~static_cast<size_t>(63));
+        const size_t walk_bytes = 3 * static_cast<size_t>(n) * kOverlapSlots * sizeof(double);
+        if (off_w + walk_bytes > b->h_alpha_cap) {
+            const size_t cap = std::max(off_w + walk_bytes, 2 * b->h_alpha_cap);
+            if (b->h_alpha) HIP_TRY(hipHostFree(b->h_alpha));
+            b->h_alpha = nullptr;
+            b->h_alpha_cap = 0;
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&b->h_alpha), cap, hipHostMallocDefault));
+            b->h_alpha_cap = cap;
+        }
+        if (fused) {
+            hipLaunchKernelGGL(k_merge_fused, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        } else {
+            hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
+            hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+            hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
+            hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+            hipLaunchKernelGGL(k_alpha_walk, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+            hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        }
         HIP_TRY(hipGetLastError());
+        // the state and the walk rows in one wait (the rows are unused when the re-check failed)
+        HIP_TRY(hipMemcpyAsync(b->h_alpha, b->d.ms, sizeof(MergeState), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->h_alpha + off_n, b->d.walk_n, static_cast<size_t>(n) * sizeof(int),
+                               hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->h_alpha + off_w, b->d.walk, walk_bytes, hipMemcpyDeviceToHost, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         MergeState ms{};
-        HIP_TRY(hipMemcpy(&ms, b->d.ms, sizeof(MergeState), hipMemcpyDeviceToHost));
+        std::memcpy(&ms, b->h_alpha, sizeof(MergeState));
         if (ms.ok) {
-            b->h_walk_n.resize(static_cast<size_t>(n));
-            b->h_walk.resize(3 * static_cast<size_t>(n) * kOverlapSlots);
-            HIP_TRY(hipMemcpy(b->h_walk_n.data(), b->d.walk_n, static_cast<size_t>(n) * sizeof(int),
-                              hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(b->h_walk.data(), b->d.walk, b->h_walk.size() * sizeof(double), hipMemcpyDeviceToHost));
-            const AlphaFactors f = alpha_factors(n, b->h_walk_n.data(), b->h_walk.data(), ms.sums[0], ms.sums[1]);
-            hipLaunchKernelGGL(k_alpha_store, dim3(1), dim3(64), 0, b->stream, b->d, a, f.volume, f.surface,
-                               f.overlapping, f.coordination, f.any ? 1 : 0);
-            hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-            hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+            const AlphaFactors f = alpha_factors(n, reinterpret_cast<const int *>(b->h_alpha + off_n),
+                                                 reinterpret_cast<const double *>(b->h_alpha + off_w), ms.sums[0],
+                                                 ms.sums[1]);
+            if (fused) {
+                hipLaunchKernelGGL(k_alpha_finish, dim3(1), dim3(kBlock), 0, b->stream, b->d, a, f.volume,
+                                   f.surface, f.overlapping, f.coordination, f.any ? 1 : 0);
+            } else {
+                hipLaunchKernelGGL(k_alpha_store, dim3(1), dim3(64), 0, b->stream, b->d, a, f.volume, f.surface,
+                                   f.overlapping, f.coordination, f.any ? 1 : 0);
+                hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
+                hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+            }
         }
     } else if (n <= kSmallMerge) {
         hipLaunchKernelGGL(k_merge_small, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);

@@ -2,7 +2,10 @@
 caps, alphas and arvo) against the oracle: merges of overlapping clusters
 (several contacts per sphere, so the neighbour-map walk order matters) and
 full runs.  caps and alphas are bit for bit; arvo (device sin, cos, atan) is
-within 1e-10 relative, and bit for bit where no two spheres overlap."""
+within 1e-10 relative, and bit for bit where no two spheres overlap.  Each
+case runs both launch forms of these merges: one workgroup for merged
+aggregates of up to 512 spheres (default) and the kernel sequence for every
+merge (MCAC_FUSED_MERGE=0, read when a box is created)."""
 import numpy as np
 import pytest
 
@@ -88,10 +91,13 @@ def merge_case(seed, sizes, L=4e-6):
     return spheres, aggs, first
 
 
+@pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("volsurf", [1, 2, 3])
-@pytest.mark.parametrize("seed,sizes", [(1, (12, 9, 7)), (2, (40, 30, 300)), (3, (1, 1, 25)), (4, (150, 200, 60))])
-def test_overlap_merges_match_oracle(seed, sizes, volsurf):
-    say(f"case: overlap merges volsurf={volsurf} seed={seed} sizes={sizes}")
+@pytest.mark.parametrize("seed,sizes", [(1, (12, 9, 7)), (2, (40, 30, 300)), (3, (1, 1, 25)), (4, (150, 200, 60)),
+                                        (9, (300, 150, 100))])
+def test_overlap_merges_match_oracle(seed, sizes, volsurf, fused, monkeypatch):
+    say(f"case: overlap merges volsurf={volsurf} seed={seed} sizes={sizes} fused={fused}")
+    monkeypatch.setenv("MCAC_FUSED_MERGE", str(fused))
     M = mcac()
     spheres, aggs, first = merge_case(seed, sizes)
     box = M.Box(4e-6, 10, spheres["r"].size, len(sizes), volsurf=volsurf)
@@ -125,8 +131,10 @@ def test_overlap_merges_match_oracle(seed, sizes, volsurf):
     assert box.aggregates()["count"][0] == sum(sizes)
 
 
+@pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("method", ["caps", "alphas", "arvo"])
-def test_overlap_run_bit_exact(method):
+def test_overlap_run_bit_exact(method, fused, monkeypatch):
     """N=3000 at FV=0.05 with an overlap correction to 60 aggregates: every step and merge identical."""
-    say(f"case: dense N=3000 FV=0.05 {method} -> 60 aggregates")
+    say(f"case: dense N=3000 FV=0.05 {method} fused={fused} -> 60 aggregates")
+    monkeypatch.setenv("MCAC_FUSED_MERGE", str(fused))
     compare_runs((CFG / f"dense_small_{method}.ini").read_text(), 10 ** 7)
