@@ -49,7 +49,10 @@ typedef struct mcac_merge_rec {
  * enough for one workgroup, up to 8 moves are picked ahead and run in one
  * group launch (mcac_move_enqueue_group); depth 1 or an output directory
  * turns this off.  Off by default: on the N=1e6 box it is no faster than the
- * one-launch pipeline, and slower where windows hold a few thousand entries. */
+ * one-launch pipeline, and slower where windows hold a few thousand entries.
+ * MCAC_FUSED_MERGE (default 1; 0 off, read by mcac_box_create): merges with
+ * an overlap correction of at most 512 spheres run in one workgroup launch
+ * instead of the method's kernel sequence (results are identical). */
 int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mcac_sim **out, char *err,
                     int errlen);
 int mcac_sim_destroy(mcac_sim *sim);
