@@ -25,6 +25,8 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = ROOT / "profiles" / "k_step_n1m_hbm_traffic.json"
 WORKLOAD_INI = ROOT / "mcac_amd" / "configs" / "n1m_lognormal_fv3e-3.ini"
 METRIC = "MC steps/s at N=1M; ensemble runs/hour + p99 latency vs plain GPU & CPU"
+MIN_TIMED_STEPS = 20000   # floor of the timed region (about 0.4 s of MC steps at N=1e6)
+MIN_WARMUP_STEPS = 2000
 
 
 def cpu_baseline(text: str, steps: int):
@@ -291,25 +293,31 @@ def main() -> int:
         return overlap_runs()
 
     text = WORKLOAD_INI.read_text()
+    # --steps and --warmup are floors: a timed region of a few MC steps (tens of
+    # microseconds each) would measure launch jitter, not the path
+    steps = max(args.steps, MIN_TIMED_STEPS)
+    warmup = max(args.warmup, MIN_WARMUP_STEPS)
+    stride = max(1, min(args.timing_stride, steps // 16))
     print("bench: placing N=1e6 monomers", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     sim = mcac_amd.Simulation(text, device=0)
     t_init = time.perf_counter() - t0
-    print("bench: warm-up", file=sys.stderr, flush=True)
-    done = sim.run(args.warmup)
-    if done != args.warmup:
+    print(f"bench: warm-up, {warmup} MC steps", file=sys.stderr, flush=True)
+    done = sim.run(warmup)
+    if done != warmup:
         raise RuntimeError("warm-up ended early")
     box = sim.box()
-    box.set_timing(max(1, args.timing_stride))
+    box.set_timing(stride)  # the first search of the timed region is timed, then every stride-th
     s0 = {k: sim.scalar(k) for k in ("query", "filtered", "pairs", "merges", "t_host_pick_ms", "t_host_merge_ms",
                                      "t_device_ms", "pick_rescales", "ties", "t_tie_ms", "t_enqueue_ms", "t_wait_ms",
                                      "t_merge_call_ms", "t_drain_ms")}
+    print(f"bench: timed region, {steps} MC steps", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    done = sim.run(args.steps)
+    done = sim.run(steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    if done != args.steps:
+    if done != steps:
         raise RuntimeError("timed run ended early")
     s1 = {k: sim.scalar(k) for k in s0}
     tm = box.timing()
@@ -317,7 +325,7 @@ def main() -> int:
     st = box.stats()
     prune = box.prune_stats() if args.precision > 0 else None
     d = {k: s1[k] - s0[k] for k in s0}
-    steps_per_s = args.steps / elapsed
+    steps_per_s = steps / elapsed
     # dominant kernel: the contact search (k_step, or k_filter + k_solve for
     # large movers), HBM-bound.  Algorithmic bytes per launch: 36 B per
     # aggregate of the visited Verlet cells (CoM x,y,z + rmax 8 B each, 4 B id).
@@ -326,21 +334,21 @@ def main() -> int:
     if TRAFFIC_FILE.exists():  # rocprofv3 FETCH_SIZE + WRITE_SIZE per k_step dispatch, copy-calibrated
         traffic = json.loads(TRAFFIC_FILE.read_text()).get("k_step_bytes_per_dispatch")
     n_search = max(tm["n_search"], 1.0)
-    if tm["n_search"] < 1:
-        raise RuntimeError("no timed contact search: lower --timing-stride")
-    filt_ms = tm["filter_ms"] / n_search
-    solve_ms = tm["solve_ms"] / n_search
-    bytes_per_launch = 36.0 * d["query"] / args.steps
-    achieved = bytes_per_launch / (filt_ms * 1e-3) / 1e9
+    filt_ms = tm["filter_ms"] / n_search if tm["n_search"] >= 1 else None
+    solve_ms = tm["solve_ms"] / n_search if tm["n_search"] >= 1 else None
+    bytes_per_launch = 36.0 * d["query"] / steps
+    achieved = bytes_per_launch / (filt_ms * 1e-3) / 1e9 if filt_ms else None
     pair_tests = d["query"] + d["pairs"]
     out = {
         "metric": METRIC,
         "value": steps_per_s,
         "unit": "MC steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -348,23 +356,25 @@ def main() -> int:
         "data": "synthetic (seeded lognormal monomers placed by the reference algorithm)",
         "config": {"workload": "N=1e6 spheres lognormal 30nm/1.25, FV=3e-3, 1700 K, "
                                + ("plain FP64 path, " if args.precision == 0 else "FP32 broad phase + FP64 solve, ")
-                               + f"MC steps {args.warmup}..{args.warmup + args.steps} from t=0",
+                               + f"MC steps {warmup}..{warmup + steps} from t=0",
                    "n_spheres": 1_000_000, "parallelism": "single box, 1 GPU", "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": "k_step" if one_launch else "k_filter", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": filt_ms},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": filt_ms,
+                     "timed_launches": int(tm["n_search"]),
+                     "note": None if achieved else "no search was timed (events on every n-th search)"},
         "candidate_pair_tests_per_s": pair_tests / elapsed,
-        "breakdown": {"init_s": t_init, "host_pick_ms_per_step": d["t_host_pick_ms"] / args.steps,
-                      "host_merge_ms_per_step": d["t_host_merge_ms"] / args.steps,
+        "breakdown": {"init_s": t_init, "host_pick_ms_per_step": d["t_host_pick_ms"] / steps,
+                      "host_merge_ms_per_step": d["t_host_merge_ms"] / steps,
                       "pick_rescales": d["pick_rescales"], "pick_ties": d["ties"],
-                      "host_us_per_step": {k[2:-3]: 1e3 * d[k] / args.steps for k in
+                      "host_us_per_step": {k[2:-3]: 1e3 * d[k] / steps for k in
                                            ("t_tie_ms", "t_enqueue_ms", "t_wait_ms", "t_merge_call_ms", "t_drain_ms")},
-                      "device_call_ms_per_step": d["t_device_ms"] / args.steps,
+                      "device_call_ms_per_step": d["t_device_ms"] / steps,
                       "search_launch1_ms": filt_ms, "search_launch2_ms": solve_ms,
                       "k_translate_ms": tm["translate_ms"] / max(tm["n_translate"], 1.0),
                       "merge_ms": tm["merge_ms"] / max(tm["n_merge"], 1.0),
-                      "merges": d["merges"], "avg_candidates_visited": d["query"] / args.steps,
-                      "avg_candidates_filtered": d["filtered"] / args.steps,
+                      "merges": d["merges"], "avg_candidates_visited": d["query"] / steps,
+                      "avg_candidates_filtered": d["filtered"] / steps,
                       "timed_searches": tm["n_search"], "launch_stats": st, "fp32_prune": prune},
     }
     if args.ensemble_boxes > 0:

@@ -215,8 +215,8 @@ int mcac_box_sphere_owners(mcac_box *box, int64_t *owner);
 /* Members of one aggregate in list order; returns the count via *n. */
 int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int64_t *n);
 
-/* Kernel timing on the box's stream.  enable = n >= 1 times every n-th
- * search with start/stop events stamped by the runtime at the kernel's own
+/* Kernel timing on the box's stream.  enable = n >= 1 times the first search
+ * enqueued after the call and every n-th one after it, with start/stop events stamped by the runtime at the kernel's own
  * start and end (hipExtLaunchKernel), and every merge launch with events
  * around it; 0 turns timing off.  The host never blocks on the events of a
  * search.  mcac_box_timing returns accumulated milliseconds and counts:

@@ -39,7 +39,7 @@ int main(int argc, char **argv) {
     while (!mcac_sim_finished(sim)) {
         rc = mcac_sim_run(sim, 100000, &done);
         total += done;
-        if (rc) break;
+        if (rc || done == 0) break;  // an error, or a run that can make no further step
         std::printf("steps %lld  aggregates %.0f  time %.6e s\n", static_cast<long long>(total),
                     mcac_sim_scalar(sim, "n_agg"), mcac_sim_scalar(sim, "time"));
         std::fflush(stdout);

@@ -93,6 +93,7 @@ struct mcac_box {
     unsigned long long merge_seq = 0;  // merge records
     std::vector<void *> allocs;
     int timing = 0;                  // 0 off; n >= 1: every n-th search is timed
+    unsigned long long timing_base = 1;  // first ticket after timing was set: timed
     int slot_timed[kRing] = {0};     // 1 timed, 2 timed and consumed, events not read yet
     hipEvent_t ev[4]{};
     hipEvent_t sev[kRing][4]{};      // per-slot events of pipelined steps
@@ -312,7 +313,7 @@ double window_items(const mcac_box *b, double reach) {
 bool rebuild_due(const mcac_box *b) { return b->dirty_count + 3 >= rebuild_threshold(b); }
 
 bool next_step_timed(const mcac_box *b) {
-    return b->timing > 0 && (b->seq + 1) % static_cast<unsigned long long>(b->timing) == 0;
+    return b->timing > 0 && (b->seq + 1 - b->timing_base) % static_cast<unsigned long long>(b->timing) == 0;
 }
 
 // Picks the launch of one step and enqueues it.  With job_out (given only for
@@ -405,7 +406,7 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     }
     __atomic_store_n(&b->h_step[slot].seq, 0ull, __ATOMIC_RELEASE);
     hipEvent_t *ev = b->sev[slot];
-    const bool timed = b->timing > 0 && seq % static_cast<unsigned long long>(b->timing) == 0;
+    const bool timed = b->timing > 0 && (seq - b->timing_base) % static_cast<unsigned long long>(b->timing) == 0;
     // one launch when the mover is small enough for the finishing block
     const bool fused = n_mov <= kStepMover && static_cast<long long>(n_mov) * b->max_count <= kStepPairs;
     // filter grid sized to the expected work: entries of the visited cells
@@ -1455,6 +1456,7 @@ int mcac_box_sphere_owners(mcac_box *b, int64_t *owner) {
 int mcac_box_set_timing(mcac_box *b, int enable) {
     if (!b) return MCAC_E_ARGUMENT;
     b->timing = enable < 0 ? 0 : enable;
+    b->timing_base = b->seq + 1;  // the next search is the first timed one
     for (int k = 0; k < mcac_box::kRing; ++k) b->slot_timed[k] = 0;
     b->t_search = b->t_translate = b->t_merge = b->t_filter = b->t_solve = 0;
     b->n_search = b->n_translate = b->n_merge = 0;

@@ -302,17 +302,6 @@ def test_fp32_broad_phase_bit_exact(cfg, monkeypatch):
     assert ps["tested"] > 0 and ps["missed"] == 0 and ps["kept"] < ps["tested"]  # counted with precision 2
 
 
-@pytest.mark.parametrize("cfg", ["lognormal_small.ini", "dense_small.ini"])
-def test_fp32_broad_phase_bit_exact(cfg, monkeypatch):
-    """FP32 broad phase (precision 2: misses counted): same steps and merges as the oracle, no miss."""
-    say(f"case: FP32 broad phase, {cfg}")
-    monkeypatch.setenv("MCAC_PRECISION", "2")
-    ora, sim = compare_runs((CFG / cfg).read_text(), 10 ** 7)
-    ps = sim.box().prune_stats()
-    say("prune stats", ps)
-    assert ps["tested"] > 0 and ps["missed"] == 0 and ps["kept"] < ps["tested"]  # counted with precision 2
-
-
 @pytest.mark.parametrize("depth", [1, 2, 5])
 def test_pipeline_depths_bit_exact(depth, monkeypatch):
     """Steps kept in flight (1, 2, 5) do not change any step or merge."""
