@@ -122,11 +122,17 @@ def _pct(x, q):
 
 
 def run_ensemble(args) -> dict:
-    """Batched form: the first K boxes of the 512-box list, W boxes at a time on the GPU."""
+    """Batched form on the 512-box list (the first --ensemble-boxes of it): every
+    box run to N_agg <= N/10 from t=0, --workers host threads each stepping
+    --group-boxes boxes in lock step.  The plain GPU path one box at a time is
+    measured on every --plain-stride-th box of the list and scaled to the whole
+    list by MC steps (its MC steps per second of service, times the list's MC
+    steps); the sampled boxes must take the same step counts in both forms."""
     import numpy as np
     from mcac_amd import ensemble
     boxes = ensemble.ensemble_configs(args.ensemble_boxes)
-    print(f"bench: ensemble of {len(boxes)} boxes, {args.workers} workers", file=sys.stderr, flush=True)
+    print(f"bench: ensemble of {len(boxes)} boxes, {args.workers} workers x {args.group_boxes} boxes",
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     r = ensemble.batch_run([b["ini"] for b in boxes], workers=args.workers, max_steps=args.max_job_steps,
                            boxes_per_worker=args.group_boxes, depth=args.group_depth)
@@ -135,18 +141,26 @@ def run_ensemble(args) -> dict:
     dur = (r["end"] - r["start"])[done]
     out = {"boxes": len(boxes), "of": 512, "completed": len(done), "workers": args.workers,
            "boxes_per_worker": args.group_boxes, "steps_per_launch": args.group_depth, "wall_s": wall,
-           "runs_per_hour": 3600.0 * len(done) / wall, "mc_steps_per_s": float(r["steps"].sum()) / wall,
+           "runs_per_hour": 3600.0 * len(done) / wall, "mc_steps": int(r["steps"].sum()),
+           "mc_steps_per_s": float(r["steps"].sum()) / wall,
            "box_time_p50_s": _pct(dur, 50), "box_time_p99_s": _pct(dur, 99)}
     if args.plain_ensemble:
-        # the plain GPU path one run at a time: one worker, one box, the single-box pipeline
-        print("bench: the same boxes one at a time", file=sys.stderr, flush=True)
+        idx = list(range(0, len(boxes), max(1, args.plain_stride)))
+        print(f"bench: {len(idx)} of the boxes one at a time (plain path)", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        p = ensemble.batch_run([b["ini"] for b in boxes], workers=1, max_steps=args.max_job_steps)
+        p = ensemble.batch_run([boxes[i]["ini"] for i in idx], workers=1, max_steps=args.max_job_steps)
         pw = time.perf_counter() - t0
-        same = bool(np.array_equal(p["steps"], r["steps"]))
-        out["plain_one_at_a_time"] = {"wall_s": pw, "runs_per_hour": 3600.0 * len(done) / pw,
-                                      "same_step_counts": same}
-        out["vs_plain_one_at_a_time"] = pw / wall
+        service = float((p["end"] - p["start"]).sum())
+        rate = float(p["steps"].sum()) / service
+        est = float(r["steps"].sum()) / rate
+        out["plain_one_at_a_time"] = {
+            "sample_boxes": len(idx), "sample_mc_steps": int(p["steps"].sum()), "sample_service_s": service,
+            "sample_wall_s": pw, "mc_steps_per_s": rate, "estimated_total_s": est,
+            "runs_per_hour": 3600.0 * len(done) / est,
+            "same_step_counts": bool(np.array_equal(p["steps"], r["steps"][idx])),
+            "method": f"every {args.plain_stride}th box of the list run one at a time on the single-box "
+                      "pipeline; the list's MC steps divided by the sample's MC steps per second of service"}
+        out["vs_plain_one_at_a_time"] = est / wall
     return out | {
         "sample": f"boxes 0..{len(boxes) - 1} of the 512-box list (N log-uniform 500..5e4, normal 30/3 nm, "
                   f"FV U[1e-4,5e-3], T U[1000,2000] K, until N_agg <= N/10), all started at t=0"}
@@ -161,7 +175,8 @@ def run_ensemble_compare(args) -> dict:
     from mcac_amd import ensemble
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib
-    boxes = [b for b in ensemble.ensemble_configs(args.ensemble_boxes) if b["n"] <= args.compare_nmax]
+    boxes = [b for b in ensemble.ensemble_configs(args.ensemble_boxes)[::max(1, args.compare_stride)]
+             if b["n"] <= args.compare_nmax]
     inis = [b["ini"] for b in boxes]
     print(f"bench: ensemble comparison on {len(boxes)} boxes (N <= {args.compare_nmax:g})", file=sys.stderr,
           flush=True)
@@ -184,8 +199,9 @@ def run_ensemble_compare(args) -> dict:
     return {"boxes": len(boxes), "gpu_runs_per_hour": 3600.0 * len(boxes) / gpu_wall, "gpu_wall_s": gpu_wall,
             "gpu_workers": args.workers, "gpu_boxes_per_worker": args.group_boxes, "cpu_runs_per_hour": 3600.0 * len(boxes) / cpu_wall, "cpu_wall_s": cpu_wall,
             "cpu_threads": procs, "ratio": cpu_wall / gpu_wall, "same_step_counts": same,
-            "sample": f"boxes with N <= {args.compare_nmax:g} among boxes 0..{args.ensemble_boxes - 1} of the "
-                      f"512-box list, run to N_agg <= N/10 on both sides; CPU = the oracle, one thread per run"}
+            "sample": f"boxes with N <= {args.compare_nmax:g} among every {args.compare_stride}th of boxes "
+                      f"0..{args.ensemble_boxes - 1} of the 512-box list, run to N_agg <= N/10 on both sides; "
+                      f"CPU = the oracle, one thread per run"}
 
 
 def run_serving(args) -> dict:
@@ -238,8 +254,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 20 times (HBM counter calibration)")
-    ap.add_argument("--ensemble-boxes", type=int, default=16,
+    ap.add_argument("--ensemble-boxes", type=int, default=512,
                     help="boxes of the 512-box ensemble list run to completion (0: skip)")
+    ap.add_argument("--plain-stride", type=int, default=16,
+                    help="every n-th box of the ensemble list is also run one at a time on the plain path")
+    ap.add_argument("--compare-stride", type=int, default=16,
+                    help="every n-th box of the ensemble list (with N <= --compare-nmax) is run on the CPU too")
     ap.add_argument("--no-plain-ensemble", dest="plain_ensemble", action="store_false",
                     help="skip running the ensemble sample again one box at a time")
     ap.add_argument("--compare-nmax", type=float, default=8000,

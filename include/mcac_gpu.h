@@ -206,6 +206,10 @@ int mcac_box_block_steps(mcac_box *box, double reach);
 
 int mcac_box_download_spheres(mcac_box *box, double *x, double *y, double *z, double *rx, double *ry,
                               double *rz);
+/* Radius, volume and surface of every sphere (index order; NULL skips an
+ * array): the SphereList fields that surface growth changes
+ * (sphere.cpp:113-120). */
+int mcac_box_download_radii(mcac_box *box, double *r, double *vol, double *surf);
 int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rmax, double *volume,
                            double *surface, double *rg, int64_t *count);
 /* Stable id of the aggregate owning each sphere (n_spheres entries; -1 for a
@@ -214,6 +218,40 @@ int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rm
 int mcac_box_sphere_owners(mcac_box *box, int64_t *owner);
 /* Members of one aggregate in list order; returns the count via *n. */
 int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int64_t *n);
+
+/* Surface growth of the whole box and the aggregate update that follows it
+ * (calcul.cpp:155-205 with individual_surf_reactions off and
+ * full_aggregate_update_frequency 1, volsurf none): every sphere of a live
+ * aggregate grows by dr = u_sg * dt (Sphere::croissance_surface,
+ * sphere.cpp:113-120, PhysicalModel::grow, physical_model.cpp:587-590), then
+ * every live aggregate gets Aggregate::update (aggregat.cpp:284-288, 321-483:
+ * volume, surface, centre of mass and position, rmax, Rg).  The outputs are
+ * indexed by stable id (arrays of the box's aggregate capacity; dead ids are
+ * not written): volume, surface, list-order radius sum (dp = 2 sum / n), Rg,
+ * rmax.  The caller then merges a contact (its geometry comes from the grown
+ * spheres, as AggregatList::merge runs after the growth) and computes the
+ * physics.  No step may be in flight and no inline merge pending. */
+int mcac_box_grow(mcac_box *box, double dr, double *volume, double *surface, double *radius_sum, double *rg,
+                  double *rmax);
+
+/* Persistent step engine: with blocks >= 1 (at most 255), queued one-launch
+ * steps (mcac_move_enqueue of a mover the step's own finishing block moves,
+ * not timed) are posted to a ring in pinned host memory and taken in order by
+ * a resident grid of that many workgroups (fewer while the steps' windows
+ * are small), instead of one kernel launch each; results, tickets, gate and
+ * skipped steps are exactly as with launches.  Every other operation on the
+ * box first ends the resident grid, which is launched again at the next
+ * step; a grid with no step for 0.5 s ends by itself.  Call it again (with
+ * the same count) to end the resident grid, so that the device is idle (for
+ * a device-wide synchronize), or with 0 to turn the engine off.  Not for a
+ * box in a group.  No counterpart in the reference, which calls its search
+ * once per step (src/calcul.cpp:127). */
+int mcac_box_set_engine(mcac_box *box, int blocks);
+/* [grids launched, steps posted, steps completed by the engine, device ms of
+ * those steps (each from when it could start: its job taken and the previous
+ * step done, to its record), grids launched again after an idle end, the last
+ * grid's end reason (2 idle, 3 ended by the host, -1 a wait ran out)]. */
+int mcac_box_engine_stats(mcac_box *box, double out[6]);
 
 /* Kernel timing on the box's stream.  enable = n >= 1 times the first search
  * enqueued after the call and every n-th one after it, with start/stop events stamped by the runtime at the kernel's own

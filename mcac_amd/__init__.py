@@ -57,6 +57,8 @@ GPU_SYMBOLS = (
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
     "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
+    "mcac_box_set_engine", "mcac_box_engine_stats", "mcac_box_grow",
+    "mcac_box_download_radii",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
@@ -117,6 +119,10 @@ def load_gpu():
         L.mcac_box_members.argtypes = [_P, _I64, _I64P, _I64, _I64P]
         L.mcac_box_sphere_owners.argtypes = [_P, _I64P]
         L.mcac_box_set_timing.argtypes = [_P, _I]
+        L.mcac_box_set_engine.argtypes = [_P, _I]
+        L.mcac_box_grow.argtypes = [_P, _D] + [_DP] * 5
+        L.mcac_box_download_radii.argtypes = [_P] + [_DP] * 3
+        L.mcac_box_engine_stats.argtypes = [_P, _DP]
         L.mcac_box_timing.argtypes = [_P, _DP]
         L.mcac_box_stats.argtypes = [_P, _I64P]
         L.mcac_box_prune_stats.argtypes = [_P, _I64P]
@@ -316,6 +322,13 @@ class Box:
         _check(load_gpu().mcac_box_download_spheres(self.h, *(_dp(a) for a in arrs)), "download_spheres")
         return dict(zip(("x", "y", "z", "rx", "ry", "rz"), arrs))
 
+    def radii(self):
+        """Radius, volume and surface of every sphere (they change with surface growth)."""
+        n = self.n_spheres
+        r, v, s = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(load_gpu().mcac_box_download_radii(self.h, _dp(r), _dp(v), _dp(s)), "mcac_box_download_radii")
+        return dict(r=r, vol=v, surf=s)
+
     def aggregates(self):
         n = self.n_aggs
         pos, rcm = np.zeros((n, 3)), np.zeros((n, 3))
@@ -348,6 +361,16 @@ class Box:
         out = np.zeros(3, np.int64)
         _check(load_gpu().mcac_box_prune_stats(self.h, _ip(out)), "mcac_box_prune_stats")
         return dict(zip(("tested", "kept", "missed"), (int(v) for v in out)))
+
+    def set_engine(self, blocks: int):
+        """Queued one-launch steps go to a resident grid of `blocks` workgroups (0: one launch each)."""
+        _check(load_gpu().mcac_box_set_engine(self.h, int(blocks)), "mcac_box_set_engine")
+
+    def engine_stats(self):
+        out = np.zeros(6)
+        _check(load_gpu().mcac_box_engine_stats(self.h, _dp(out)), "mcac_box_engine_stats")
+        keys = ("launches", "posted", "steps", "busy_ms", "relaunches", "end_reason")
+        return dict(zip(keys, out.tolist()))
 
     def set_auto_merge(self, enable: bool):
         """Queued moves that find a contact merge in the same launch (small aggregates)."""

@@ -36,6 +36,12 @@ struct SearchState {
     unsigned long long gate;   // seq of a gated step that found a contact (0: open)
     unsigned long long tick;   // k_step blocks: finished << 56 | appended << 32 | visited
     unsigned long long fp32_tested, fp32_kept, fp32_missed;  // FP32 broad-phase accounting
+    // persistent step engine (k_step_engine)
+    unsigned long long eng_done;      // last job index whose finishing block completed the step
+    unsigned long long eng_decision;  // job index * 4 + outcome (kEngineRun, kEngineQuit, kEngineExit), block 0
+    unsigned long long eng_last;      // clock of the last completed step
+    unsigned long long eng_busy;      // clock ticks of completed steps (from when each could start)
+    unsigned long long eng_steps;     // steps completed by the engine
 };
 
 // Record published to pinned host memory by the last block of a search.
@@ -104,6 +110,8 @@ struct DevBox {
     int *walk_n;               // entries of each walk row (alphas only)
     SearchState *ss;
     MergeState *ms;
+    struct EngineJob *eng_jobs;        // persistent engine: block 0's copy of each queued job, by ring slot
+    unsigned long long *eng_t;         // persistent engine: clock when each ring slot's job was taken
 };
 
 struct SearchArgs {
@@ -143,6 +151,25 @@ struct SearchArgs {
 struct CellTable {
     int n;
     int c0[64], sz[64];  // first CSR entry and entry count of visited cell v
+    int pad;
+};
+
+// Persistent step engine (k_step_engine): queued steps of one box in a ring
+// of pinned host memory, taken in order by a resident grid.
+constexpr int kEngineRing = 64;  // ring slots (more than the steps in flight, 8, many times over)
+constexpr int kEngineRun = 1, kEngineQuit = 2, kEngineExit = 3;
+struct EngineJob {
+    SearchArgs a;
+    CellTable cells;           // bounds of the visited cells (n = 0: the kernel reads cstart)
+    int cmd;                   // kEngineRun, or kEngineExit: the grid ends at this job
+    int pad;
+    unsigned long long seq;    // job index, written last by the host (release)
+};
+// Written by the engine into pinned host memory when its grid ends.
+struct EngineCtl {
+    unsigned long long stopped_at;  // first job index this launch did not take
+    unsigned long long launches_ended;  // number of the last launch that ended (launches count from 1)
+    int reason;                     // kEngineQuit (idle), kEngineExit (host), or -1 (a wait ran out)
     int pad;
 };
 

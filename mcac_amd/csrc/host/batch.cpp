@@ -7,6 +7,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <thread>
 #include <vector>
@@ -50,6 +51,27 @@ class HbmBudget {
     std::atomic<int64_t> used_{0}, peak_{0};
 };
 
+// With MCAC_BATCH_PROGRESS=k (k >= 1) every k-th finished job prints a line
+// to stderr (jobs finished, seconds since the call), so that a long batch shows
+// its progress; off by default.
+class Progress {
+  public:
+    Progress(int64_t n_jobs) : n_(n_jobs) {
+        if (const char *e = std::getenv("MCAC_BATCH_PROGRESS")) every_ = std::max(0, std::atoi(e));
+    }
+    void job_done(double t) {
+        const int64_t d = done_.fetch_add(1) + 1;
+        if (every_ > 0 && (d % every_ == 0 || d == n_))
+            std::fprintf(stderr, "batch: %lld/%lld jobs done at %.1f s\n", static_cast<long long>(d),
+                         static_cast<long long>(n_), t);
+    }
+
+  private:
+    int64_t n_;
+    int every_ = 0;
+    std::atomic<int64_t> done_{0};
+};
+
 }  // namespace
 
 extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs, int workers,
@@ -57,6 +79,7 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
                               int64_t *steps, int *status, int64_t *peak_bytes) {
     if (!ini_texts || n_jobs < 0 || workers < 1 || !start_s || !end_s || !steps || !status) return MCAC_E_ARGUMENT;
     HbmBudget budget(hbm_budget, ini_texts, n_jobs);
+    Progress progress(n_jobs);
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
     auto now_s = [t0] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
@@ -81,6 +104,7 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
             budget.give(j);
             end_s[j] = now_s();
             status[j] = rc;
+            progress.job_done(end_s[j]);
             if (rc) std::fprintf(stderr, "mcac_batch_run: job %lld status %d %s\n", static_cast<long long>(j), rc, err);
         }
     };
@@ -106,6 +130,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
         depth > 8 || !start_s || !end_s || !steps || !status)
         return MCAC_E_ARGUMENT;
     HbmBudget budget(hbm_budget, ini_texts, n_jobs);
+    Progress progress(n_jobs);
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
     auto now_s = [t0] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
@@ -132,6 +157,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
             budget.give(j);
             end_s[j] = now_s();
             status[j] = rc;
+            progress.job_done(end_s[j]);
             if (rc) std::fprintf(stderr, "mcac_batch_run_groups: job %lld status %d\n", static_cast<long long>(j), rc);
             act[k] = act.back();
             act.pop_back();
@@ -175,6 +201,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                     if (final_time) final_time[j] = 0.0;
                     end_s[j] = now_s();
                     status[j] = rc;
+                    progress.job_done(end_s[j]);
                     std::fprintf(stderr, "mcac_batch_run_groups: job %lld status %d %s\n", static_cast<long long>(j),
                                  rc, err);
                     continue;

@@ -124,9 +124,10 @@ void PhysicalModel::load(const IniFile &ini) {
     ini.extract("environment", "fractal_prefactor", fractal_prefactor);
     ini.extract("environment", "fractal_dimension", fractal_dimension);
     bool flag = false;
-    ini.extract("surface_growth", "with_surface_reactions", flag);
-    if (flag) unsupported = "with_surface_reactions";
+    ini.extract("surface_growth", "with_surface_reactions", with_surface_reactions);
     ini.extract("surface_growth", "flux_surfgrowth", flux_surfgrowth);
+    ini.extract("surface_growth", "full_aggregate_update_frequency", full_aggregate_update_frequency);
+    ini.extract("numerics", "individual_surf_reactions", individual_surf_reactions);
     s.clear();
     if (ini.get("surface_growth", "volsurf_method", s) && !s.empty()) {  // constants.hpp:103-111
         if (s == "caps") volsurf = 1;
@@ -134,6 +135,15 @@ void PhysicalModel::load(const IniFile &ini) {
         else if (s == "arvo") volsurf = 3;
         else if (s != "none") unsupported = "volsurf_method=" + s;
     }
+    // surface growth (calcul.cpp:155-205) runs with a non-negative flux, the
+    // whole-box update every step and no overlap correction; oxidation (a
+    // negative flux: sphere removal and split), per-aggregate reactions and
+    // sparser full updates are not offered
+    if (with_surface_reactions && flux_surfgrowth < 0) unsupported = "flux_surfgrowth<0 (oxidation)";
+    if (with_surface_reactions && individual_surf_reactions) unsupported = "individual_surf_reactions";
+    if (with_surface_reactions && full_aggregate_update_frequency != 1)
+        unsupported = "full_aggregate_update_frequency!=1";
+    if (with_surface_reactions && volsurf != 0) unsupported = "volsurf_method with surface reactions";
     flag = false;
     ini.extract("nucleation", "with_nucleation", flag);
     if (flag) unsupported = "with_nucleation";

@@ -86,6 +86,9 @@ struct PhysicalModel {
     bool with_domain_duplication = true, with_collisions = true, enforce_volume_fraction = true;
     std::string output_dir = "MCAC_output";
     int volsurf = 0;          // volsurf_method: 0 none, 1 caps (spherical caps), 2 alphas, 3 arvo
+    bool with_surface_reactions = false;  // surface growth (physical_model.cpp:79, 91, 127-159)
+    size_t full_aggregate_update_frequency = 1;
+    bool individual_surf_reactions = false;
     std::string unsupported;  // first option this path does not implement
 
     void load(const IniFile &ini);

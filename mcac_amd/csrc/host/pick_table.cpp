@@ -153,13 +153,58 @@ inline double two_sum_err(double a, double b, double s) {
 
 }  // namespace
 
-void PickTable::build(const std::vector<int> &ids, const std::vector<double> &ts_by_id) {
-    std::vector<int> order(ids);
-    std::sort(order.begin(), order.end(), [&ts_by_id](int a, int b) {
+void host_parallel_for(size_t n, const std::function<void(size_t)> &fn) {
+    auto f = [&fn](size_t k) { fn(k); };
+    Workers::get().run(n, f);
+}
+
+void PickTable::build(const std::vector<int> &ids, const std::vector<double> &ts_by_id, double factor) {
+    // the previous order, when it holds the same ids, is nearly sorted after a
+    // small change of every time step: insertion sort from it (the order is
+    // strict, so any sort gives the same result)
+    std::vector<int> order;
+    order.reserve(ids.size());
+    bool reuse = n_ > 0 && !ids.empty();
+    if (reuse) {
+        // the previous order restricted to the ids given, then the ids it lacks
+        int top = 0;
+        for (int id : ids) top = std::max(top, id);
+        for (const Chunk &c : chunks_)
+            for (int id : c.id) top = std::max(top, id);
+        std::vector<char> given(static_cast<size_t>(top) + 1, 0), kept(static_cast<size_t>(top) + 1, 0);
+        for (int id : ids) given[static_cast<size_t>(id)] = 1;
+        for (const Chunk &c : chunks_)
+            for (int id : c.id)
+                if (given[static_cast<size_t>(id)]) {
+                    order.push_back(id);
+                    kept[static_cast<size_t>(id)] = 1;
+                }
+        for (int id : ids)
+            if (!kept[static_cast<size_t>(id)]) order.push_back(id);
+        reuse = order.size() == ids.size();
+    }
+    auto less = [&ts_by_id](int a, int b) {
         return before(ts_by_id[static_cast<size_t>(a)], a, ts_by_id[static_cast<size_t>(b)], b);
-    });
+    };
+    if (reuse) {
+        size_t moves = 0;
+        for (size_t i = 1; i < order.size() && reuse; ++i) {
+            const int v = order[i];
+            size_t j = i;
+            while (j > 0 && less(v, order[j - 1])) {
+                order[j] = order[j - 1];
+                --j;
+                if (++moves > 64 * order.size()) reuse = false;  // far from sorted: sort instead
+            }
+            order[j] = v;
+        }
+    }
+    if (!reuse) {
+        order.assign(ids.begin(), ids.end());
+        std::sort(order.begin(), order.end(), less);
+    }
     n_ = order.size();
-    factor_ = n_ ? ts_by_id[static_cast<size_t>(order[0])] : 0.0;
+    factor_ = factor > 0.0 ? factor : (n_ ? ts_by_id[static_cast<size_t>(order[0])] : 0.0);
     chunks_.clear();
     for (size_t i = 0; i < n_; i += chunk_) {
         Chunk c;

@@ -20,16 +20,23 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <utility>
 #include <vector>
 
 namespace mcac {
 
+// fn(k) for k in [0, n) on the host worker pool (up to 8 threads, the caller
+// included); serially when another thread holds the pool.
+void host_parallel_for(size_t n, const std::function<void(size_t)> &fn);
+
 class PickTable {
   public:
     explicit PickTable(size_t chunk = 2048) : chunk_(chunk < 16 ? 16 : chunk) {}
-    // ids in label order and time steps indexed by id
-    void build(const std::vector<int> &ids, const std::vector<double> &ts_by_id);
+    // ids in label order and time steps indexed by id; factor > 0: the keys
+    // are factor / time step (the reference's max_time_step, which it
+    // refreshes only at events), else the largest time step
+    void build(const std::vector<int> &ids, const std::vector<double> &ts_by_id, double factor = 0.0);
     // keeper: old -> new time step; removed leaves the table
     void replace(int keep, double keep_old, double keep_new, int rem, double rem_ts);
     double factor() const { return factor_; }

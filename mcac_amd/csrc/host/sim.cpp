@@ -118,6 +118,7 @@ struct Simulation {
     mcac_group *group = nullptr;  // lock-step form: the box works on the group's stream
     mcac_group *own_group = nullptr;  // one-box group for rounds of steps in one launch
     bool rounds = false;              // issue rounds through own_group when the steps fit one workgroup
+    int engine_blocks = 128;          // persistent step engine grid (mcac_box_set_engine), 0: off
     std::vector<int64_t> r_aggs;      // scratch of issue_round
     std::vector<double> r_dirs, r_lpm, r_maxr;
     std::vector<uint64_t> r_tickets;
@@ -143,7 +144,10 @@ struct Simulation {
     double t_tie = 0, t_enqueue = 0, t_wait = 0, t_merge_call = 0, t_drain = 0;
     bool initialized = false;
     bool final_written = false;
-    std::vector<double> sphere_r;                            // radii (constant on this path)
+    std::vector<double> sphere_r;                            // radii (grow with surface reactions)
+    // surface growth: per-id outputs of mcac_box_grow
+    std::vector<double> g_vol, g_surf, g_rsum, g_rg, g_rmax;
+    double t_grow = 0;
     std::unique_ptr<SnapshotWriter> out_spheres, out_aggs;  // snapshot files when out_dir is set
 
     ~Simulation() {
@@ -349,6 +353,20 @@ struct Simulation {
         if (rc) return rc;
         rc = mcac_box_set_auto_merge(box, 1);  // every contact merges (calcul.cpp:156-160)
         if (rc) return rc;
+        if (pm.with_surface_reactions) {
+            // every step grows the whole box and updates every aggregate, so a
+            // step depends on the one before it: no look-ahead, and a contact
+            // merges after the growth (calcul.cpp:164-181)
+            depth = 1;
+            engine_blocks = 0;
+            rc = mcac_box_set_auto_merge(box, 0);
+            if (rc) return rc;
+            for (auto *v : {&g_vol, &g_surf, &g_rsum, &g_rg, &g_rmax}) v->assign(N, 0.0);
+        }
+        if (!group && !own_group && engine_blocks > 0) {
+            rc = mcac_box_set_engine(box, engine_blocks);
+            if (rc) return rc;
+        }
         pick.build(live.ids(), ts);  // first sort_time_steps (calcul.cpp:99-101)
         event = true;
         initialized = true;
@@ -397,6 +415,10 @@ struct Simulation {
         std::vector<int64_t> owner(S);
         rc = mcac_box_sphere_owners(box, owner.data());
         if (rc) return rc;
+        if (pm.with_surface_reactions) {  // the radii have grown since the placement
+            rc = mcac_box_download_radii(box, sphere_r.data(), nullptr, nullptr);
+            if (rc) return rc;
+        }
         std::vector<double> pos3(3 * S);
         std::vector<int64_t> slabel(S);
         std::vector<int32_t> zeros_s(S, 0);
@@ -631,6 +653,43 @@ struct Simulation {
         return 0;
     }
 
+    // AggregatList::croissance_surface(dt) and the update of every aggregate
+    // (calcul.cpp:164-205): the device grows the spheres and returns each
+    // aggregate's volume, surface, radius sum, Rg and rmax; the physics
+    // (update_partial, aggregat.cpp:254-281) runs here with the host's libm.
+    int grow(double dt) {
+        auto t0 = Clock::now();
+        const double dr = pm.u_sg * dt;  // PhysicalModel::grow, physical_model.cpp:587-590
+        int rc = mcac_box_grow(box, dr, g_vol.data(), g_surf.data(), g_rsum.data(), g_rg.data(), g_rmax.data());
+        if (rc) return rc;
+        const std::vector<int> &ids = live.ids();
+        constexpr size_t kPer = 512;
+        host_parallel_for((ids.size() + kPer - 1) / kPer, [&](size_t k) {
+            const size_t e = std::min(ids.size(), (k + 1) * kPer);
+            for (size_t l = k * kPer; l < e; ++l) {
+                const size_t id = static_cast<size_t>(ids[l]);
+                vol[id] = g_vol[id];
+                surf[id] = g_surf[id];
+                rg[id] = g_rg[id];
+                rmax[id] = g_rmax[id];
+                // method none: the list-order sum of the spheres' volumes is the volume
+                const AggregatePhysics ph =
+                    aggregate_physics(pm, static_cast<size_t>(nsph[id]), g_vol[id], g_vol[id], g_rsum[id], g_rg[id]);
+                dp[id] = ph.dp;
+                f_agg[id] = ph.f_agg;
+                d_m[id] = ph.d_m;
+                ts[id] = ph.time_step;
+                lpm[id] = ph.lpm;
+                dg_over_dp[id] = ph.dg_over_dp;
+            }
+        });
+        for (int id : ids)
+            if (rmax[static_cast<size_t>(id)] > maxradius) maxradius = rmax[static_cast<size_t>(id)];
+        pm_stale = true;
+        t_grow += ms_since(t0);
+        return 0;
+    }
+
     // Outputs at the end of a run (calcul.cpp:282-284), once.
     int finalize() {
         if (!finished() || final_written) return 0;
@@ -699,6 +758,10 @@ struct Simulation {
         deltatemps = deltatemps * (move / full + static_cast<double>(1 - 1));
         ptime[U] += deltatemps;
         pm.time = pm.time + deltatemps;
+        if (pm.with_surface_reactions) {
+            rc = grow(deltatemps);
+            if (rc) return rc;
+        }
         bool merged = false;
         if (contact_flag) {
             auto tdr0 = Clock::now();
@@ -738,7 +801,8 @@ struct Simulation {
                 if (rmax[K] > maxradius) maxradius = rmax[K];  // aggregat.cpp:277-281
                 live.erase(R);
                 ptime[K] = newtime;
-                pick.replace(static_cast<int>(K), keep_old_ts, ts[K], static_cast<int>(R), rem_ts);
+                if (!pm.with_surface_reactions)  // with growth the table is rebuilt below
+                    pick.replace(static_cast<int>(K), keep_old_ts, ts[K], static_cast<int>(R), rem_ts);
                 ++merges;
                 t_merge_host += ms_since(th0);
                 if (trace_merges) {
@@ -774,11 +838,24 @@ struct Simulation {
             pm.n_iter_without_event = 0;
             total_events++;
             // refresh (aggregat_list.cpp:100-108): the largest time step heads the pick table
-            max_time_step = pick.factor();
+            if (pm.with_surface_reactions) {
+                const std::vector<int> &ids = live.ids();
+                max_time_step = ts[static_cast<size_t>(ids[0])];
+                for (int id : ids) max_time_step = smax(ts[static_cast<size_t>(id)], max_time_step);
+            } else {
+                max_time_step = pick.factor();
+            }
             avg_npp = static_cast<double>(N) / static_cast<double>(live.size());
             pm_stale = true;
         } else {
             pm.n_iter_without_event++;
+        }
+        if (pm.with_surface_reactions) {
+            // sort_time_steps(max) at the top of every step (calcul.cpp:99-101),
+            // with max refreshed only at events
+            auto tp0 = Clock::now();
+            pick.build(live.ids(), ts, max_time_step);
+            t_pick += ms_since(tp0);
         }
         if (trace_steps) {
             mcac_step_rec r{};
@@ -819,6 +896,9 @@ struct Simulation {
             // steps, which already ran on the device, complete first
         }
         *done = n;
+        // the resident engine grid ends with the call, so that the device is
+        // idle for the caller (it is launched again by the next step)
+        if (!group && !own_group && engine_blocks > 0) return mcac_box_set_engine(box, engine_blocks);
         return 0;
     }
 };
@@ -900,6 +980,10 @@ int mcac_sim_create(const char *ini_text, const char *output_dir, int device, mc
     }
     if (const char *e = std::getenv("MCAC_ROUNDS")) {  // 0: always one launch per step
         h->s.rounds = std::strtol(e, nullptr, 10) != 0;
+    }
+    if (const char *e = std::getenv("MCAC_ENGINE")) {  // 0: one launch per step; n: resident grid of n blocks
+        const long v = std::strtol(e, nullptr, 10);
+        h->s.engine_blocks = static_cast<int>(v < 0 ? 0 : (v > 255 ? 255 : v));
     }
     *out = h;
     return 0;

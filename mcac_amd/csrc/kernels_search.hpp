@@ -756,8 +756,10 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
 // gate is read with the first loads and a step behind it does no more than
 // read-only filter work (the host knows it is skipped: no record).  Returns (in
 // thread 0) whether the step closed the gate for the steps behind it.
+// role (optional, LDS, set by thread 0): 0 the step was skipped behind the
+// gate, 1 this block took part, 2 this block finished the step.
 __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, unsigned vb, unsigned vg,
-                                          bool gate_open, const CellTable *ct, bool publish) {
+                                          bool gate_open, const CellTable *ct, bool publish, int *role = nullptr) {
     // independent loads first, so that their latencies overlap: the gate, the
     // mover's cell entry and sphere indices, the bounds of the visited cells
     const unsigned long long gate = (!gate_open && a.gated) ? b.ss->gate : 0ull;
@@ -773,6 +775,7 @@ __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, 
         mover.rz = b.srz[msp];
     }
     const unsigned long long cnt = block_sum(filter_phase(b, a, vb, vg, cb, skip));
+    if (role && threadIdx.x == 0) *role = skip ? 0 : 1;
     if (skip) return false;
     // one atomic per block: finished blocks, appended candidates, visited aggregates
     __shared__ bool last;
@@ -788,7 +791,9 @@ __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, 
         __syncthreads();
         if (!last) return false;
         __threadfence();
+        if (role && threadIdx.x == 0) *role = 2;
     } else {
+        if (role && threadIdx.x == 0) *role = 2;
         if (threadIdx.x == 0) s_total = cnt;
         __syncthreads();  // the block's appended candidates are visible to all its threads
     }
@@ -959,6 +964,140 @@ __global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, cons
         __threadfence_system();
         for (int q = 0; q < ran; ++q)
             __hip_atomic_store(&args[q].out->seq, args[q].seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// Persistent form of k_step for one box: a resident grid takes the queued
+// steps from a ring in pinned host memory, in order, instead of one launch
+// per step.  Block 0 alone polls the ring; it copies job k into device memory
+// (the box's eng_jobs slot) and publishes its decision for k (run, exit, or
+// quit after idle_ticks of clock without a job) in eng_decision; the other
+// blocks wait for that decision.  A step starts only when the finishing block
+// of the previous step that ran has set eng_done (its moves, dirty list, gate
+// and reset counters are then visible); a step behind the closed gate is
+// skipped by every block alike.  Every wait is bounded (a wait that runs out
+// ends the grid and reports reason -1), so the grid always ends.
+constexpr unsigned long long kEngineWaitFactor = 8;  // a block's wait for the others: 8 idle periods
+
+__device__ __forceinline__ void copy_words(unsigned long long *dst, const unsigned long long *src, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJob *ring, EngineCtl *ctl,
+                                                        unsigned long long first, unsigned long long idle_ticks,
+                                                        unsigned long long launch_no) {
+    __shared__ EngineJob job;
+    __shared__ int s_cmd, s_role;
+    constexpr int kJobWords = static_cast<int>(offsetof(EngineJob, seq) / sizeof(unsigned long long));
+    static_assert(offsetof(EngineJob, seq) % sizeof(unsigned long long) == 0, "job body in 8-byte words");
+    SearchState *s = b.ss;
+    unsigned long long need_done = 0;  // eng_done must reach this before the next step (0: nothing ran yet)
+    unsigned long long k = first;
+    int reason = kEngineExit;
+    for (;; ++k) {
+        const int slot = static_cast<int>(k % kEngineRing);
+        if (threadIdx.x == 0) {
+            const unsigned long long t0 = wall_clock64();
+            int cmd = -1;
+            if (blockIdx.x == 0) {
+                cmd = kEngineQuit;
+                while (wall_clock64() - t0 <= idle_ticks) {
+                    if (__hip_atomic_load(&ring[slot].seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == k) {
+                        cmd = ring[slot].cmd == kEngineExit ? kEngineExit : kEngineRun;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            } else {
+                const unsigned long long limit = kEngineWaitFactor * idle_ticks;
+                while (wall_clock64() - t0 <= limit) {
+                    const unsigned long long d =
+                        __hip_atomic_load(&s->eng_decision, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((d >> 2) > k) {  // block 0 has gone on: job k ran
+                        cmd = kEngineRun;
+                        break;
+                    }
+                    if ((d >> 2) == k && (d & 3ull) != 0ull) {
+                        cmd = static_cast<int>(d & 3ull);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            s_cmd = cmd;
+        }
+        __syncthreads();
+        int cmd = s_cmd;
+        if (blockIdx.x == 0) {
+            if (cmd == kEngineRun) {
+                copy_words(reinterpret_cast<unsigned long long *>(&job),
+                           reinterpret_cast<const unsigned long long *>(&ring[slot]), kJobWords);
+                __syncthreads();
+                copy_words(reinterpret_cast<unsigned long long *>(&b.eng_jobs[slot]),
+                           reinterpret_cast<const unsigned long long *>(&job), kJobWords);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                if (cmd == kEngineRun) b.eng_t[slot] = wall_clock64();
+                __threadfence();  // the job copy and its clock before the decision
+                __hip_atomic_store(&s->eng_decision, k * 4ull + static_cast<unsigned long long>(cmd),
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (cmd == kEngineRun) {
+            copy_words(reinterpret_cast<unsigned long long *>(&job),
+                       reinterpret_cast<const unsigned long long *>(&b.eng_jobs[slot]), kJobWords);
+            __syncthreads();
+        }
+        if (cmd != kEngineRun) {
+            reason = cmd;
+            break;
+        }
+        // the previous step that ran is complete
+        if (threadIdx.x == 0) {
+            int ok = 1;
+            if (need_done) {
+                const unsigned long long t0 = wall_clock64(), limit = kEngineWaitFactor * idle_ticks;
+                ok = 0;
+                while (wall_clock64() - t0 <= limit) {
+                    if (__hip_atomic_load(&s->eng_done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= need_done) {
+                        ok = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            s_cmd = ok;
+            s_role = 0;
+        }
+        __syncthreads();
+        if (!s_cmd) {
+            reason = -1;
+            break;
+        }
+        const CellTable *ct = job.cells.n > 0 ? &job.cells : nullptr;
+        step_body(b, job.a, blockIdx.x, gridDim.x, false, ct, true, &s_role);
+        __syncthreads();
+        const int role = s_role;
+        if (role >= 1) need_done = k;
+        if (role == 2 && threadIdx.x == 0) {
+            // the record is published; the step's writes and counter resets are
+            // made visible to the other blocks before eng_done
+            const unsigned long long now = wall_clock64(), t_job = b.eng_t[slot], last = s->eng_last;
+            s->eng_busy += now - (t_job > last ? t_job : last);
+            s->eng_last = now;
+            s->eng_steps += 1;
+            __threadfence();
+            __hip_atomic_store(&s->eng_done, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->stopped_at = k;
+        ctl->reason = reason;
+        __threadfence_system();
+        __hip_atomic_store(&ctl->launches_ended, launch_no, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else if (reason == -1 && threadIdx.x == 0) {
+        ctl->reason = -1;
+        __threadfence_system();
     }
 }
 

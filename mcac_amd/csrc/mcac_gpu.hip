@@ -25,6 +25,7 @@
 
 #include "../../include/mcac_gpu.h"
 #include "kernels_cells.hpp"
+#include "kernels_growth.hpp"
 #include "kernels_merge.hpp"
 #include "kernels_search.hpp"
 
@@ -100,6 +101,23 @@ struct mcac_box {
     double t_search = 0, t_translate = 0, t_merge = 0, t_filter = 0, t_solve = 0;
     double n_search = 0, n_translate = 0, n_merge = 0;
     int64_t n_rebuilds = 0;
+    // persistent step engine (k_step_engine): steps posted to a ring in pinned
+    // memory instead of one launch each, while eng_running
+    int eng_blocks = 0;               // grid of the engine (0: one launch per step)
+    int eng_grid = 0;                 // grid of the resident engine
+    bool eng_running = false;         // an engine grid was launched and no exit job posted since
+    unsigned long long eng_next = 1;  // next job index
+    EngineJob *h_ring = nullptr, *dp_ring = nullptr;
+    EngineCtl *h_ctl = nullptr, *dp_ctl = nullptr;
+    unsigned long long eng_idle_ticks = 0;  // device clock ticks an idle grid waits before it ends
+    double eng_clock_khz = 0;
+    int64_t eng_launches = 0, eng_posted = 0, eng_relaunches = 0;
+    // surface growth (mcac_box_grow): per-aggregate results, device and pinned
+    // host copies (5 arrays of the live-list length), allocated at the first call
+    double *d_grow = nullptr, *h_grow = nullptr;
+    int64_t grow_cap = 0;
+    bool cells_stale = false;    // positions changed everywhere: rebuild the cell list before the next step
+    bool mirrors_exact = true;   // the host mirrors of positions and rmax hold every aggregate exactly
 };
 
 // Boxes that share one stream; their queued steps go into k_step_batch
@@ -141,6 +159,8 @@ template <class T> int dalloc(mcac_box *b, T **ptr, int64_t n) {
     return 0;
 }
 
+int engine_start(mcac_box *b, int grid);
+
 // Waits until a kernel has published sequence number `seq` into pinned memory.
 int wait_seq(mcac_box *b, const unsigned long long *seqp, unsigned long long seq) {
     auto t0 = std::chrono::steady_clock::now();
@@ -148,6 +168,12 @@ int wait_seq(mcac_box *b, const unsigned long long *seqp, unsigned long long seq
     while (__atomic_load_n(seqp, __ATOMIC_ACQUIRE) != seq) {
         if ((++spins & 1023u) == 0) {
             hipError_t e = hipStreamQuery(b->stream);
+            if (e == hipSuccess && b->eng_running) {
+                // an engine grid ended idle before its last job was posted: launch it again
+                int rc = engine_start(b, b->eng_grid);
+                if (rc) return rc;
+                continue;
+            }
             if (e == hipSuccess) {
                 if (__atomic_load_n(seqp, __ATOMIC_ACQUIRE) == seq) break;
                 std::fprintf(stderr, "mcac_gpu: stream idle without a published record\n");
@@ -167,6 +193,85 @@ int wait_seq(mcac_box *b, const unsigned long long *seqp, unsigned long long seq
     return 0;
 }
 
+// Persistent step engine.  The host posts a job (one queued step) into the
+// ring with its index written last; engine_stop posts the exit job, after
+// which the grid ends, so that work enqueued later on the box stream runs
+// after the engine's steps.  Nothing here waits for the device.
+void engine_post(mcac_box *b, const SearchArgs *a, const CellTable *ct) {
+    const unsigned long long k = b->eng_next++;
+    // the slot's previous job, k - kEngineRing, was taken long ago: at most
+    // kRing steps are in flight and a grid takes jobs in order
+    EngineJob &j = b->h_ring[k % kEngineRing];
+    if (a) {
+        j.a = *a;
+        j.cells.n = 0;
+        if (ct && ct->n > 0) {
+            j.cells.n = ct->n;
+            std::memcpy(j.cells.c0, ct->c0, static_cast<size_t>(ct->n) * sizeof(int));
+            std::memcpy(j.cells.sz, ct->sz, static_cast<size_t>(ct->n) * sizeof(int));
+        }
+        j.cmd = kEngineRun;
+        b->eng_posted += 1;
+    } else {
+        j.cmd = kEngineExit;
+    }
+    __atomic_store_n(&j.seq, k, __ATOMIC_RELEASE);
+}
+
+void engine_stop(mcac_box *b) {
+    if (!b->eng_running) return;
+    engine_post(b, nullptr, nullptr);
+    b->eng_running = false;
+}
+
+// The box stream for work other than engine steps: a resident engine is told
+// to end first.
+hipStream_t bstream(mcac_box *b) {
+    engine_stop(b);
+    return b->stream;
+}
+
+// A resident grid that ended on its own (no job for its idle period) while the
+// host still counts it as running: its first job not taken.
+bool engine_ended(const mcac_box *b, unsigned long long *first) {
+    if (!b->eng_running) return false;
+    const unsigned long long ended = __atomic_load_n(&b->h_ctl->launches_ended, __ATOMIC_ACQUIRE);
+    if (ended < static_cast<unsigned long long>(b->eng_launches)) return false;
+    *first = b->h_ctl->stopped_at;
+    return true;
+}
+
+// Makes sure an engine grid of `grid` blocks is resident, launching one (from
+// the first job not taken) when none is or when it is smaller.
+int engine_start(mcac_box *b, int grid) {
+    unsigned long long first = b->eng_next;
+    if (b->eng_running) {
+        unsigned long long f = 0;
+        if (engine_ended(b, &f)) {
+            if (b->h_ctl->reason < 0) {
+                std::fprintf(stderr, "mcac_gpu: engine grid ended on a wait that ran out\n");
+                return MCAC_E_DEVICE;
+            }
+            first = f;  // jobs from there on were posted but not taken
+            b->eng_running = false;
+            b->eng_relaunches += 1;
+        } else if (grid <= b->eng_grid) {
+            return 0;
+        } else {
+            engine_stop(b);  // a larger grid takes over at the next job
+            first = b->eng_next;
+        }
+    }
+    HIP_TRY(hipMemsetAsync(&b->d.ss->eng_decision, 0, sizeof(unsigned long long), b->stream));
+    b->eng_launches += 1;
+    hipLaunchKernelGGL(k_step_engine, dim3(static_cast<unsigned>(grid)), dim3(kBlock), 0, b->stream, b->d, b->dp_ring,
+                       b->dp_ctl, first, b->eng_idle_ticks, static_cast<unsigned long long>(b->eng_launches));
+    HIP_TRY(hipGetLastError());
+    b->eng_running = true;
+    b->eng_grid = grid;
+    return 0;
+}
+
 int rebuild_alive(mcac_box *b) {
     std::vector<int> ids;
     ids.reserve(b->alive_ids.size());
@@ -176,8 +281,8 @@ int rebuild_alive(mcac_box *b) {
     b->dead_in_list = 0;
     if (!b->alive_ids.empty())
         HIP_TRY(hipMemcpyAsync(b->d.alive, b->alive_ids.data(), b->alive_ids.size() * sizeof(int),
-                               hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));
+                               hipMemcpyHostToDevice, bstream(b)));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     return 0;
 }
 
@@ -190,22 +295,23 @@ int rebuild_cells(mcac_box *b) {
     const int n_alive = static_cast<int>(b->alive_ids.size());
     const unsigned g = static_cast<unsigned>(std::min<int64_t>(2048, std::max<int64_t>(1, (n_alive + kBlock - 1) / kBlock)));
     const unsigned gc = static_cast<unsigned>(std::min<int64_t>(2048, (b->n_cells + kBlock) / kBlock));
-    hipLaunchKernelGGL(k_cell_clear, dim3(gc), dim3(kBlock), 0, b->stream, b->d, b->n_cells);
-    hipLaunchKernelGGL(k_cell_count, dim3(g), dim3(kBlock), 0, b->stream, b->d, n_alive);
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, b->stream, b->d, b->n_cells);
-    hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(kBlock), 0, b->stream, b->d, n_alive);
+    hipLaunchKernelGGL(k_cell_clear, dim3(gc), dim3(kBlock), 0, bstream(b), b->d, b->n_cells);
+    hipLaunchKernelGGL(k_cell_count, dim3(g), dim3(kBlock), 0, bstream(b), b->d, n_alive);
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, bstream(b), b->d, b->n_cells);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(kBlock), 0, bstream(b), b->d, n_alive);
     HIP_TRY(hipGetLastError());
     b->dirty_count = 0;
     std::fill(b->h_dirty.begin(), b->h_dirty.end(), 0);
     b->dirty_epoch += 1;
     b->n_built = n_alive;
     b->n_rebuilds++;
+    b->cells_stale = false;
     b->cstart_ok = false;
     if (b->h_cstart) {
         // the host passes the visited cells' bounds with each step
         HIP_TRY(hipMemcpyAsync(b->h_cstart, b->d.cstart, (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
-                               hipMemcpyDeviceToHost, b->stream));
-        HIP_TRY(hipStreamSynchronize(b->stream));
+                               hipMemcpyDeviceToHost, bstream(b)));
+        HIP_TRY(hipStreamSynchronize(bstream(b)));
         b->cstart_ok = true;
     }
     return 0;
@@ -242,9 +348,9 @@ int compact_pool(mcac_box *b, int64_t need) {
     HIP_TRY(hipMemcpy(d_os, b->start.data(), b->start.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!ids.empty())
         hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(std::min<size_t>(ids.size(), 65535))), dim3(kBlock), 0,
-                           b->stream, b->d.members, nm, d_os, d_ns, b->d.acount, d_ids, static_cast<int>(ids.size()));
+                           bstream(b), b->d.members, nm, d_os, d_ns, b->d.acount, d_ids, static_cast<int>(ids.size()));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     for (size_t a = 0; a < ids.size(); ++a) {
         const size_t id = static_cast<size_t>(ids[a]);
         b->start[id] = nstart[a];
@@ -274,14 +380,14 @@ int harvest_timing(mcac_box *b, int slot);
 // k_solve of a pending step (at most one: it closes the gate) is launched now.
 int settle_handovers(mcac_box *b) {
     if (b->inflight == 0) return 0;
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     for (unsigned long long t = b->oldest; t <= b->seq; ++t) {
         const int slot = static_cast<int>(t % mcac_box::kRing);
         if (b->slot_void[slot] || b->slot_solving[slot]) continue;
         StepRecord &r = b->h_step[slot];
         if (__atomic_load_n(&r.seq, __ATOMIC_ACQUIRE) != t || r.skipped || r.status != kPendingSolve) continue;
         __atomic_store_n(&r.seq, 0ull, __ATOMIC_RELEASE);  // the k_solve publishes the final record
-        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, b->slot_args[slot]);
+        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, bstream(b), b->d, b->slot_args[slot]);
         HIP_TRY(hipGetLastError());
         b->slot_solving[slot] = 1;
         b->n_handover += 1;
@@ -326,7 +432,7 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     if (agg < 0 || agg >= b->n_aggs || b->count[static_cast<size_t>(agg)] <= 0) return MCAC_E_ARGUMENT;
     if (b->inflight >= mcac_box::kRing) return MCAC_E_CAPACITY;
     if (b->inline_pending) return MCAC_E_ARGUMENT;  // the caller merges the contact first
-    if (rebuild_due(b)) {
+    if (rebuild_due(b) || b->cells_stale) {
         int rc = settle_handovers(b);
         if (rc == 0) rc = rebuild_cells(b);
         if (rc) return rc;
@@ -341,7 +447,7 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     a.start = b->start[A];
     // a queued step behind a known closed gate does nothing: no predictions
     const bool doomed = gated && b->gate_closed;
-    if (b->pend[A] == 0) {
+    if (b->pend[A] == 0 && b->mirrors_exact) {
         a.known = 1;
         a.sx = b->h_x[A];
         a.sy = b->h_y[A];
@@ -421,13 +527,18 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     // timed launches carry start/stop events that the runtime stamps with the
     // kernel's own execution interval; the queue is drained first so that the
     // interval does not include waiting behind the steps queued before it
-    if (timed && b->inflight > 0) HIP_TRY(hipStreamSynchronize(b->stream));
-    if (job_out) {  // the caller checked one_block_step and window_items before the enqueue
-        // a batched step's arguments sit in LDS: the host adds the bounds of
-        // the visited cells, saving the kernel a dependent load (a launch of
-        // its own reads them faster from cstart than from its arguments)
+    if (timed && b->inflight > 0) HIP_TRY(hipStreamSynchronize(bstream(b)));
+    // the persistent engine takes one-launch steps whose mover the finishing
+    // block moves (not timed: a timed step is its own launch between events)
+    const bool engine = !job_out && b->eng_blocks > 0 && !b->group && fused && !timed &&
+                        (!apply || a.translate_in_finish);
+    if (job_out || engine) {
+        // queued step arguments sit in LDS: the host adds the bounds of the
+        // visited cells, saving the kernel a dependent load (a launch of its
+        // own reads them faster from cstart than from its arguments)
         const int V = a.len[0] * a.len[1] * a.len[2];
-        CellTable &ct = job_out->cells;
+        CellTable ct_local;
+        CellTable &ct = job_out ? job_out->cells : ct_local;
         ct.n = 0;
         if (a.known && b->cstart_ok && V <= 64) {
             const int n = static_cast<int>(b->p.n_div), lj = a.len[1], lk = a.len[2];
@@ -440,22 +551,30 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
             }
             ct.n = V;
         }
-        job_out->b = b->d;
-        job_out->a = a;
-        *batched = true;
+        if (job_out) {
+            job_out->b = b->d;
+            job_out->a = a;
+            *batched = true;
+        } else {
+            // the grid grows to the largest step seen since it was launched
+            const int eg = std::min(b->eng_blocks, std::max(16, (grid + 15) / 16 * 16));
+            int rc = engine_start(b, eg);
+            if (rc) return rc;
+            engine_post(b, &a, &ct);
+        }
     } else if (fused) {
-        if (timed) hipExtLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, b->stream, ev[0], ev[1], 0, b->d, a);
-        else hipLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, b->stream, b->d, a);
+        if (timed) hipExtLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, bstream(b), ev[0], ev[1], 0, b->d, a);
+        else hipLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, bstream(b), b->d, a);
     } else if (timed) {
-        hipExtLaunchKernelGGL(k_filter, dim3(grid), dim3(kBlock), 0, b->stream, ev[0], ev[1], 0, b->d, a);
-        hipExtLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, ev[2], ev[3], 0, b->d, a);
+        hipExtLaunchKernelGGL(k_filter, dim3(grid), dim3(kBlock), 0, bstream(b), ev[0], ev[1], 0, b->d, a);
+        hipExtLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, bstream(b), ev[2], ev[3], 0, b->d, a);
     } else {
-        hipLaunchKernelGGL(k_filter, dim3(grid), dim3(kBlock), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_filter, dim3(grid), dim3(kBlock), 0, bstream(b), b->d, a);
+        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, bstream(b), b->d, a);
     }
     if (apply && !a.translate_in_finish) {
         const int tb = std::min(1024, (n_mov + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, a.src, gated, seq, a.released);
+        hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, bstream(b), b->d, a.src, gated, seq, a.released);
     }
     b->slot_args[slot] = a;
     b->slot_fused[slot] = fused ? 1 : 0;
@@ -529,7 +648,7 @@ int wait_search(mcac_box *b, unsigned long long ticket, int64_t agg, mcac_contac
         // closed keeps the queued steps behind it from running meanwhile
         __atomic_store_n(&b->h_step[slot].seq, 0ull, __ATOMIC_RELEASE);
         SearchArgs sa = b->slot_args[slot];
-        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, b->stream, b->d, sa);
+        hipLaunchKernelGGL(k_solve, dim3(kSolveBlocks), dim3(kBlock), 0, bstream(b), b->d, sa);
         HIP_TRY(hipGetLastError());
         b->n_handover += 1;
         rc = wait_seq(b, &b->h_step[slot].seq, ticket);
@@ -644,6 +763,7 @@ template <class F> int box_buffers(DevBox &d, const BoxSizes &z, int volsurf, F 
         ALLOC(d.walk_n, Ns);
     }
     ALLOC(d.ss, 1); ALLOC(d.ms, 1);
+    ALLOC(d.eng_jobs, kEngineRing); ALLOC(d.eng_t, kEngineRing);
 #undef ALLOC
     return 0;
 }
@@ -733,6 +853,22 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (hipHostGetDevicePointer(reinterpret_cast<void **>(&b->d_step), b->h_step, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void **>(&b->d_merge), b->h_merge, 0) != hipSuccess)
         return fail(MCAC_E_DEVICE);
+    if (hipHostMalloc(reinterpret_cast<void **>(&b->h_ring), kEngineRing * sizeof(EngineJob), hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&b->h_ctl), sizeof(EngineCtl), hipHostMallocCoherent) != hipSuccess)
+        return fail(MCAC_E_DEVICE);
+    std::memset(b->h_ring, 0, kEngineRing * sizeof(EngineJob));
+    std::memset(b->h_ctl, 0, sizeof(EngineCtl));
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&b->dp_ring), b->h_ring, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&b->dp_ctl), b->h_ctl, 0) != hipSuccess)
+        return fail(MCAC_E_DEVICE);
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, params->device) != hipSuccess || khz <= 0)
+            khz = 100000;  // the MI300-series constant clock, 100 MHz
+        b->eng_clock_khz = khz;
+        b->eng_idle_ticks = static_cast<unsigned long long>(khz) * 500ull;  // an idle grid ends after 0.5 s
+    }
     for (auto &e : b->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(MCAC_E_DEVICE);
     for (auto &row : b->sev)
@@ -744,12 +880,16 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
 
 int mcac_box_destroy(mcac_box *b) {
     if (!b) return 0;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(bstream(b));  // a resident engine ends first
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->h_step) (void)hipHostFree(b->h_step);
     if (b->h_merge) (void)hipHostFree(b->h_merge);
     if (b->h_alpha) (void)hipHostFree(b->h_alpha);
     if (b->h_cstart) (void)hipHostFree(b->h_cstart);
+    if (b->h_ring) (void)hipHostFree(b->h_ring);
+    if (b->h_grow) (void)hipHostFree(b->h_grow);
+    if (b->d_grow) (void)hipFree(b->d_grow);
+    if (b->h_ctl) (void)hipHostFree(b->h_ctl);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &row : b->sev)
@@ -875,7 +1015,7 @@ int mcac_box_upload(mcac_box *b, int64_t ns, const double *x, const double *y, c
     b->inline_pending = false;
     rc = rebuild_cells(b);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     return 0;
 }
 
@@ -897,7 +1037,7 @@ int mcac_translate(mcac_box *b, int64_t agg, const double vec[3]) {
         int rc = rebuild_cells(b);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_set_move, dim3(1), dim3(1), 0, b->stream, b->d, static_cast<int>(agg), vec[0], vec[1], vec[2],
+    hipLaunchKernelGGL(k_set_move, dim3(1), dim3(1), 0, bstream(b), b->d, static_cast<int>(agg), vec[0], vec[1], vec[2],
                        static_cast<int>(b->p.n_div), b->p.box_length);
     {
         const size_t A = static_cast<size_t>(agg);
@@ -912,9 +1052,9 @@ int mcac_translate(mcac_box *b, int64_t agg, const double vec[3]) {
     }
     const int n = b->count[static_cast<size_t>(agg)];
     const int tb = std::min(1024, std::max(1, (n + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, b->stream, b->d, static_cast<int>(agg), 0, 0ull, 0ull);
+    hipLaunchKernelGGL(k_translate, dim3(tb), dim3(kBlock), 0, bstream(b), b->d, static_cast<int>(agg), 0, 0ull, 0ull);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     return 0;
 }
 
@@ -1066,24 +1206,24 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
     a.seq = ++b->merge_seq;
     a.volsurf = b->p.volsurf;
     const int n = nk + nr;
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    if (b->timing) HIP_TRY(hipEventRecord(b->ev[0], bstream(b)));
     const bool fused = b->fused_merge && a.volsurf != 0 && n <= kFusedMerge;
     if (fused && a.volsurf != 2) {
-        hipLaunchKernelGGL(k_merge_fused, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_merge_fused, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
     } else if (a.volsurf == 1 || a.volsurf == 3) {
         // spherical caps or arvo: the two-way contact rows first, then the
         // per-sphere parts, the full ordered sums, dc, rmax and Rg (publishes)
         const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
         const unsigned tiles = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
-        hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
-        if (a.volsurf == 1) hipLaunchKernelGGL(k_caps_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        else hipLaunchKernelGGL(k_arvo_volumes, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, bstream(b), b->d, a);
+        hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, bstream(b), b->d, 1);
+        hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
+        if (a.volsurf == 1) hipLaunchKernelGGL(k_caps_volumes, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+        else hipLaunchKernelGGL(k_arvo_volumes, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+        hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
+        hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+        hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
     } else if (a.volsurf == 2) {
         // alphas: contact rows, the neighbour-map walks and the list-order sums
         // on the device; the correction factors on the host (glibc pow, as
@@ -1102,22 +1242,22 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
             b->h_alpha_cap = cap;
         }
         if (fused) {
-            hipLaunchKernelGGL(k_merge_fused, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+            hipLaunchKernelGGL(k_merge_fused, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
         } else {
-            hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
-            hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-            hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 1);
-            hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
-            hipLaunchKernelGGL(k_alpha_walk, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-            hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+            hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, bstream(b), b->d, a);
+            hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+            hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, bstream(b), b->d, 1);
+            hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
+            hipLaunchKernelGGL(k_alpha_walk, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+            hipLaunchKernelGGL(k_caps_sums, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
         }
         HIP_TRY(hipGetLastError());
         // the state and the walk rows in one wait (the rows are unused when the re-check failed)
-        HIP_TRY(hipMemcpyAsync(b->h_alpha, b->d.ms, sizeof(MergeState), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->h_alpha, b->d.ms, sizeof(MergeState), hipMemcpyDeviceToHost, bstream(b)));
         HIP_TRY(hipMemcpyAsync(b->h_alpha + off_n, b->d.walk_n, static_cast<size_t>(n) * sizeof(int),
-                               hipMemcpyDeviceToHost, b->stream));
-        HIP_TRY(hipMemcpyAsync(b->h_alpha + off_w, b->d.walk, walk_bytes, hipMemcpyDeviceToHost, b->stream));
-        HIP_TRY(hipStreamSynchronize(b->stream));
+                               hipMemcpyDeviceToHost, bstream(b)));
+        HIP_TRY(hipMemcpyAsync(b->h_alpha + off_w, b->d.walk, walk_bytes, hipMemcpyDeviceToHost, bstream(b)));
+        HIP_TRY(hipStreamSynchronize(bstream(b)));
         MergeState ms{};
         std::memcpy(&ms, b->h_alpha, sizeof(MergeState));
         if (ms.ok) {
@@ -1125,29 +1265,29 @@ int mcac_merge(mcac_box *b, const mcac_contact *c, mcac_merge_result *out) {
                                                  reinterpret_cast<const double *>(b->h_alpha + off_w), ms.sums[0],
                                                  ms.sums[1]);
             if (fused) {
-                hipLaunchKernelGGL(k_alpha_finish, dim3(1), dim3(kBlock), 0, b->stream, b->d, a, f.volume,
+                hipLaunchKernelGGL(k_alpha_finish, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a, f.volume,
                                    f.surface, f.overlapping, f.coordination, f.any ? 1 : 0);
             } else {
-                hipLaunchKernelGGL(k_alpha_store, dim3(1), dim3(64), 0, b->stream, b->d, a, f.volume, f.surface,
+                hipLaunchKernelGGL(k_alpha_store, dim3(1), dim3(64), 0, bstream(b), b->d, a, f.volume, f.surface,
                                    f.overlapping, f.coordination, f.any ? 1 : 0);
-                hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-                hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+                hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+                hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
             }
         }
     } else if (n <= kSmallMerge) {
-        hipLaunchKernelGGL(k_merge_small, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_merge_small, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
     } else {
         const int nb = std::min(4096, (n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        hipLaunchKernelGGL(k_merge_seq, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
-        hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, b->stream, b->d);
-        hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_merge_prep, dim3(1), dim3(64), 0, bstream(b), b->d, a);
+        hipLaunchKernelGGL(k_merge_lists, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+        hipLaunchKernelGGL(k_merge_seq, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
+        hipLaunchKernelGGL(k_merge_dc, dim3(nb), dim3(kBlock), 0, bstream(b), b->d);
+        hipLaunchKernelGGL(k_merge_rg, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
         const unsigned tiles = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, b->stream, b->d, 0);
-        hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, b->stream, b->d, a);
+        hipLaunchKernelGGL(k_overlap_pairs, dim3(tiles, tiles), dim3(kBlock), 0, bstream(b), b->d, 0);
+        hipLaunchKernelGGL(k_overlap_sum, dim3(1), dim3(kBlock), 0, bstream(b), b->d, a);
     }
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    if (b->timing) HIP_TRY(hipEventRecord(b->ev[1], bstream(b)));
     HIP_TRY(hipGetLastError());
     int rc = wait_seq(b, &b->h_merge->seq, a.seq);
     if (rc) return rc;
@@ -1210,7 +1350,8 @@ int mcac_group_destroy(mcac_group *g) {
 
 int mcac_group_add(mcac_group *g, mcac_box *b) {
     if (!g || !b || b->group || b->inflight || b->p.device != g->device) return MCAC_E_ARGUMENT;
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->eng_blocks = 0;  // the group's launches take the box's steps
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     if (b->own_stream) HIP_TRY(hipStreamDestroy(b->stream));
     b->stream = g->stream;
     b->own_stream = false;
@@ -1367,7 +1508,7 @@ int mcac_neighbour_order(int prev, const int64_t *keys, int m, int64_t *out) {
 int mcac_box_download_spheres(mcac_box *b, double *x, double *y, double *z, double *rx, double *ry, double *rz) {
     if (!b) return MCAC_E_ARGUMENT;
     const size_t n = static_cast<size_t>(b->n_spheres) * sizeof(double);
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     if (x) HIP_TRY(hipMemcpy(x, b->d.sx, n, hipMemcpyDeviceToHost));
     if (y) HIP_TRY(hipMemcpy(y, b->d.sy, n, hipMemcpyDeviceToHost));
     if (z) HIP_TRY(hipMemcpy(z, b->d.sz, n, hipMemcpyDeviceToHost));
@@ -1377,11 +1518,21 @@ int mcac_box_download_spheres(mcac_box *b, double *x, double *y, double *z, doub
     return 0;
 }
 
+int mcac_box_download_radii(mcac_box *b, double *r, double *vol, double *surf) {
+    if (!b) return MCAC_E_ARGUMENT;
+    const size_t n = static_cast<size_t>(b->n_spheres) * sizeof(double);
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
+    if (r) HIP_TRY(hipMemcpy(r, b->d.sr, n, hipMemcpyDeviceToHost));
+    if (vol) HIP_TRY(hipMemcpy(vol, b->d.svol, n, hipMemcpyDeviceToHost));
+    if (surf) HIP_TRY(hipMemcpy(surf, b->d.ssurf, n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int mcac_box_download_aggs(mcac_box *b, double *pos3, double *rcm3, double *rmax, double *volume, double *surface,
                            double *rg, int64_t *count) {
     if (!b) return MCAC_E_ARGUMENT;
     const size_t A = static_cast<size_t>(b->n_aggs);
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     std::vector<double> t0(A), t1(A), t2(A);
     if (pos3) {
         HIP_TRY(hipMemcpy(t0.data(), b->d.ax, A * sizeof(double), hipMemcpyDeviceToHost));
@@ -1419,9 +1570,9 @@ int mcac_box_members(mcac_box *b, int64_t agg, int64_t *out, int64_t max, int64_
     if (!out || cnt == 0) return 0;
     int *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, static_cast<size_t>(cnt) * sizeof(int)));
-    hipLaunchKernelGGL(k_gather_members, dim3(std::min(1024, (cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream,
+    hipLaunchKernelGGL(k_gather_members, dim3(std::min(1024, (cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0, bstream(b),
                        b->d, static_cast<int>(agg), tmp, cnt);
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     std::vector<int> h(static_cast<size_t>(cnt));
     HIP_TRY(hipMemcpy(h.data(), tmp, h.size() * sizeof(int), hipMemcpyDeviceToHost));
     HIP_TRY(hipFree(tmp));
@@ -1442,14 +1593,76 @@ int mcac_box_sphere_owners(mcac_box *b, int64_t *owner) {
     HIP_TRY(hipMemcpy(d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!ids.empty())
         hipLaunchKernelGGL(k_sphere_owner, dim3(static_cast<unsigned>(std::min<size_t>(ids.size(), 65535))),
-                           dim3(kBlock), 0, b->stream, b->d, d_ids, static_cast<int>(ids.size()), d_own);
+                           dim3(kBlock), 0, bstream(b), b->d, d_ids, static_cast<int>(ids.size()), d_own);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     std::vector<int> h(S);
     if (S) HIP_TRY(hipMemcpy(h.data(), d_own, S * sizeof(int), hipMemcpyDeviceToHost));
     HIP_TRY(hipFree(d_ids));
     HIP_TRY(hipFree(d_own));
     for (size_t i = 0; i < S; ++i) owner[i] = h[i];
+    return 0;
+}
+
+int mcac_box_grow(mcac_box *b, double dr, double *volume, double *surface, double *radius_sum, double *rg,
+                  double *rmax) {
+    if (!b || !volume || !surface || !radius_sum || !rg || !rmax || b->inflight || b->inline_pending ||
+        !(dr >= 0.0))
+        return MCAC_E_ARGUMENT;
+    const int64_t n = static_cast<int64_t>(b->alive_ids.size());
+    if (n == 0) return 0;
+    if (b->grow_cap < n) {
+        if (b->h_grow) HIP_TRY(hipHostFree(b->h_grow));
+        if (b->d_grow) HIP_TRY(hipFree(b->d_grow));
+        b->h_grow = nullptr;
+        b->d_grow = nullptr;
+        b->grow_cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&b->d_grow), 5 * static_cast<size_t>(n) * sizeof(double)));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&b->h_grow), 5 * static_cast<size_t>(n) * sizeof(double),
+                              hipHostMallocDefault));
+        b->grow_cap = n;
+    }
+    const size_t N = static_cast<size_t>(n);
+    double *g = b->d_grow;
+    hipStream_t st = bstream(b);
+    const unsigned blocks = static_cast<unsigned>((n * 64 + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_grow_update, dim3(blocks), dim3(kBlock), 0, st, b->d, b->d.alive, static_cast<int>(n), dr,
+                       b->p.box_length, static_cast<int>(b->p.n_div), g, g + N, g + 2 * N, g + 3 * N, g + 4 * N);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(b->h_grow, g, 5 * N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t w = 0; w < N; ++w) {
+        const int id = b->alive_ids[w];
+        if (b->count[static_cast<size_t>(id)] <= 0) continue;
+        volume[id] = b->h_grow[w];
+        surface[id] = b->h_grow[N + w];
+        radius_sum[id] = b->h_grow[2 * N + w];
+        rg[id] = b->h_grow[3 * N + w];
+        rmax[id] = b->h_grow[4 * N + w];
+    }
+    b->cells_stale = true;
+    b->mirrors_exact = false;
+    return 0;
+}
+
+int mcac_box_set_engine(mcac_box *b, int blocks) {
+    if (!b || blocks < 0 || blocks > 255 || (blocks > 0 && b->group)) return MCAC_E_ARGUMENT;
+    engine_stop(b);
+    b->eng_blocks = blocks;
+    return 0;
+}
+
+int mcac_box_engine_stats(mcac_box *b, double out[6]) {
+    if (!b || !out) return MCAC_E_ARGUMENT;
+    SearchState s{};
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
+    HIP_TRY(hipMemcpy(&s, b->d.ss, sizeof(SearchState), hipMemcpyDeviceToHost));
+    out[0] = static_cast<double>(b->eng_launches);
+    out[1] = static_cast<double>(b->eng_posted);
+    out[2] = static_cast<double>(s.eng_steps);
+    out[3] = static_cast<double>(s.eng_busy) / b->eng_clock_khz;  // ms
+    out[4] = static_cast<double>(b->eng_relaunches);
+    out[5] = static_cast<double>(b->h_ctl->reason);
     return 0;
 }
 
@@ -1479,7 +1692,7 @@ int mcac_box_stats(mcac_box *b, int64_t out[8]) {
 int mcac_box_prune_stats(mcac_box *b, int64_t out[3]) {
     if (!b || !out) return MCAC_E_ARGUMENT;
     SearchState s{};
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
     HIP_TRY(hipMemcpy(&s, b->d.ss, sizeof(SearchState), hipMemcpyDeviceToHost));
     out[0] = static_cast<int64_t>(s.fp32_tested);
     out[1] = static_cast<int64_t>(s.fp32_kept);

@@ -18,8 +18,15 @@ from . import _check, _dp, _ip, load_sim
 
 
 def box_ini(n: int, fv: float, temperature: float, seed: int, mean_d: float = 30.0, sd_d: float = 3.0,
-            mode: str = "normal", stop_fraction: float = 0.1) -> str:
-    """One parameter file; the run stops at N_agg <= stop_fraction * N."""
+            mode: str = "normal", stop_fraction: float = 0.1, growth_flux: float = 0.0) -> str:
+    """One parameter file; the run stops at N_agg <= stop_fraction * N.  With
+    growth_flux > 0 the spheres grow by surface reactions at that flux
+    (kg/m^2/s; [surface_growth] flux_surfgrowth, u_sg = flux / density)."""
+    growth = (f"""
+[surface_growth]
+with_surface_reactions = true
+flux_surfgrowth = {growth_flux!r}
+""" if growth_flux > 0 else "")
     return f"""[monomers]
 number = {n}
 density = 1800
@@ -41,7 +48,7 @@ number_of_aggregates = {max(1, int(n * stop_fraction))}
 random_seed = {seed}
 with_domain_duplication = false
 n_verlet_divisions = 10
-"""
+{growth}"""
 
 
 def ensemble_configs(n_boxes: int = 512, s0: int = 1000, rng_seed: int = 4) -> list[dict]:
@@ -57,17 +64,23 @@ def ensemble_configs(n_boxes: int = 512, s0: int = 1000, rng_seed: int = 4) -> l
 
 
 def serving_requests(n_requests: int, rate_per_s: float, s0: int = 5000, rng_seed: int = 5,
-                     n_max: float = 2e5) -> list[dict]:
-    """Serving form: Poisson arrivals at rate_per_s (first arrival at 0)."""
+                     n_max: float = 2e5, growth_flux: float = 1e-3, growth_n_max: float = 2e5) -> list[dict]:
+    """Serving form: Poisson arrivals at rate_per_s (first arrival at 0); every
+    other request (the odd ones) with surface growth at growth_flux, its N
+    drawn up to growth_n_max (growth_flux 0: none with growth)."""
     rng = np.random.default_rng(rng_seed)
     t = 0.0
     out = []
     for r in range(n_requests):
-        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(n_max)))))
+        grow = growth_flux > 0 and r % 2 == 1
+        u = rng.uniform(0.0, 1.0)
+        hi = min(n_max, growth_n_max) if grow else n_max
+        n = int(round(np.exp(np.log(1e3) + u * (np.log(hi) - np.log(1e3)))))
         fv = float(rng.uniform(1e-4, 5e-3))
         temp = float(rng.uniform(1200, 2000))
-        out.append(dict(n=n, fv=fv, temperature=temp, seed=s0 + r, arrival=t,
-                        ini=box_ini(n, fv, temp, s0 + r, mode="lognormal", sd_d=1.25)))
+        out.append(dict(n=n, fv=fv, temperature=temp, seed=s0 + r, arrival=t, growth=grow,
+                        ini=box_ini(n, fv, temp, s0 + r, mode="lognormal", sd_d=1.25,
+                                    growth_flux=growth_flux if grow else 0.0)))
         t += float(rng.exponential(1.0 / rate_per_s))
     return out
 

@@ -288,6 +288,10 @@ struct Physics {
     bool with_domain_duplication = true, with_domain_reduction = false, with_nucleation = false;
     bool with_collisions = true, with_surface_reactions = false, with_flame_coupling = false;
     bool enforce_volume_fraction = true, with_potentials = false;
+    // surface reactions (physical_model.cpp:59, 79, 91, 127-159, 264)
+    size_t full_aggregate_update_frequency = 1;
+    bool individual_surf_reactions = false;
+    double rp_min_oxid = 0.166e-09;
     std::string unsupported;
 
     void load(const Ini &ini) {
@@ -308,6 +312,9 @@ struct Physics {
         extract(ini, "environment", "fractal_dimension", fractal_dimension);
         extract(ini, "surface_growth", "with_surface_reactions", with_surface_reactions);
         extract(ini, "surface_growth", "flux_surfgrowth", flux_surfgrowth);
+        extract(ini, "surface_growth", "full_aggregate_update_frequency", full_aggregate_update_frequency);
+        extract(ini, "oxidation", "rp_min", rp_min_oxid);
+        extract(ini, "numerics", "individual_surf_reactions", individual_surf_reactions);
         s.clear();
         extract(ini, "surface_growth", "volsurf_method", s);
         if (s == "caps") volsurf = 1;
@@ -330,7 +337,14 @@ struct Physics {
         extract(ini, "numerics", "random_seed", random_seed);
         extract(ini, "inter_potential", "with_potentials", with_potentials);
         extract(ini, "flame_coupling", "with_flame_coupling", with_flame_coupling);
-        if (with_surface_reactions) unsupported = "with_surface_reactions";
+        // surface growth (flux >= 0) with the whole-box update every step and no
+        // overlap correction; oxidation (flux < 0: sphere removal and split),
+        // per-aggregate reactions and sparser full updates are not restated
+        if (with_surface_reactions && flux_surfgrowth < 0) unsupported = "flux_surfgrowth<0 (oxidation)";
+        if (with_surface_reactions && individual_surf_reactions) unsupported = "individual_surf_reactions";
+        if (with_surface_reactions && full_aggregate_update_frequency != 1)
+            unsupported = "full_aggregate_update_frequency!=1";
+        if (with_surface_reactions && volsurf != 0) unsupported = "volsurf_method with surface reactions";
         if (with_nucleation) unsupported = "with_nucleation";
         if (with_potentials) unsupported = "with_potentials";
         if (with_flame_coupling) unsupported = "with_flame_coupling";
@@ -1297,11 +1311,30 @@ struct Sim {
         return false;
     }
 
-    // calcul.cpp:66-281 (one iteration, no surface reactions / nucleation / potentials)
+    // AggregatList::croissance_surface(dt) (aggregat_list.cpp:549-566) with
+    // u_sg >= 0: every sphere of every aggregate grows (Aggregate::
+    // croissance_surface, aggregat.cpp:230-243, Sphere::croissance_surface,
+    // sphere.cpp:113-120, PhysicalModel::grow, physical_model.cpp:587-590); no
+    // radius falls to rp_min_oxid, so no sphere or aggregate is removed
+    void grow_all(double dt) {
+        for (const auto &a : aggs)
+            for (size_t sp : a.sph) {
+                const double new_r = sr[sp] + pm.u_sg * dt;
+                const double new_r_2 = new_r * new_r;
+                const double new_r_3 = new_r_2 * new_r;
+                sr[sp] = new_r;
+                svol[sp] = kVolumeFactor * new_r_3;
+                ssurf[sp] = kSurfaceFactor * new_r_2;
+            }
+    }
+
+    // calcul.cpp:66-281 (one iteration; surface growth as calcul.cpp:155-205
+    // with individual_surf_reactions off and full_aggregate_update_frequency 1;
+    // no nucleation or potentials)
     int step() {
         if (event && pm.with_domain_duplication && aggs.size() <= pm.n_monomeres / 8) return 1;
         double max = max_time_step;
-        if (event) sort_time_steps(max);
+        if (event || pm.with_surface_reactions) sort_time_steps(max);
         size_t num = pick_random();
         double deltatemps = max / cumulative[aggs.size() - 1];
         double full = aggs[num].lpm;
@@ -1319,8 +1352,14 @@ struct Sim {
         deltatemps = deltatemps * (move / full + static_cast<double>(1 - 1));
         aggs[num].proper_time += deltatemps;
         pm.time = pm.time + deltatemps;
+        if (pm.with_surface_reactions) grow_all(deltatemps);  // calcul.cpp:164-172
         bool merged = false;
         if (contact_flag) merged = merge(nc);
+        if (pm.with_surface_reactions) {
+            // calcul.cpp:183-205: n_iter_without_event % 1 == 0, so every
+            // aggregate gets the full update (the merged one a second time)
+            for (size_t l = 0; l < aggs.size(); l++) update(l);
+        }
         event = merged;
         if (event) {
             pm.n_iter_without_event = 0;
@@ -1329,7 +1368,8 @@ struct Sim {
             pm.n_iter_without_event++;
         }
         if (event) refresh();
-        if (event) pm.update(aggs.size(), sx.size(), get_total_volume(), get_total_surface());
+        if (event || pm.with_surface_reactions)
+            pm.update(aggs.size(), sx.size(), get_total_volume(), get_total_surface());
         if (trace_steps) {
             ora_step_rec r{};
             r.step = step_count;

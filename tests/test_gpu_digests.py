@@ -63,8 +63,7 @@ def test_c3_precision_form_prefix_digest(precision, monkeypatch):
     got, sim = device_digest(ref)
     diff = first_difference(ref, got)
     assert diff == "", diff
-    ps = sim.box().prune_stats()
-    say("prune stats", ps)
-    assert ps["tested"] > 0 and ps["kept"] < ps["tested"]
-    if precision == "2":
-        assert ps["missed"] == 0
+    if precision == "2":  # precision 2 counts the FP32 test's work and its misses
+        ps = sim.box().prune_stats()
+        say("prune stats", ps)
+        assert ps["tested"] > 0 and ps["kept"] < ps["tested"] and ps["missed"] == 0
