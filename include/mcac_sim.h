@@ -53,7 +53,7 @@ typedef struct mcac_merge_rec {
  * MCAC_FUSED_MERGE (default 1; 0 off, read by mcac_box_create): merges with
  * an overlap correction of at most 512 spheres run in one workgroup launch
  * instead of the method's kernel sequence (results are identical).
- * MCAC_ENGINE (default 128; 0 off): queued one-launch steps go to a resident
+ * MCAC_ENGINE (default 0, off; n >= 1 on): queued one-launch steps go to a resident
  * grid of up to that many workgroups (mcac_box_set_engine) instead of one
  * launch each; the grid ends when mcac_sim_run returns (results are
  * identical). */

@@ -118,7 +118,7 @@ struct Simulation {
     mcac_group *group = nullptr;  // lock-step form: the box works on the group's stream
     mcac_group *own_group = nullptr;  // one-box group for rounds of steps in one launch
     bool rounds = false;              // issue rounds through own_group when the steps fit one workgroup
-    int engine_blocks = 128;          // persistent step engine grid (mcac_box_set_engine), 0: off
+    int engine_blocks = 0;            // persistent step engine grid (mcac_box_set_engine), 0: off
     std::vector<int64_t> r_aggs;      // scratch of issue_round
     std::vector<double> r_dirs, r_lpm, r_maxr;
     std::vector<uint64_t> r_tickets;
