@@ -114,7 +114,6 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
     __shared__ double rlx[kSmallMerge], rly[kSmallMerge], rlz[kSmallMerge];
     __shared__ int nrow[kSmallMerge];
     __shared__ double crow[kSmallMerge * 8];
-    __shared__ unsigned int row_over;
     __shared__ double s_V, s_svr2;
     __shared__ mcac_merge_result s_res;
     const int n = a.n_k + a.n_r;
@@ -157,7 +156,6 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         dsk = b.dslot[k], dsr = b.dslot[a.rem];
         dcount = b.ss->dirty_count;
         s0 = b.members[a.start_k];  // the keeper's first sphere
-        row_over = 0;
     }
     lds_barrier();
     // batch 2: the sphere rows (relative positions; absolute ones are not needed)
@@ -328,8 +326,7 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
             if (q > 1.0001 * (s2 * s2)) continue;
             double dist = __builtin_sqrt(q);
             if (dist <= (1. + kCoordinationEpsilon) * s2) {
-                if (cnt < 8) crow[i * 8 + cnt] = (s2 - dist) / s2;
-                else atomicOr(&row_over, 1u);
+                if (cnt < 8) crow[i * 8 + cnt] = (s2 - dist) / s2;  // a longer row is summed again in order
                 ++cnt;
             }
         }
@@ -349,11 +346,28 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         b.arg[k] = rg;
         double overlapping = 0.0;
         long long inter = 0;
-        for (int i = 0; i < n; ++i)
-            for (int c = 0; c < nrow[i] && c < 8; ++c) {
-                overlapping += 2.0 * crow[i * 8 + c];  // aggregat.cpp:755
-                inter += 2;
+        for (int i = 0; i < n; ++i) {
+            if (nrow[i] <= 8) {
+                for (int c = 0; c < nrow[i]; ++c) {
+                    overlapping += 2.0 * crow[i * 8 + c];  // aggregat.cpp:755
+                    inter += 2;
+                }
+                continue;
             }
+            // more contacts than the row holds (spheres grown into many
+            // neighbours): the row again, in the same order and arithmetic
+            for (int j = i + 1; j < n; ++j) {
+                double d0 = rlx[i] - rlx[j], d1 = rly[i] - rly[j], d2 = rlz[i] - rlz[j];
+                double s2 = rads[i] + rads[j];
+                double q = d0 * d0 + d1 * d1 + d2 * d2;
+                if (q > 1.0001 * (s2 * s2)) continue;
+                double dist = __builtin_sqrt(q);
+                if (dist <= (1. + kCoordinationEpsilon) * s2) {
+                    overlapping += 2.0 * ((s2 - dist) / s2);
+                    inter += 2;
+                }
+            }
+        }
         double coord = 0.0;
         if (inter > 0) {
             overlapping /= static_cast<double>(inter);
@@ -364,7 +378,6 @@ __device__ __forceinline__ void merge_small_block(const DevBox &b, const MergeAr
         res.rg = rg;
         res.overlapping = overlapping;
         res.coordination = coord;
-        if (row_over) res.status = MCAC_E_CAPACITY;
         res_out = res;
     }
 }
@@ -602,6 +615,28 @@ __global__ __launch_bounds__(kBlock) void k_overlap_pairs(DevBox b, int both) {
     overlap_pairs_tile(b, blockIdx.y, blockIdx.x, both);
 }
 
+// Row i of the merged aggregate's contacts toward higher j, summed into the
+// overlap sum in order (aggregat.cpp:736-759), read from global memory: for
+// a row with more contacts than its slots (single thread).
+__device__ __forceinline__ void overlapping_row_again(const DevBox &b, const MergeState *m, int i, int n,
+                                                      double &overlapping, long long &inter) {
+    const int st = m->new_start;
+    const int si = b.members[st + i];
+    const double ix = b.srx[si], iy = b.sry[si], iz = b.srz[si], ir = b.sr[si];
+    for (int j = i + 1; j < n; ++j) {
+        const int sj = b.members[st + j];
+        double d0 = ix - b.srx[sj], d1 = iy - b.sry[sj], d2 = iz - b.srz[sj];
+        double s2 = ir + b.sr[sj];
+        double q = d0 * d0 + d1 * d1 + d2 * d2;
+        if (q > 1.0001 * (s2 * s2)) continue;
+        double dist = __builtin_sqrt(q);
+        if (dist <= (1. + kCoordinationEpsilon) * s2) {
+            overlapping += 2.0 * ((s2 - dist) / s2);
+            inter += 2;
+        }
+    }
+}
+
 // Ordered overlap sum in (i, j) order; clears the row counters; publishes.
 // Each thread sorts its row by j in registers and stages its coefficients
 // toward higher j in LDS, from which thread 0 adds them in order.  With an
@@ -620,8 +655,8 @@ __device__ __forceinline__ void overlap_sum_block(const DevBox &b, const MergeAr
         const int cnt = (n - t0 < kBlock) ? n - t0 : kBlock;
         const int i = t0 + threadIdx.x;
         if (threadIdx.x < cnt) {
-            int rc = b.row_cnt[i];
-            if (rc > kOverlapSlots) rc = kOverlapSlots;
+            const int rc_all = b.row_cnt[i];
+            const int rc = rc_all > kOverlapSlots ? kOverlapSlots : rc_all;
             const long long base = static_cast<long long>(i) * kOverlapSlots;
             // the row in registers (unused slots sort last), a bitonic network by j
             int kj[kOverlapSlots];
@@ -664,11 +699,17 @@ __device__ __forceinline__ void overlap_sum_block(const DevBox &b, const MergeAr
                 if (x < rc && kj[x] > i) tc[threadIdx.x * kOverlapSlots + hi++] = kc[x];  // j > i only
             }
             if (!a.volsurf) b.row_cnt[i] = 0;
-            tcnt[threadIdx.x] = hi;
+            // a row longer than its slots (method none only: the corrections
+            // need whole rows and report the overflow) is summed again below
+            tcnt[threadIdx.x] = (!a.volsurf && rc_all > kOverlapSlots) ? -1 : hi;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int x = 0; x < cnt; ++x) {
+                if (tcnt[x] < 0) {
+                    overlapping_row_again(b, m, t0 + x, n, overlapping, inter);
+                    continue;
+                }
                 for (int y = 0; y < tcnt[x]; ++y) {
                     overlapping += 2.0 * tc[x * kOverlapSlots + y];
                     inter += 2;
@@ -685,7 +726,7 @@ __device__ __forceinline__ void overlap_sum_block(const DevBox &b, const MergeAr
     }
     m->res.overlapping = overlapping;
     m->res.coordination = coord;
-    if (m->row_overflow) m->res.status = MCAC_E_CAPACITY;
+    if (m->row_overflow && a.volsurf) m->res.status = MCAC_E_CAPACITY;  // method none sums long rows again
     if (!a.volsurf) publish_merge(a, m->res);
 }
 
