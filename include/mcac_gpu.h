@@ -216,6 +216,11 @@ int mcac_box_download_aggs(mcac_box *box, double *pos3, double *rcm3, double *rm
  * sphere no live aggregate owns): the spheres' aggregate labels written by
  * SphereList::save (src/io/sphere_list.cpp:44-46) once mapped to labels. */
 int mcac_box_sphere_owners(mcac_box *box, int64_t *owner);
+/* Member lists of every aggregate: count[id] for each id below the box's
+ * aggregate count (0 for a dead id), and members the concatenated lists in
+ * id order, each in list order (SphereList order of Aggregate::myspheres).
+ * members must hold the box's live sphere count. */
+int mcac_box_download_members(mcac_box *box, int64_t *count, int64_t *members);
 /* Members of one aggregate in list order; returns the count via *n. */
 int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int64_t *n);
 
@@ -250,8 +255,10 @@ int mcac_box_set_engine(mcac_box *box, int blocks);
 /* [grids launched, steps posted, steps completed by the engine, device ms of
  * those steps (each from when it could start: its job taken and the previous
  * step done, to its record), grids launched again after an idle end, the last
- * grid's end reason (2 idle, 3 ended by the host, -1 a wait ran out)]. */
-int mcac_box_engine_stats(mcac_box *box, double out[6]);
+ * grid's end reason (2 idle, 3 ended by the host, -1 a wait ran out), ms the
+ * grid waited for the host's next job between steps, workgroups of the last
+ * grid]. */
+int mcac_box_engine_stats(mcac_box *box, double out[8]);
 
 /* Kernel timing on the box's stream.  enable = n >= 1 times the first search
  * enqueued after the call and every n-th one after it, with start/stop events stamped by the runtime at the kernel's own

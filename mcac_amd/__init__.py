@@ -58,7 +58,7 @@ GPU_SYMBOLS = (
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
     "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
     "mcac_box_set_engine", "mcac_box_engine_stats", "mcac_box_grow",
-    "mcac_box_download_radii",
+    "mcac_box_download_radii", "mcac_box_download_members",
 )
 SIM_SYMBOLS = (
     "mcac_sim_create", "mcac_sim_destroy", "mcac_sim_init", "mcac_sim_run", "mcac_sim_finished",
@@ -122,6 +122,7 @@ def load_gpu():
         L.mcac_box_set_engine.argtypes = [_P, _I]
         L.mcac_box_grow.argtypes = [_P, _D] + [_DP] * 5
         L.mcac_box_download_radii.argtypes = [_P] + [_DP] * 3
+        L.mcac_box_download_members.argtypes = [_P, _I64P, _I64P]
         L.mcac_box_engine_stats.argtypes = [_P, _DP]
         L.mcac_box_timing.argtypes = [_P, _DP]
         L.mcac_box_stats.argtypes = [_P, _I64P]
@@ -233,8 +234,8 @@ def swept_sphere_batch(p1, r1, p2, r2, direction, dist, box: float, device: int 
     r2 = np.ascontiguousarray(np.broadcast_to(r2, (n,)), np.float64)
     dist = np.ascontiguousarray(np.broadcast_to(dist, (n,)), np.float64)
     out = np.zeros(n)
-    _check(load_gpu().mcac_swept_sphere_batch(n, _dp(p1), _dp(r1), _dp(p2), _dp(r2), _dp(d), _dp(dist), box,
-                                              _dp(out), device), "mcac_swept_sphere_batch")
+    f = load_gpu().mcac_swept_sphere_batch
+    _check(f(n, _dp(p1), _dp(r1), _dp(p2), _dp(r2), _dp(d), _dp(dist), box, _dp(out), device), f.__name__)
     return out
 
 
@@ -367,9 +368,9 @@ class Box:
         _check(load_gpu().mcac_box_set_engine(self.h, int(blocks)), "mcac_box_set_engine")
 
     def engine_stats(self):
-        out = np.zeros(6)
+        out = np.zeros(8)
         _check(load_gpu().mcac_box_engine_stats(self.h, _dp(out)), "mcac_box_engine_stats")
-        keys = ("launches", "posted", "steps", "busy_ms", "relaunches", "end_reason")
+        keys = ("launches", "posted", "steps", "busy_ms", "relaunches", "end_reason", "idle_ms", "grid")
         return dict(zip(keys, out.tolist()))
 
     def set_auto_merge(self, enable: bool):

@@ -42,6 +42,7 @@ struct SearchState {
     unsigned long long eng_last;      // clock of the last completed step
     unsigned long long eng_busy;      // clock ticks of completed steps (from when each could start)
     unsigned long long eng_steps;     // steps completed by the engine
+    unsigned long long eng_idle;      // clock ticks between a step's end and the next job's arrival
 };
 
 // Record published to pinned host memory by the last block of a search.
@@ -162,7 +163,7 @@ struct EngineJob {
     SearchArgs a;
     CellTable cells;           // bounds of the visited cells (n = 0: the kernel reads cstart)
     int cmd;                   // kEngineRun, or kEngineExit: the grid ends at this job
-    int pad;
+    int vg;                    // workgroups that take part in the step (the first vg of the grid)
     unsigned long long seq;    // job index, written last by the host (release)
 };
 // Written by the engine into pinned host memory when its grid ends.

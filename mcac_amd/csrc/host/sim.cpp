@@ -124,7 +124,10 @@ struct Simulation {
     std::vector<uint64_t> r_tickets;
     std::vector<mcac_box *> r_boxes;
     // spheres (index order) — host copy of the initial state only
-    size_t N = 0;
+    size_t N = 0;              // spheres in the box
+    size_t n_ids = 0;          // stable ids handed out (per-id host arrays hold this many)
+    size_t dup_threshold = 0;  // calcul.cpp:58: duplication when an event leaves at most this many aggregates
+    int n_duplications = 0;
     // aggregates by stable id
     std::vector<double> ts, lpm, f_agg, d_m, dp, rg, rmax, vol, surf, ptime, dg_over_dp, overlapping, coord;
     std::vector<Vec3> com, rcm;
@@ -288,6 +291,146 @@ struct Simulation {
         return 0;
     }
 
+    // The device box for the current side, spheres and ids; it joins the
+    // group when the run is one of a group's.
+    int create_box() {
+        mcac_box_params bp{};
+        bp.box_length = pm.box_length;
+        bp.n_div = static_cast<int64_t>(pm.n_verlet_divisions);
+        bp.max_spheres = static_cast<int64_t>(N);
+        bp.max_aggregates = static_cast<int64_t>(n_ids);
+        bp.device = device;
+        bp.precision = precision;
+        bp.volsurf = pm.volsurf;
+        int rc = mcac_box_create(&bp, &box);
+        if (rc) return rc;
+        if (group) rc = mcac_group_add(group, box);
+        return rc;
+    }
+    // Launch modes of an uploaded box: merges in the contact step's launch
+    // (calcul.cpp:156-160 always merges on contact) unless a growth step must
+    // come first, and the persistent engine when asked for.
+    int box_modes() {
+        int rc = mcac_box_set_auto_merge(box, pm.with_surface_reactions ? 0 : 1);
+        if (rc) return rc;
+        if (pm.with_surface_reactions)
+            for (auto *v : {&g_vol, &g_surf, &g_rsum, &g_rg, &g_rmax}) v->assign(n_ids, 0.0);
+        if (!group && !own_group && engine_blocks > 0) rc = mcac_box_set_engine(box, engine_blocks);
+        return rc;
+    }
+
+    // AggregatList::duplication (aggregat_list.cpp:142-190), at the top of the
+    // loop after an event left at most N0/8 aggregates (calcul.cpp:72-81): the
+    // box side doubles and every live aggregate, in label order, gets seven
+    // copies, (i, j, k) in {0,1}^3 minus the origin in loop order, each a copy
+    // of all its fields and spheres (aggregat_storage.cpp:122-164) translated by
+    // (i, j, k) times the old side (Aggregate::translate, aggregat.cpp:148-161,
+    // wrapping into the new box).  Copies take the next stable ids and their
+    // spheres the next sphere indices, in that order.  The device box is
+    // rebuilt at the new size from the state read back.
+    bool duplication_due() const {
+        return event && pm.with_domain_duplication && live.size() <= dup_threshold && !(pm.u_sg < 0.0) &&
+               flight.empty();
+    }
+    int duplicate() {
+        // the copies of the overlap corrections' neighbour maps would need the
+        // maps' own copy history replayed: duplication runs with method none
+        if (pm.volsurf != 0) {
+            std::fprintf(stderr, "mcac_sim: domain duplication with volsurf_method other than none is not offered\n");
+            return MCAC_INPUT_ERROR;
+        }
+        const size_t S = N, A = n_ids;
+        std::vector<double> x(S), y(S), z(S), rx(S), ry(S), rz(S), r(S), sv(S), ss(S);
+        int rc = mcac_box_download_spheres(box, x.data(), y.data(), z.data(), rx.data(), ry.data(), rz.data());
+        if (rc == 0) rc = mcac_box_download_radii(box, r.data(), sv.data(), ss.data());
+        std::vector<int64_t> cnt(A), mem(S);
+        if (rc == 0) rc = mcac_box_download_members(box, cnt.data(), mem.data());
+        if (rc) return rc;
+        std::vector<int64_t> off(A + 1, 0);
+        for (size_t a = 0; a < A; ++a) off[a + 1] = off[a] + cnt[a];
+        const std::vector<int> ids(live.ids());
+        const size_t n_live = ids.size();
+        const double old_l = pm.box_length;
+        pm.box_length *= 2;
+        pm.n_monomeres *= 8;
+        pm.box_volume = std::pow(pm.box_length, 3);
+        const double L = pm.box_length;
+        const size_t S2 = 8 * S, A2 = A + 7 * n_live;
+        for (auto *v : {&x, &y, &z, &rx, &ry, &rz, &r, &sv, &ss}) v->reserve(S2);
+        for (auto *v : {&ts, &lpm, &f_agg, &d_m, &dp, &rg, &rmax, &vol, &surf, &ptime, &dg_over_dp, &overlapping, &coord})
+            v->resize(A2, 0.0);
+        com.resize(A2, {0, 0, 0});
+        rcm.resize(A2, {0, 0, 0});
+        nsph.resize(A2, 0);
+        cnt.resize(A2, 0);
+        std::vector<int64_t> mem2(mem.begin(), mem.end());
+        mem2.reserve(S2);
+        size_t next = A;
+        for (int id : ids) {
+            const size_t a = static_cast<size_t>(id);
+            for (int i = 0; i <= 1; ++i)
+                for (int j = 0; j <= 1; ++j)
+                    for (int k = 0; k <= 1; ++k) {
+                        if (i == 0 && j == 0 && k == 0) continue;
+                        const size_t c = next++;
+                        for (auto *v : {&ts, &lpm, &f_agg, &d_m, &dp, &rg, &rmax, &vol, &surf, &ptime, &dg_over_dp,
+                                        &overlapping, &coord})
+                            (*v)[c] = (*v)[a];
+                        rcm[c] = rcm[a];
+                        nsph[c] = nsph[a];
+                        cnt[c] = cnt[a];
+                        const double vec[3] = {static_cast<double>(i) * old_l, static_cast<double>(j) * old_l,
+                                               static_cast<double>(k) * old_l};
+                        Vec3 pos;
+                        for (int l = 0; l < 3; ++l)
+                            pos[static_cast<size_t>(l)] =
+                                periodic_position(com[a][static_cast<size_t>(l)] + vec[l], L);
+                        com[c] = pos;
+                        const Vec3 ref{pos[0] - rcm[c][0], pos[1] - rcm[c][1], pos[2] - rcm[c][2]};
+                        for (int64_t q = off[a]; q < off[a + 1]; ++q) {
+                            const size_t sp = static_cast<size_t>(mem[static_cast<size_t>(q)]);
+                            mem2.push_back(static_cast<int64_t>(x.size()));
+                            x.push_back(ref[0] + rx[sp]);
+                            y.push_back(ref[1] + ry[sp]);
+                            z.push_back(ref[2] + rz[sp]);
+                            rx.push_back(rx[sp]);
+                            ry.push_back(ry[sp]);
+                            rz.push_back(rz[sp]);
+                            r.push_back(r[sp]);
+                            sv.push_back(sv[sp]);
+                            ss.push_back(ss[sp]);
+                        }
+                    }
+        }
+        // the device state at the new size
+        mcac_box_destroy(box);
+        box = nullptr;
+        N = S2;
+        n_ids = A2;
+        rc = create_box();
+        if (rc) return rc;
+        std::vector<double> pos3(3 * A2), rcm3(3 * A2);
+        for (size_t a = 0; a < A2; ++a)
+            for (size_t l = 0; l < 3; ++l) {
+                pos3[3 * a + l] = com[a][l];
+                rcm3[3 * a + l] = rcm[a][l];
+            }
+        rc = mcac_box_upload(box, static_cast<int64_t>(N), x.data(), y.data(), z.data(), r.data(), sv.data(),
+                             ss.data(), rx.data(), ry.data(), rz.data(), static_cast<int64_t>(A2), pos3.data(),
+                             rcm3.data(), rmax.data(), vol.data(), surf.data(), cnt.data(), mem2.data());
+        if (rc) return rc;
+        rc = box_modes();
+        if (rc) return rc;
+        live.reset(A2);
+        for (size_t a = 0; a < A; ++a)
+            if (cnt[a] == 0) live.erase(a);
+        if (!sphere_r.empty()) sphere_r = r;
+        pm_stale = true;
+        n_duplications += 1;
+        pick.build(live.ids(), ts, max_time_step);  // sort_time_steps(max) at the event
+        return 0;
+    }
+
     int init() {
         auto t0 = Clock::now();
         N = pm.n_monomeres;
@@ -321,20 +464,12 @@ struct Simulation {
             }
         }
         live.reset(N);
+        n_ids = N;
+        dup_threshold = N / 8;
         // device state
-        mcac_box_params bp{};
-        bp.box_length = pm.box_length;
-        bp.n_div = static_cast<int64_t>(pm.n_verlet_divisions);
-        bp.max_spheres = static_cast<int64_t>(N);
-        bp.max_aggregates = static_cast<int64_t>(N);
-        bp.device = device;
-        bp.precision = precision;
-        bp.volsurf = pm.volsurf;
-        rc = mcac_box_create(&bp, &box);
+        rc = create_box();
         if (rc) return rc;
         if (group) {
-            rc = mcac_group_add(group, box);
-            if (rc) return rc;
             depth = group_depth;  // steps picked ahead per box and group launch
         } else if (rounds && out_dir.empty() && depth > 1) {
             rc = mcac_group_create(device, 1, &own_group);
@@ -351,22 +486,15 @@ struct Simulation {
                              ssurf.data(), zero.data(), zero.data(), zero.data(), static_cast<int64_t>(N), pos3.data(),
                              rcm3.data(), rmax.data(), vol.data(), surf.data(), cnt.data(), mem.data());
         if (rc) return rc;
-        rc = mcac_box_set_auto_merge(box, 1);  // every contact merges (calcul.cpp:156-160)
-        if (rc) return rc;
         if (pm.with_surface_reactions) {
             // every step grows the whole box and updates every aggregate, so a
             // step depends on the one before it: no look-ahead, and a contact
             // merges after the growth (calcul.cpp:164-181)
             depth = 1;
             engine_blocks = 0;
-            rc = mcac_box_set_auto_merge(box, 0);
-            if (rc) return rc;
-            for (auto *v : {&g_vol, &g_surf, &g_rsum, &g_rg, &g_rmax}) v->assign(N, 0.0);
         }
-        if (!group && !own_group && engine_blocks > 0) {
-            rc = mcac_box_set_engine(box, engine_blocks);
-            if (rc) return rc;
-        }
+        rc = box_modes();
+        if (rc) return rc;
         pick.build(live.ids(), ts);  // first sort_time_steps (calcul.cpp:99-101)
         event = true;
         initialized = true;
@@ -434,8 +562,8 @@ struct Simulation {
                             {"Label", 2, slabel.data(), S}},
                            phys);
         // aggregates in label order (ascending live id)
-        std::vector<double> apos(3 * N), arcm(3 * N), armax(N), avol(N), asurf(N), arg(N);
-        std::vector<int64_t> acount(N);
+        std::vector<double> apos(3 * n_ids), arcm(3 * n_ids), armax(n_ids), avol(n_ids), asurf(n_ids), arg(n_ids);
+        std::vector<int64_t> acount(n_ids);
         rc = mcac_box_download_aggs(box, apos.data(), arcm.data(), armax.data(), avol.data(), asurf.data(), arg.data(),
                                     acount.data());
         if (rc) return rc;
@@ -600,6 +728,10 @@ struct Simulation {
     static constexpr size_t kRoundDepth = 8, kRoundRefill = 3;
 
     int issue_more(int64_t consumed, int64_t max_steps) {
+        if (duplication_due() && !finished()) {
+            int rc = duplicate();
+            if (rc) return rc;
+        }
         if (!own_group) {
             while (!finished() && can_issue(consumed, max_steps)) {
                 int rc = issue();
@@ -739,7 +871,6 @@ struct Simulation {
             if (rc) return rc;
             save_advancement();
         }
-        if (event && pm.with_domain_duplication && live.size() <= pm.n_monomeres / 8) return MCAC_INPUT_ERROR;
         *done = true;
         const int num = P.num;
         const size_t U = static_cast<size_t>(num);
@@ -921,6 +1052,7 @@ int sim_join_group(mcac_sim *sim, mcac_group *group, int depth) {
 bool sim_next_move(mcac_sim *sim, int64_t steps_done, int64_t max_steps, GroupMove *move) {
     Simulation &S = sim->s;
     if (!S.initialized || S.finished() || !S.can_issue(steps_done, max_steps)) return false;
+    if (S.duplication_due() && S.duplicate() != 0) return false;  // the next consume reports nothing: stop
     Simulation::Pending P = S.pick_move();  // the draw after the ones already in flight
     S.flight.push_back(P);
     S.unticketed += 1;

@@ -987,7 +987,7 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
                                                         unsigned long long first, unsigned long long idle_ticks,
                                                         unsigned long long launch_no) {
     __shared__ EngineJob job;
-    __shared__ int s_cmd, s_role;
+    __shared__ int s_cmd, s_role, s_gate;
     constexpr int kJobWords = static_cast<int>(offsetof(EngineJob, seq) / sizeof(unsigned long long));
     static_assert(offsetof(EngineJob, seq) % sizeof(unsigned long long) == 0, "job body in 8-byte words");
     SearchState *s = b.ss;
@@ -1074,8 +1074,21 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
             reason = -1;
             break;
         }
-        const CellTable *ct = job.cells.n > 0 ? &job.cells : nullptr;
-        step_body(b, job.a, blockIdx.x, gridDim.x, false, ct, true, &s_role);
+        // the gate is settled once the previous step is done: a step behind it
+        // is skipped by every workgroup at once (no read-only sweep); otherwise
+        // the step's own share of the grid works on it and the other
+        // workgroups only wait for it
+        const unsigned vg = job.vg < 1 ? 1u : (job.vg > static_cast<int>(gridDim.x) ? gridDim.x : static_cast<unsigned>(job.vg));
+        if (threadIdx.x == 0) s_gate = behind_gate(b, job.a) ? 1 : 0;
+        __syncthreads();
+        if (s_gate) {
+            if (threadIdx.x == 0) s_role = 0;
+        } else if (blockIdx.x < vg) {
+            const CellTable *ct = job.cells.n > 0 ? &job.cells : nullptr;
+            step_body(b, job.a, blockIdx.x, vg, true, ct, true, &s_role);
+        } else if (threadIdx.x == 0) {
+            s_role = 1;
+        }
         __syncthreads();
         const int role = s_role;
         if (role >= 1) need_done = k;
@@ -1084,6 +1097,7 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
             // made visible to the other blocks before eng_done
             const unsigned long long now = wall_clock64(), t_job = b.eng_t[slot], last = s->eng_last;
             s->eng_busy += now - (t_job > last ? t_job : last);
+            if (last && t_job > last) s->eng_idle += t_job - last;  // the grid waited for this job
             s->eng_last = now;
             s->eng_steps += 1;
             __threadfence();

@@ -197,7 +197,7 @@ int wait_seq(mcac_box *b, const unsigned long long *seqp, unsigned long long seq
 // ring with its index written last; engine_stop posts the exit job, after
 // which the grid ends, so that work enqueued later on the box stream runs
 // after the engine's steps.  Nothing here waits for the device.
-void engine_post(mcac_box *b, const SearchArgs *a, const CellTable *ct) {
+void engine_post(mcac_box *b, const SearchArgs *a, const CellTable *ct, int vg) {
     const unsigned long long k = b->eng_next++;
     // the slot's previous job, k - kEngineRing, was taken long ago: at most
     // kRing steps are in flight and a grid takes jobs in order
@@ -211,6 +211,7 @@ void engine_post(mcac_box *b, const SearchArgs *a, const CellTable *ct) {
             std::memcpy(j.cells.sz, ct->sz, static_cast<size_t>(ct->n) * sizeof(int));
         }
         j.cmd = kEngineRun;
+        j.vg = vg;
         b->eng_posted += 1;
     } else {
         j.cmd = kEngineExit;
@@ -220,7 +221,7 @@ void engine_post(mcac_box *b, const SearchArgs *a, const CellTable *ct) {
 
 void engine_stop(mcac_box *b) {
     if (!b->eng_running) return;
-    engine_post(b, nullptr, nullptr);
+    engine_post(b, nullptr, nullptr, 0);
     b->eng_running = false;
 }
 
@@ -556,11 +557,11 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
             job_out->a = a;
             *batched = true;
         } else {
-            // the grid grows to the largest step seen since it was launched
-            const int eg = std::min(b->eng_blocks, std::max(16, (grid + 15) / 16 * 16));
-            int rc = engine_start(b, eg);
+            // the resident grid has eng_blocks workgroups; the step takes the
+            // share its window needs, as a launch of its own would
+            int rc = engine_start(b, b->eng_blocks);
             if (rc) return rc;
-            engine_post(b, &a, &ct);
+            engine_post(b, &a, &ct, std::min(grid, b->eng_blocks));
         }
     } else if (fused) {
         if (timed) hipExtLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, bstream(b), ev[0], ev[1], 0, b->d, a);
@@ -1580,6 +1581,28 @@ int mcac_box_members(mcac_box *b, int64_t agg, int64_t *out, int64_t max, int64_
     return 0;
 }
 
+int mcac_box_download_members(mcac_box *b, int64_t *count, int64_t *members) {
+    if (!b || !count || !members || b->inflight) return MCAC_E_ARGUMENT;
+    const size_t A = static_cast<size_t>(b->n_aggs);
+    std::vector<int> st(A), cnt(A), pool(static_cast<size_t>(std::max<int64_t>(b->pool_used, 1)));
+    HIP_TRY(hipStreamSynchronize(bstream(b)));
+    if (A) {
+        HIP_TRY(hipMemcpy(st.data(), b->d.astart, A * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(cnt.data(), b->d.acount, A * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(hipMemcpy(pool.data(), b->d.members, pool.size() * sizeof(int), hipMemcpyDeviceToHost));
+    int64_t off = 0;
+    for (size_t a = 0; a < A; ++a) {
+        const int c = cnt[a] > 0 ? cnt[a] : 0;
+        count[a] = c;
+        if (c > 0 && (st[a] < 0 || static_cast<size_t>(st[a]) + static_cast<size_t>(c) > pool.size()))
+            return MCAC_E_DEVICE;
+        for (int i = 0; i < c; ++i) members[off + i] = pool[static_cast<size_t>(st[a]) + static_cast<size_t>(i)];
+        off += c;
+    }
+    return 0;
+}
+
 int mcac_box_sphere_owners(mcac_box *b, int64_t *owner) {
     if (!b || !owner || b->inflight) return MCAC_E_ARGUMENT;
     std::vector<int> ids;
@@ -1652,7 +1675,7 @@ int mcac_box_set_engine(mcac_box *b, int blocks) {
     return 0;
 }
 
-int mcac_box_engine_stats(mcac_box *b, double out[6]) {
+int mcac_box_engine_stats(mcac_box *b, double out[8]) {
     if (!b || !out) return MCAC_E_ARGUMENT;
     SearchState s{};
     HIP_TRY(hipStreamSynchronize(bstream(b)));
@@ -1663,6 +1686,8 @@ int mcac_box_engine_stats(mcac_box *b, double out[6]) {
     out[3] = static_cast<double>(s.eng_busy) / b->eng_clock_khz;  // ms
     out[4] = static_cast<double>(b->eng_relaunches);
     out[5] = static_cast<double>(b->h_ctl->reason);
+    out[6] = static_cast<double>(s.eng_idle) / b->eng_clock_khz;  // ms
+    out[7] = static_cast<double>(b->eng_grid);
     return 0;
 }
 

@@ -741,6 +741,8 @@ struct Sim {
     // loop state (calcul.cpp:57-61)
     bool event = true;
     size_t total_events = 0;
+    size_t dup_threshold = 0;  // duplication when an event leaves at most this many aggregates
+    int64_t next_uid = 0;      // stable id of the next aggregate created
     int64_t step_count = 0;
     bool finished_flag = false;
     bool trace_steps = true, trace_merges = true;
@@ -1122,7 +1124,52 @@ struct Sim {
             for (auto &a : aggs) compute_volume_surface(a);
         }
         hg.cells.clear();
+        dup_threshold = aggs.size() / 8;  // calcul.cpp:58, fixed for the run
+        next_uid = static_cast<int64_t>(N);
         return 0;
+    }
+
+    // AggregatList::duplication (aggregat_list.cpp:142-190): the box doubles
+    // per axis and every aggregate, in label order, gets seven copies (all its
+    // fields and spheres, aggregat_storage.cpp:122-164, list_storage_methods.hpp:
+    // 123-131) appended in (i, j, k) order and translated by (i, j, k) times
+    // the old side; the Verlet grid is rebuilt for the new side.  Copies take
+    // the next stable ids, in that order.
+    void duplication() {
+        const size_t old_n_agg = aggs.size();
+        const double old_l = pm.box_length;
+        pm.box_length *= 2;
+        pm.n_monomeres *= 8;
+        pm.box_volume = std::pow(pm.box_length, 3);
+        for (size_t iagg = 0; iagg < old_n_agg; iagg++)
+            for (int i = 0; i <= 1; i++)
+                for (int j = 0; j <= 1; j++)
+                    for (int k = 0; k <= 1; k++) {
+                        if (i == 0 && j == 0 && k == 0) continue;
+                        Agg copy = aggs[iagg];
+                        const size_t label = aggs.size();
+                        copy.uid = next_uid++;
+                        copy.sph.clear();
+                        for (size_t sp : aggs[iagg].sph) {
+                            const size_t ns = sx.size();
+                            sx.push_back(sx[sp]); sy.push_back(sy[sp]); sz.push_back(sz[sp]);
+                            sr.push_back(sr[sp]); svol.push_back(svol[sp]); ssurf.push_back(ssurf[sp]);
+                            srx.push_back(srx[sp]); sry.push_back(sry[sp]); srz.push_back(srz[sp]);
+                            slabel.push_back(static_cast<long>(label));
+                            copy.sph.push_back(ns);
+                        }
+                        aggs.push_back(std::move(copy));
+                        // Aggregate::translate with no Verlet grid attached (unset_verlet)
+                        translate(label, {static_cast<double>(i) * old_l, static_cast<double>(j) * old_l,
+                                          static_cast<double>(k) * old_l}, false);
+                    }
+        width = pm.box_length;  // Verlet(n_verlet_divisions, box_length), set_verlet for every aggregate
+        grid.assign(n_div * n_div * n_div, {});
+        for (size_t l = 0; l < aggs.size(); l++) {
+            aggs[l].vidx = compute_index_verlet(aggs[l]);
+            cell(aggs[l].vidx).insert(l);
+        }
+        pm.update(aggs.size(), sx.size(), get_total_volume(), get_total_surface());
     }
 
     // aggregat_list.cpp:28-36
@@ -1228,19 +1275,21 @@ struct Sim {
         return closest;
     }
     // aggregat.cpp:148-161
-    void translate(size_t label, const V3 &v) {
+    void translate(size_t label, const V3 &v, bool in_verlet = true) {
         Agg &a = aggs[label];
-        set_position(label, a.pos + v);
+        set_position(label, a.pos + v, in_verlet);
         V3 refpos = a.pos - a.rcm;
         set_spos(a.sph[0], refpos);
         for (size_t s : a.sph) set_spos(s, refpos + srel(s));
     }
 
     // ------------------------------------------------------------ merge
+    // aggs stays sorted by uid: erasing a row keeps the others' order
+    // (aggregat_list_storage.cpp:37-44), so a label is found by bisection
     size_t label_of_uid(int64_t uid) const {
-        for (size_t l = 0; l < aggs.size(); l++)
-            if (aggs[l].uid == uid) return l;
-        throw std::runtime_error("uid not found");
+        auto it = std::lower_bound(aggs.begin(), aggs.end(), uid, [](const Agg &a, int64_t u) { return a.uid < u; });
+        if (it == aggs.end() || it->uid != uid) throw std::runtime_error("uid not found");
+        return static_cast<size_t>(it - aggs.begin());
     }
     // aggregat_list.cpp:367-410
     bool merge(const Contact &ci) {
@@ -1332,7 +1381,8 @@ struct Sim {
     // with individual_surf_reactions off and full_aggregate_update_frequency 1;
     // no nucleation or potentials)
     int step() {
-        if (event && pm.with_domain_duplication && aggs.size() <= pm.n_monomeres / 8) return 1;
+        // calcul.cpp:72-81
+        if (event && pm.with_domain_duplication && aggs.size() <= dup_threshold && !(pm.u_sg < 0.0)) duplication();
         double max = max_time_step;
         if (event || pm.with_surface_reactions) sort_time_steps(max);
         size_t num = pick_random();
