@@ -287,6 +287,17 @@ int mcac_box_prune_stats(mcac_box *box, int64_t out[3]);
 int mcac_swept_sphere_batch(int64_t n, const double *p1, const double *r1, const double *p2, const double *r2,
                             const double *dir, const double *dist, double box, double *out, int device);
 
+/* mcac_swept_sphere_batch without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results; the device staging is a static array that each GPU holds until
+ * the process ends, so n is limited to 65,536 inputs (larger n: returns
+ * MCAC_E_CAPACITY and writes nothing).  Calls that overlap on one GPU share
+ * that array and race. */
+int mcac_swept_sphere_batch_noalloc(int64_t n, const double *p1, const double *r1, const double *p2,
+                                    const double *r2, const double *dir, const double *dist, double box, double *out,
+                                    int device);
+
 /* Library self-description for loaders and tests. */
 const char *mcac_version(void);
 

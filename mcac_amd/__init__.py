@@ -57,7 +57,7 @@ GPU_SYMBOLS = (
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
     "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
-    "mcac_box_set_engine", "mcac_box_engine_stats", "mcac_box_grow",
+    "mcac_swept_sphere_batch_noalloc", "mcac_box_set_engine", "mcac_box_engine_stats", "mcac_box_grow",
     "mcac_box_download_radii", "mcac_box_download_members",
 )
 SIM_SYMBOLS = (
@@ -128,6 +128,7 @@ def load_gpu():
         L.mcac_box_stats.argtypes = [_P, _I64P]
         L.mcac_box_prune_stats.argtypes = [_P, _I64P]
         L.mcac_swept_sphere_batch.argtypes = [_I64, _DP, _DP, _DP, _DP, _DP, _DP, _D, _DP, _I]
+        L.mcac_swept_sphere_batch_noalloc.argtypes = [_I64, _DP, _DP, _DP, _DP, _DP, _DP, _D, _DP, _I]
         L.mcac_group_create.argtypes = [_I, _I, ctypes.POINTER(_P)]
         L.mcac_group_destroy.argtypes = [_P]
         L.mcac_group_add.argtypes = [_P, _P]
@@ -224,8 +225,9 @@ def arvo_part(balls) -> tuple:
     return float(out[0]), float(out[1])
 
 
-def swept_sphere_batch(p1, r1, p2, r2, direction, dist, box: float, device: int = 0):
-    """Device evaluation of distance_to_contact(Sphere, Sphere) for n inputs."""
+def swept_sphere_batch(p1, r1, p2, r2, direction, dist, box: float, device: int = 0, noalloc: bool = False):
+    """Device evaluation of distance_to_contact(Sphere, Sphere) for n inputs
+    (noalloc: through mcac_swept_sphere_batch_noalloc, n <= 65,536)."""
     p1 = np.ascontiguousarray(p1, np.float64).reshape(-1, 3)
     n = p1.shape[0]
     p2 = np.ascontiguousarray(p2, np.float64).reshape(n, 3)
@@ -234,7 +236,7 @@ def swept_sphere_batch(p1, r1, p2, r2, direction, dist, box: float, device: int 
     r2 = np.ascontiguousarray(np.broadcast_to(r2, (n,)), np.float64)
     dist = np.ascontiguousarray(np.broadcast_to(dist, (n,)), np.float64)
     out = np.zeros(n)
-    f = load_gpu().mcac_swept_sphere_batch
+    f = load_gpu().mcac_swept_sphere_batch_noalloc if noalloc else load_gpu().mcac_swept_sphere_batch
     _check(f(n, _dp(p1), _dp(r1), _dp(p2), _dp(r2), _dp(d), _dp(dist), box, _dp(out), device), f.__name__)
     return out
 

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -769,6 +770,33 @@ template <class F> int box_buffers(DevBox &d, const BoxSizes &z, int volsurf, F 
     return 0;
 }
 
+// Static staging of mcac_swept_sphere_batch_noalloc (13 doubles per input,
+// 6.5 MiB).  In host code `scratch` names a host-side stand-in, not the
+// device array: its device address comes from hipGetSymbolAddress, once per
+// device with that device current, and only that address is used.
+constexpr int64_t kNoAllocInputs = int64_t(1) << 16;
+static __device__ __align__(256) double scratch[13 * kNoAllocInputs];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on `device` (current), looked up on first use.
+int scratch_address(int device, double **out) {
+    if (device < 0 || device >= kMaxDevices) return MCAC_E_ARGUMENT;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return MCAC_E_DEVICE;
+        g_scratch[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch[device];
+    return 0;
+}
+}  // namespace
+
 extern "C" {
 
 int mcac_swept_sphere_batch(int64_t n, const double *p1, const double *r1, const double *p2, const double *r2,
@@ -800,6 +828,44 @@ int mcac_swept_sphere_batch(int64_t n, const double *p1, const double *r1, const
     }
     (void)hipFree(d);
     return rc;
+}
+
+// mcac_swept_sphere_batch without device allocation: the same staging
+// (inputs, then outputs, 13 doubles per input) lives in `scratch`, a
+// file-scope device array of kNoAllocInputs inputs that each GPU holds until
+// the process ends.  Every element a call reads is written earlier in the
+// same call, so nothing needs clearing; two calls that overlap on one GPU
+// share the array and race.
+int mcac_swept_sphere_batch_noalloc(int64_t n, const double *p1, const double *r1, const double *p2,
+                                    const double *r2, const double *dir, const double *dist, double box, double *out,
+                                    int device) {
+    if (n < 0 || (n > 0 && (!p1 || !r1 || !p2 || !r2 || !dir || !dist || !out)) || box <= 0) return MCAC_E_ARGUMENT;
+    if (n > kNoAllocInputs) {
+        std::fprintf(stderr, "mcac_swept_sphere_batch_noalloc: %lld inputs exceed the static staging of %lld\n",
+                     static_cast<long long>(n), static_cast<long long>(kNoAllocInputs));
+        return MCAC_E_CAPACITY;
+    }
+    if (n == 0) return 0;
+    HIP_TRY(hipSetDevice(device));
+    double *d = nullptr;
+    int rc = scratch_address(device, &d);
+    if (rc) return rc;
+    const size_t N = static_cast<size_t>(n);
+    double *dp1 = d, *dr1 = dp1 + 3 * N, *dp2 = dr1 + N, *dr2 = dp2 + 3 * N, *ddir = dr2 + N, *ddist = ddir + 3 * N,
+           *dout = ddist + N;
+    HIP_TRY(hipMemcpy(dp1, p1, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dr1, r1, N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dp2, p2, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dr2, r2, N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ddir, dir, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ddist, dist, N * sizeof(double), hipMemcpyHostToDevice));
+    const unsigned blocks = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_swept_batch, dim3(blocks), dim3(kBlock), 0, nullptr, static_cast<long long>(n), dp1, dr1, dp2,
+                       dr2, ddir, ddist, box, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, dout, N * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
 }
 
 const char *mcac_version(void) { return "mcac_gpu 0.2 (gfx950, FP64 collision path)"; }
