@@ -116,6 +116,11 @@ def overlap_runs() -> int:
     return 0
 
 
+def _progress_every(k: int):
+    """The batch runner prints a line to stderr every k-th finished job."""
+    os.environ["MCAC_BATCH_PROGRESS"] = str(k)
+
+
 def _pct(x, q):
     import numpy as np
     return float(np.percentile(np.asarray(x, float), q)) if len(x) else None
@@ -133,6 +138,7 @@ def run_ensemble(args) -> dict:
     boxes = ensemble.ensemble_configs(args.ensemble_boxes)
     print(f"bench: ensemble of {len(boxes)} boxes, {args.workers} workers x {args.group_boxes} boxes",
           file=sys.stderr, flush=True)
+    _progress_every(32)
     t0 = time.perf_counter()
     r = ensemble.batch_run([b["ini"] for b in boxes], workers=args.workers, max_steps=args.max_job_steps,
                            boxes_per_worker=args.group_boxes, depth=args.group_depth)
@@ -147,6 +153,7 @@ def run_ensemble(args) -> dict:
     if args.plain_ensemble:
         idx = list(range(0, len(boxes), max(1, args.plain_stride)))
         print(f"bench: {len(idx)} of the boxes one at a time (plain path)", file=sys.stderr, flush=True)
+        _progress_every(1)
         t0 = time.perf_counter()
         p = ensemble.batch_run([boxes[i]["ini"] for i in idx], workers=1, max_steps=args.max_job_steps)
         pw = time.perf_counter() - t0
@@ -180,6 +187,7 @@ def run_ensemble_compare(args) -> dict:
     inis = [b["ini"] for b in boxes]
     print(f"bench: ensemble comparison on {len(boxes)} boxes (N <= {args.compare_nmax:g})", file=sys.stderr,
           flush=True)
+    _progress_every(4)
     t0 = time.perf_counter()
     r = ensemble.batch_run(inis, workers=args.workers, max_steps=args.max_job_steps, boxes_per_worker=args.group_boxes,
                            depth=args.group_depth)
@@ -210,6 +218,7 @@ def run_serving(args) -> dict:
     from mcac_amd import ensemble
     reqs = ensemble.serving_requests(args.serving_requests, args.serving_rate, n_max=args.serving_nmax)
     print(f"bench: serving {len(reqs)} requests at {args.serving_rate}/s", file=sys.stderr, flush=True)
+    _progress_every(4)
     t0 = time.perf_counter()
     budget = int(args.serving_hbm_mib) << 20
     r = ensemble.batch_run([q["ini"] for q in reqs], arrivals=[q["arrival"] for q in reqs], workers=args.workers,
@@ -256,7 +265,7 @@ def main() -> int:
                     help="only run a plain 256 MiB device copy 20 times (HBM counter calibration)")
     ap.add_argument("--ensemble-boxes", type=int, default=512,
                     help="boxes of the 512-box ensemble list run to completion (0: skip)")
-    ap.add_argument("--plain-stride", type=int, default=16,
+    ap.add_argument("--plain-stride", type=int, default=32,
                     help="every n-th box of the ensemble list is also run one at a time on the plain path")
     ap.add_argument("--compare-stride", type=int, default=16,
                     help="every n-th box of the ensemble list (with N <= --compare-nmax) is run on the CPU too")
@@ -272,8 +281,8 @@ def main() -> int:
                     help="largest request size in the serving sample (the full form allows 2e5)")
     ap.add_argument("--tuned", default="",
                     help="shape file of python3 -m mcac_amd.autotune: its workers, boxes per worker and depth")
-    ap.add_argument("--workers", type=int, default=4, help="host threads sharing the GPU (ensemble, serving)")
-    ap.add_argument("--group-boxes", type=int, default=4,
+    ap.add_argument("--workers", type=int, default=8, help="host threads sharing the GPU (ensemble, serving)")
+    ap.add_argument("--group-boxes", type=int, default=64,
                     help="boxes per worker stepped in lock step, one launch per round (0: one box per worker)")
     ap.add_argument("--group-depth", type=int, default=8, help="steps of each box per grouped launch (1..8)")
     ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")

@@ -104,9 +104,11 @@ int mcac_write_test_snapshots(const char *prefix, int64_t n, int64_t grids, int6
 
 /* Device memory (bytes) the box of this parameter file takes
  * (mcac_box_device_bytes of the parameters Simulation uses: one sphere and
- * one aggregate id per monomer); -1 when the text does not parse or asks for
- * an option this path does not run.  No reference counterpart: admission of
- * runs by an HBM budget. */
+ * one aggregate id per monomer; with domain duplication, the box after the
+ * last duplication a mean-size limit allows, 8^k times the spheres where
+ * 8^k < mean_monomere_per_aggregate); -1 when the text does not parse or asks
+ * for an option this path does not run.  No reference counterpart: admission
+ * of runs by an HBM budget. */
 int64_t mcac_sim_device_bytes(const char *ini_text);
 
 /* Ensemble / serving runs (no reference counterpart: the reference runs one

@@ -345,7 +345,17 @@ struct Simulation {
         if (rc == 0) rc = mcac_box_download_radii(box, r.data(), sv.data(), ss.data());
         std::vector<int64_t> cnt(A), mem(S);
         if (rc == 0) rc = mcac_box_download_members(box, cnt.data(), mem.data());
+        // the host keeps positions only from placement and merges: the moves
+        // since then are on the device
+        std::vector<double> pos_d(3 * A), rcm_d(3 * A);
+        if (rc == 0)
+            rc = mcac_box_download_aggs(box, pos_d.data(), rcm_d.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
+        for (size_t a = 0; a < A; ++a)
+            for (size_t l = 0; l < 3; ++l) {
+                com[a][l] = pos_d[3 * a + l];
+                rcm[a][l] = rcm_d[3 * a + l];
+            }
         std::vector<int64_t> off(A + 1, 0);
         for (size_t a = 0; a < A; ++a) off[a + 1] = off[a] + cnt[a];
         const std::vector<int> ids(live.ids());
@@ -1132,11 +1142,23 @@ int64_t mcac_sim_device_bytes(const char *ini_text) {
         mcac::PhysicalModel pm;
         pm.load(mcac::IniFile(ini_text));
         if (!pm.unsupported.empty()) return -1;
-        mcac_box_params bp{};  // the box Simulation::init creates (one sphere and one id per monomer)
-        bp.box_length = pm.box_length > 0 ? pm.box_length : 1.0;
+        // the box Simulation::init creates (one sphere and one id per monomer),
+        // or with domain duplication the largest one the run can reach: the
+        // k-th duplication finds at least 8^k monomers per aggregate, so it
+        // happens only below a mean-size limit above 8^k (without that limit,
+        // the initial box)
+        int64_t dups = 0, scale = 1;
+        if (pm.with_domain_duplication && pm.mean_monomere_per_aggregate_limit > 0)
+            for (double npp = 8.0; npp < pm.mean_monomere_per_aggregate_limit && dups < 6; npp *= 8.0) {
+                dups += 1;
+                scale *= 8;
+            }
+        const int64_t n0 = static_cast<int64_t>(pm.n_monomeres);
+        mcac_box_params bp{};
+        bp.box_length = (pm.box_length > 0 ? pm.box_length : 1.0) * static_cast<double>(int64_t(1) << dups);
         bp.n_div = static_cast<int64_t>(pm.n_verlet_divisions);
-        bp.max_spheres = static_cast<int64_t>(pm.n_monomeres);
-        bp.max_aggregates = static_cast<int64_t>(pm.n_monomeres);
+        bp.max_spheres = n0 * scale;
+        bp.max_aggregates = n0 + 7 * dups * (n0 / 8);
         bp.volsurf = pm.volsurf;
         return mcac_box_device_bytes(&bp);
     } catch (const std::exception &) {

@@ -37,6 +37,19 @@ def test_sim_bytes_follow_the_parameter_file():
     assert ensemble.device_bytes("[numerics]\nrandom_seed = -1\n") == -1  # unsupported (clock-seeded) run
 
 
+def test_sim_bytes_cover_the_duplications_the_mean_size_limit_allows():
+    # k-th duplication needs at least 8^k monomers per aggregate: a limit of
+    # 60 allows one (8 < 60 <= 64), 300 two, 8 none
+    base = ensemble.box_ini(1000, 1e-3, 1500.0, 1)
+    def dup(npp):
+        return (base.replace("with_domain_duplication = false", "with_domain_duplication = true")
+                .replace("number_of_aggregates = 100", f"number_of_aggregates = 1\nmean_monomere_per_aggregate = {npp}"))
+    assert "with_domain_duplication = true" in dup(60) and "mean_monomere_per_aggregate = 60" in dup(60)
+    assert ensemble.device_bytes(dup(8)) == box_bytes(1000, 1000)
+    assert ensemble.device_bytes(dup(60)) == box_bytes(8000, 1000 + 7 * 125)
+    assert ensemble.device_bytes(dup(300)) == box_bytes(64000, 1000 + 14 * 125)
+
+
 @pytest.mark.gpu
 def test_box_allocation_matches_the_estimate():
     import torch

@@ -27,17 +27,18 @@ def test_duplication_run_bit_exact(npp, dups):
     while done < n_ora and not sim.finished():
         done += sim.run(min(50000, n_ora - done))
         print(f"  device: {done} steps, {sim.scalar('n_sph'):.0f} spheres", flush=True)
-    assert done == n_ora and sim.finished() and ora.finished()
-    assert sim.scalar("n_sph") == 400 * 8 ** dups
-    assert sim.scalar("box_length") == ora.scalar("box_length")
     a, b = ora.steps(), sim.steps()
-    assert len(a) == len(b)
+    m = min(len(a), len(b))
     for f in a.dtype.names:
-        eq = a[f] == b[f]
+        eq = a[f][:m] == b[f][:m]
         if eq.ndim > 1:
             eq = eq.all(axis=1)
         bad = np.nonzero(~eq)[0]
         assert bad.size == 0, f"step field {f} differs first at step {bad[0]}: {a[bad[0]]} vs {b[bad[0]]}"
+    assert done == n_ora and sim.finished() and ora.finished()
+    assert sim.scalar("n_sph") == 400 * 8 ** dups
+    assert sim.scalar("box_length") == ora.scalar("box_length")
+    assert len(a) == len(b)
     assert ora.merges().tobytes() == sim.merges().tobytes()
     assert sim.scalar("time") == ora.scalar("time")
     _, agg = ora.export()
