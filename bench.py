@@ -216,7 +216,8 @@ def run_serving(args) -> dict:
     """Serving form: Poisson arrivals, time-to-result percentiles."""
     import numpy as np
     from mcac_amd import ensemble
-    reqs = ensemble.serving_requests(args.serving_requests, args.serving_rate, n_max=args.serving_nmax)
+    reqs = ensemble.serving_requests(args.serving_requests, args.serving_rate, n_max=args.serving_nmax,
+                                     growth_flux=args.serving_growth_flux, growth_n_max=args.serving_growth_nmax)
     print(f"bench: serving {len(reqs)} requests at {args.serving_rate}/s", file=sys.stderr, flush=True)
     _progress_every(4)
     t0 = time.perf_counter()
@@ -250,8 +251,10 @@ def run_serving(args) -> dict:
                               "same_step_counts": bool(np.array_equal(p["steps"], r["steps"])),
                               "note": "first-come-first-served over the measured one-at-a-time run times"}
     return out | {
-        "sample": f"N log-uniform 1e3..{args.serving_nmax:g} (full form 2e5), lognormal 30/1.25, FV U[1e-4,5e-3], "
-                  f"T U[1200,2000] K, until N_agg <= N/10, no surface reactions"}
+        "sample": f"N log-uniform 1e3..{args.serving_nmax:g} (full form 2e5; odd requests with surface growth at "
+                  f"flux {args.serving_growth_flux:g}, N up to {args.serving_growth_nmax:g}), lognormal 30/1.25, "
+                  f"FV U[1e-4,5e-3], "
+                  f"T U[1200,2000] K, until N_agg <= N/10"}
 
 
 def main() -> int:
@@ -271,12 +274,16 @@ def main() -> int:
                     help="every n-th box of the ensemble list (with N <= --compare-nmax) is run on the CPU too")
     ap.add_argument("--no-plain-ensemble", dest="plain_ensemble", action="store_false",
                     help="skip running the ensemble sample again one box at a time")
-    ap.add_argument("--compare-nmax", type=float, default=8000,
+    ap.add_argument("--compare-nmax", type=float, default=4000,
                     help="largest box of the ensemble sample also run on the CPU for the runs/hour comparison")
     ap.add_argument("--serving-requests", type=int, default=16, help="serving requests (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=4.0, help="serving arrival rate, requests/s")
     ap.add_argument("--serving-hbm-mib", type=int, default=4096,
                     help="device memory the admitted serving boxes may hold at once (0: no limit)")
+    ap.add_argument("--serving-growth-flux", type=float, default=1e-3,
+                    help="surface growth flux of the odd serving requests (0: none)")
+    ap.add_argument("--serving-growth-nmax", type=float, default=5e3,
+                    help="largest N of a serving request with surface growth (its steps update every aggregate)")
     ap.add_argument("--serving-nmax", type=float, default=2e4,
                     help="largest request size in the serving sample (the full form allows 2e5)")
     ap.add_argument("--tuned", default="",

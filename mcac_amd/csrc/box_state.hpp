@@ -37,12 +37,11 @@ struct SearchState {
     unsigned long long tick;   // k_step blocks: finished << 56 | appended << 32 | visited
     unsigned long long fp32_tested, fp32_kept, fp32_missed;  // FP32 broad-phase accounting
     // persistent step engine (k_step_engine)
-    unsigned long long eng_done;      // last job index whose finishing block completed the step
+    unsigned long long eng_done;      // 2 * last job index completed + 1 if that step closed the gate
     unsigned long long eng_decision;  // job index * 4 + outcome (kEngineRun, kEngineQuit, kEngineExit), block 0
-    unsigned long long eng_last;      // clock of the last completed step
-    unsigned long long eng_busy;      // clock ticks of completed steps (from when each could start)
+    unsigned long long eng_busy;      // clock ticks of completed steps (finishing block: done wait to done)
     unsigned long long eng_steps;     // steps completed by the engine
-    unsigned long long eng_idle;      // clock ticks between a step's end and the next job's arrival
+    unsigned long long eng_idle;      // clock ticks block 0 waited for posted jobs
 };
 
 // Record published to pinned host memory by the last block of a search.
@@ -112,7 +111,7 @@ struct DevBox {
     SearchState *ss;
     MergeState *ms;
     struct EngineJob *eng_jobs;        // persistent engine: block 0's copy of each queued job, by ring slot
-    unsigned long long *eng_t;         // persistent engine: clock when each ring slot's job was taken
+    unsigned long long *eng_closed;    // persistent engine: eng_done's word of each completed job, by ring slot
 };
 
 struct SearchArgs {

@@ -240,6 +240,19 @@ double PhysicalModel::friction_coeff(double av, double sv, double sr) const {
     double cc = cunningham(sr);
     return (6. * kPi * viscosity * sr / cc) * std::pow(av / sv, fe / fractal_dimension);
 }
+// friction_coeff and mobility_diameter of one aggregate with their common
+// terms (friction exponent, Cunningham factor of the monomer radius, the
+// volume-ratio power) evaluated once: the same operations on the same
+// arguments, so the same results, with 5 libm calls instead of 8.
+void PhysicalModel::friction_and_mobility(double av, double sv, double sr, double *f, double *dm) const {
+    const double fe = friction_exponent(sr);
+    const double cc = cunningham(sr);
+    const double ratio_pow = std::pow(av / sv, fe / fractal_dimension);
+    *f = (6. * kPi * viscosity * sr / cc) * ratio_pow;
+    const double ar = sr * std::pow(av / sv, fe / fractal_dimension / 2.0);
+    const double cc_a = cunningham(ar);
+    *dm = (cc_a / cc) * 2.0 * sr * ratio_pow;
+}
 // physical_model.cpp:607-617
 double PhysicalModel::mobility_diameter(double av, double sv, double sr) const {
     double fe = friction_exponent(sr);
@@ -288,8 +301,7 @@ AggregatePhysics aggregate_physics(const PhysicalModel &pm, size_t n, double vol
     AggregatePhysics o{};
     o.dp = 2 * radius_sum / static_cast<double>(n);
     double vol_pp = sphere_volume / static_cast<double>(n);  // list-order sum of the spheres' own volumes
-    o.f_agg = pm.friction_coeff(volume, vol_pp, 0.5 * o.dp);
-    o.d_m = pm.mobility_diameter(volume, vol_pp, 0.5 * o.dp);
+    pm.friction_and_mobility(volume, vol_pp, 0.5 * o.dp, &o.f_agg, &o.d_m);
     double masse = pm.density * volume;
     double relax_time = masse / o.f_agg;
     o.time_step = 3. * relax_time;

@@ -98,6 +98,8 @@ struct PhysicalModel {
     double friction_exponent(double r) const;
     double friction_coeff(double aggregate_volume, double sphere_volume, double sphere_radius) const;
     double mobility_diameter(double aggregate_volume, double sphere_volume, double sphere_radius) const;
+    void friction_and_mobility(double aggregate_volume, double sphere_volume, double sphere_radius, double *friction,
+                               double *mobility_diam) const;
     double diffusivity(double f_agg) const { return kBoltzmann * temperature / f_agg; }
     double random_diameter(RandomStream &rng) const;
     void update(size_t n_aggregates, size_t n_monomers, double total_volume, double total_surface);

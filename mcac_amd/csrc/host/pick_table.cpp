@@ -105,7 +105,7 @@ class Workers {
     };
     static constexpr std::chrono::microseconds kSpin{300};
     Workers() {
-        const size_t T = std::min<size_t>(8, std::max(1u, std::thread::hardware_concurrency()));
+        const size_t T = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
         for (size_t t = 1; t < T; ++t) threads_.emplace_back([this] { loop(); });
     }
     // claims and runs jobs of the task; the thread that completes the last one wakes the caller
@@ -261,6 +261,34 @@ void PickTable::materialize(Chunk &c, double A) {
     c.dirty = false;
 }
 
+// The sums from A again from position `from` on, after entries changed there:
+// the positions before it keep their sums, their flags and the chunk's A.
+void PickTable::resum_from(Chunk &c, size_t from) {
+    const size_t m = c.ts.size();
+    c.L.resize(m);
+    c.flags.resize(static_cast<size_t>(std::lower_bound(c.flags.begin(), c.flags.end(), static_cast<int>(from)) -
+                                       c.flags.begin()));
+    double L = c.L[from - 1];
+    int64_t e = static_cast<int64_t>(bits_of(L) >> 52);
+    const double *tp = c.tp.data();
+    double *out = c.L.data();
+    for (size_t i = from; i < m; ++i) {
+        const double t = tp[i];
+        const double x = L + t;
+        const double err = (L >= t) ? t - (x - L) : two_sum_err(L, t, x);
+        const uint64_t xb = bits_of(x);
+        const int64_t ex = static_cast<int64_t>(xb >> 52);
+        const uint64_t eb = bits_of(err) & 0x7fffffffffffffffull;
+        const bool tie = eb != 0 && (ex > 53 ? eb == (static_cast<uint64_t>(ex - 53) << 52)
+                                             : std::fabs(err) == half_spacing(x));
+        if (ex != e || tie) c.flags.push_back(static_cast<int>(i));
+        e = ex;
+        L = x;
+        out[i] = L;
+    }
+    c.dirty = false;
+}
+
 // Offsets of the chunk for the true start S.
 void PickTable::settle(Chunk &c, double S) {
     const size_t m = c.ts.size();
@@ -376,7 +404,18 @@ void PickTable::walk(size_t from) {
     double S = (from == 0) ? 0.0 : chunks_[from - 1].true_end;
     for (size_t k = from; k < chunks_.size(); ++k) {
         Chunk &c = chunks_[k];
-        if (c.dirty) materialize(c, S);
+        if (c.dirty) {
+            if (c.dirty_from > 0 && c.dirty_from < c.ts.size()) resum_from(c, c.dirty_from);
+            else if (c.dirty_from == 0 || c.ts.empty()) materialize(c, S);
+            else {  // only the last entries were removed: the sums before them hold
+                c.L.resize(c.ts.size());
+                c.flags.resize(static_cast<size_t>(
+                    std::lower_bound(c.flags.begin(), c.flags.end(), static_cast<int>(c.ts.size())) -
+                    c.flags.begin()));
+                c.dirty = false;
+            }
+            c.dirty_from = 0;
+        }
         settle(c, S);
         S = c.true_end;
     }
@@ -416,6 +455,7 @@ void PickTable::erase_entry(double t, int id, size_t &first) {
     c.ts.erase(c.ts.begin() + static_cast<long>(lo));
     c.tp.erase(c.tp.begin() + static_cast<long>(lo));
     c.id.erase(c.id.begin() + static_cast<long>(lo));
+    c.dirty_from = c.dirty ? std::min(c.dirty_from, lo) : lo;
     c.dirty = true;
     if (c.ts.empty()) {
         chunks_.erase(chunks_.begin() + static_cast<long>(k));
@@ -437,6 +477,7 @@ void PickTable::insert_entry(double t, int id, size_t &first) {
     c.ts.insert(c.ts.begin() + static_cast<long>(lo), t);
     c.tp.insert(c.tp.begin() + static_cast<long>(lo), factor_ / t);
     c.id.insert(c.id.begin() + static_cast<long>(lo), id);
+    c.dirty_from = c.dirty ? std::min(c.dirty_from, lo) : lo;
     c.dirty = true;
     if (c.ts.size() > 2 * chunk_) {
         Chunk tail;

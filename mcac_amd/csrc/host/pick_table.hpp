@@ -67,6 +67,7 @@ class PickTable {
         double fast_d = 0.0;
         double true_end = 0.0;
         bool dirty = true;         // entries changed: re-sum from the exact start
+        size_t dirty_from = 0;     // first changed position (the sums before it from A still hold)
     };
     size_t chunk_;
     size_t n_ = 0;
@@ -87,6 +88,7 @@ class PickTable {
     std::vector<uint32_t> tie_l_, tie_r_;  // scratch of its parallel partitions
 
     void materialize(Chunk &c, double A);
+    void resum_from(Chunk &c, size_t from);
     void settle(Chunk &c, double S);
     void walk(size_t from);
     double true_at(const Chunk &c, size_t i) const;

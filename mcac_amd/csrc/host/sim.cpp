@@ -1242,6 +1242,7 @@ double mcac_sim_scalar(mcac_sim *sim, const char *name) {
     if (n == "t_wait_ms") return s.t_wait;
     if (n == "t_merge_call_ms") return s.t_merge_call;
     if (n == "t_drain_ms") return s.t_drain;
+    if (n == "t_grow_ms") return s.t_grow;
     if (n == "init_ms") return s.t_init;
     if (n == "total_volume") {
         double t = 0.0;

@@ -20,9 +20,14 @@ constexpr double kSurfaceFactorD = 4 * kPi;     // constants.hpp:115
 
 // ids[0..n_ids) may hold dead ids (count 0): skipped.  Outputs by position in
 // ids: volume, surface, list-order radius sum, Rg and rmax of each live aggregate.
+// The contact search's copies of CoM and rmax follow in place: a dirty
+// aggregate's dirty-list entry, or the packed cell entry of one still in the
+// cell it was built in; *o_left counts the aggregates whose growth took them
+// out of their built cell (their entries are stale: the cell list must be
+// rebuilt before the next search).
 __global__ __launch_bounds__(kBlock) void k_grow_update(DevBox b, const int *ids, int n_ids, double dr, double L,
                                                        int n_div, double *o_vol, double *o_surf, double *o_rsum,
-                                                       double *o_rg, double *o_rmax) {
+                                                       double *o_rg, double *o_rmax, int *o_left) {
     const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = static_cast<int>(threadIdx.x & 63);
     if (wave >= n_ids) return;
@@ -102,8 +107,25 @@ __global__ __launch_bounds__(kBlock) void k_grow_update(DevBox b, const int *ids
         b.ax[id] = px;
         b.ay[id] = py;
         b.az[id] = pz;
-        b.acell[id] = ref_cell(px, py, pz, n_div, L);
+        const int cell = ref_cell(px, py, pz, n_div, L);
+        const int built = b.acell[id];
+        b.acell[id] = cell;
         b.armax[id] = rmax;
+        if (b.dirty[id]) {
+            set_dirty_entry(b, b.dslot[id], px, py, pz, rmax, cell);
+        } else {
+            const int cs = b.cslot[id];
+            if (cs >= 0 && cell == built) {
+                b.cpx[cs] = px;
+                b.cpy[cs] = py;
+                b.cpz[cs] = pz;
+                b.cpr[cs] = rmax;
+                b.cf[cs] = make_float4(static_cast<float>(px), static_cast<float>(py), static_cast<float>(pz),
+                                       static_cast<float>(rmax));
+            } else {
+                atomicAdd(o_left, 1);
+            }
+        }
         b.arg[id] = rg;
         o_vol[wave] = V;
         o_surf[wave] = S;

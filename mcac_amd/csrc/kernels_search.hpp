@@ -393,6 +393,22 @@ __device__ __forceinline__ MoverSphere load_mover_sphere(const DevBox &b, const 
     return m;
 }
 
+// What a step loads first that no other step changes while it is queued
+// behind them without a contact: the mover's packed-entry slot (thread 0) and
+// its spheres with their relative positions.  The persistent engine loads it
+// while the previous step finishes.
+struct StepPrefetch {
+    int src_slot;
+    MoverSphere mover;
+};
+
+__device__ __forceinline__ StepPrefetch prefetch_step(const DevBox &b, const SearchArgs &a) {
+    StepPrefetch p;
+    p.src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
+    p.mover = load_mover_sphere(b, a);
+    return p;
+}
+
 __device__ __forceinline__ long long chunk_count(long long pairs, long long chunk) {
     return (pairs + chunk - 1) / chunk;
 }
@@ -759,17 +775,26 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
 // role (optional, LDS, set by thread 0): 0 the step was skipped behind the
 // gate, 1 this block took part, 2 this block finished the step.
 __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, unsigned vb, unsigned vg,
-                                          bool gate_open, const CellTable *ct, bool publish, int *role = nullptr) {
+                                          bool gate_open, const CellTable *ct, bool publish, int *role = nullptr,
+                                          const StepPrefetch *pre = nullptr) {
     // independent loads first, so that their latencies overlap: the gate, the
     // mover's cell entry and sphere indices, the bounds of the visited cells
     const unsigned long long gate = (!gate_open && a.gated) ? b.ss->gate : 0ull;
-    const int src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
     const bool mload = a.apply && a.n_mov <= kBlock;
-    const int msp = (mload && static_cast<int>(threadIdx.x) < a.n_mov) ? b.members[a.start + threadIdx.x] : -1;
+    int src_slot = -1, msp = -1;
+    if (pre) {
+        src_slot = pre->src_slot;
+        msp = pre->mover.sp;
+    } else {
+        src_slot = (threadIdx.x == 0) ? b.cslot[a.src] : -1;
+        msp = (mload && static_cast<int>(threadIdx.x) < a.n_mov) ? b.members[a.start + threadIdx.x] : -1;
+    }
     const CellBounds cb = load_cell_bounds(b, a, ct);
     const bool skip = gate != 0ull && gate < a.seq && gate > a.released;
     MoverSphere mover{mload, msp, 0., 0., 0.};
-    if (msp >= 0) {
+    if (pre) {
+        mover = pre->mover;
+    } else if (msp >= 0) {
         mover.rx = b.srx[msp];
         mover.ry = b.sry[msp];
         mover.rz = b.srz[msp];
@@ -972,11 +997,18 @@ __global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, cons
 // per step.  Block 0 alone polls the ring; it copies job k into device memory
 // (the box's eng_jobs slot) and publishes its decision for k (run, exit, or
 // quit after idle_ticks of clock without a job) in eng_decision; the other
-// blocks wait for that decision.  A step starts only when the finishing block
-// of the previous step that ran has set eng_done (its moves, dirty list, gate
-// and reset counters are then visible); a step behind the closed gate is
-// skipped by every block alike.  Every wait is bounded (a wait that runs out
-// ends the grid and reports reason -1), so the grid always ends.
+// blocks wait for that decision, which usually comes while the previous step
+// still runs.  Each block then loads what the step reads of the mover (no
+// queued step changes it unless it closes the gate) and waits until the last
+// step that ran is done: its finishing block writes eng_done = 2 k + (1 if it
+// closed the gate) after the step's device writes and before it publishes the
+// record to the host, so the next step starts one host round trip earlier.
+// Every engine step is gated and its released floor never decreases, so a
+// step is behind the gate exactly when the last step that ran closed it and
+// was not released by it; the first step of a launch reads the gate itself.
+// Blocks beyond a job's share of the grid (vg) follow the steps without work.
+// Every wait is bounded (a wait that runs out ends the grid and reports reason
+// -1), so the grid always ends.
 constexpr unsigned long long kEngineWaitFactor = 8;  // a block's wait for the others: 8 idle periods
 
 __device__ __forceinline__ void copy_words(unsigned long long *dst, const unsigned long long *src, int n) {
@@ -987,11 +1019,13 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
                                                         unsigned long long first, unsigned long long idle_ticks,
                                                         unsigned long long launch_no) {
     __shared__ EngineJob job;
-    __shared__ int s_cmd, s_role, s_gate;
+    __shared__ int s_cmd, s_run;
     constexpr int kJobWords = static_cast<int>(offsetof(EngineJob, seq) / sizeof(unsigned long long));
     static_assert(offsetof(EngineJob, seq) % sizeof(unsigned long long) == 0, "job body in 8-byte words");
     SearchState *s = b.ss;
-    unsigned long long need_done = 0;  // eng_done must reach this before the next step (0: nothing ran yet)
+    unsigned long long last = 0;      // the last job that ran in this launch (0: none yet)
+    unsigned long long last_seq = 0;  // its step's sequence number
+    unsigned long long busy = 0, idle = 0, steps = 0;
     unsigned long long k = first;
     int reason = kEngineExit;
     for (;; ++k) {
@@ -1008,6 +1042,7 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
+                idle += wall_clock64() - t0;
             } else {
                 const unsigned long long limit = kEngineWaitFactor * idle_ticks;
                 while (wall_clock64() - t0 <= limit) {
@@ -1027,7 +1062,7 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
             s_cmd = cmd;
         }
         __syncthreads();
-        int cmd = s_cmd;
+        const int cmd = s_cmd;
         if (blockIdx.x == 0) {
             if (cmd == kEngineRun) {
                 copy_words(reinterpret_cast<unsigned long long *>(&job),
@@ -1038,8 +1073,7 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
                 __syncthreads();
             }
             if (threadIdx.x == 0) {
-                if (cmd == kEngineRun) b.eng_t[slot] = wall_clock64();
-                __threadfence();  // the job copy and its clock before the decision
+                __threadfence();  // the job copy before the decision
                 __hip_atomic_store(&s->eng_decision, k * 4ull + static_cast<unsigned long long>(cmd),
                                    __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -1052,57 +1086,78 @@ __global__ __launch_bounds__(kBlock) void k_step_engine(DevBox b, const EngineJo
             reason = cmd;
             break;
         }
-        // the previous step that ran is complete
+        const unsigned vg = job.vg < 1 ? 1u : (job.vg > static_cast<int>(gridDim.x) ? gridDim.x : static_cast<unsigned>(job.vg));
+        const bool works = blockIdx.x < vg;
+        const SearchArgs a = job.a;  // in registers for the step, as a launch's kernel arguments are
+        StepPrefetch pre{};
+        if (works) pre = prefetch_step(b, a);  // in flight during the wait below
+        unsigned long long t_start = 0;
         if (threadIdx.x == 0) {
-            int ok = 1;
-            if (need_done) {
+            int run = 1;
+            if (last == 0) {
+                run = behind_gate(b, a) ? 0 : 1;
+            } else {
                 const unsigned long long t0 = wall_clock64(), limit = kEngineWaitFactor * idle_ticks;
-                ok = 0;
+                unsigned long long w = 0;
+                run = -1;
                 while (wall_clock64() - t0 <= limit) {
-                    if (__hip_atomic_load(&s->eng_done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= need_done) {
-                        ok = 1;
+                    w = __hip_atomic_load(&s->eng_done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((w >> 1) >= last) {
+                        run = 1;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
+                if (run == 1 && (w >> 1) > last) {
+                    // later steps ran without this block: the word of `last` is in its slot
+                    w = __hip_atomic_load(&b.eng_closed[last % kEngineRing], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                    if ((w >> 1) != last) run = -1;
+                }
+                if (run == 1) {
+                    // behind_gate with the gate closed by `last` (or open)
+                    const bool closed = (w & 1ull) != 0ull;
+                    run = (a.gated && closed && last_seq > a.released) ? 0 : 1;
+                }
             }
-            s_cmd = ok;
-            s_role = 0;
+            t_start = wall_clock64();
+            s_run = run;
         }
         __syncthreads();
-        if (!s_cmd) {
+        const int run = s_run;
+        if (run < 0) {
             reason = -1;
             break;
         }
-        // the gate is settled once the previous step is done: a step behind it
-        // is skipped by every workgroup at once (no read-only sweep); otherwise
-        // the step's own share of the grid works on it and the other
-        // workgroups only wait for it
-        const unsigned vg = job.vg < 1 ? 1u : (job.vg > static_cast<int>(gridDim.x) ? gridDim.x : static_cast<unsigned>(job.vg));
-        if (threadIdx.x == 0) s_gate = behind_gate(b, job.a) ? 1 : 0;
+        if (!run) continue;  // behind the gate: no block works on it, no record
+        last = k;
+        last_seq = a.seq;
+        if (!works) continue;
+        const CellTable *ct = job.cells.n > 0 ? &job.cells : nullptr;
+        int role = 0;
+        __shared__ int s_role;
+        const bool closed = step_body(b, a, blockIdx.x, vg, true, ct, false, &s_role, &pre);
         __syncthreads();
-        if (s_gate) {
-            if (threadIdx.x == 0) s_role = 0;
-        } else if (blockIdx.x < vg) {
-            const CellTable *ct = job.cells.n > 0 ? &job.cells : nullptr;
-            step_body(b, job.a, blockIdx.x, vg, true, ct, true, &s_role);
-        } else if (threadIdx.x == 0) {
-            s_role = 1;
-        }
-        __syncthreads();
-        const int role = s_role;
-        if (role >= 1) need_done = k;
+        role = s_role;
         if (role == 2 && threadIdx.x == 0) {
-            // the record is published; the step's writes and counter resets are
-            // made visible to the other blocks before eng_done
-            const unsigned long long now = wall_clock64(), t_job = b.eng_t[slot], last = s->eng_last;
-            s->eng_busy += now - (t_job > last ? t_job : last);
-            if (last && t_job > last) s->eng_idle += t_job - last;  // the grid waited for this job
-            s->eng_last = now;
-            s->eng_steps += 1;
+            // the step's device writes, then eng_done for the other blocks,
+            // then the record for the host
+            const unsigned long long word = (k << 1) | (closed ? 1ull : 0ull);
+            b.eng_closed[slot] = word;
             __threadfence();
-            __hip_atomic_store(&s->eng_done, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&s->eng_done, word, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence_system();
+            __hip_atomic_store(&a.out->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            busy += wall_clock64() - t_start;
+            steps += 1;
         }
+    }
+    if (threadIdx.x == 0) {
+        if (steps) {
+            atomicAdd(&s->eng_steps, steps);
+            atomicAdd(&s->eng_busy, busy);
+        }
+        if (blockIdx.x == 0) atomicAdd(&s->eng_idle, idle);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->stopped_at = k;

@@ -532,7 +532,7 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
     if (timed && b->inflight > 0) HIP_TRY(hipStreamSynchronize(bstream(b)));
     // the persistent engine takes one-launch steps whose mover the finishing
     // block moves (not timed: a timed step is its own launch between events)
-    const bool engine = !job_out && b->eng_blocks > 0 && !b->group && fused && !timed &&
+    const bool engine = !job_out && b->eng_blocks > 0 && !b->group && fused && !timed && gated &&
                         (!apply || a.translate_in_finish);
     if (job_out || engine) {
         // queued step arguments sit in LDS: the host adds the bounds of the
@@ -765,7 +765,7 @@ template <class F> int box_buffers(DevBox &d, const BoxSizes &z, int volsurf, F 
         ALLOC(d.walk_n, Ns);
     }
     ALLOC(d.ss, 1); ALLOC(d.ms, 1);
-    ALLOC(d.eng_jobs, kEngineRing); ALLOC(d.eng_t, kEngineRing);
+    ALLOC(d.eng_jobs, kEngineRing); ALLOC(d.eng_closed, kEngineRing);
 #undef ALLOC
     return 0;
 }
@@ -1706,8 +1706,9 @@ int mcac_box_grow(mcac_box *b, double dr, double *volume, double *surface, doubl
         b->h_grow = nullptr;
         b->d_grow = nullptr;
         b->grow_cap = 0;
-        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&b->d_grow), 5 * static_cast<size_t>(n) * sizeof(double)));
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&b->h_grow), 5 * static_cast<size_t>(n) * sizeof(double),
+        // five outputs per aggregate and the count of aggregates that left their cell
+        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&b->d_grow), (5 * static_cast<size_t>(n) + 1) * sizeof(double)));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&b->h_grow), (5 * static_cast<size_t>(n) + 1) * sizeof(double),
                               hipHostMallocDefault));
         b->grow_cap = n;
     }
@@ -1715,11 +1716,16 @@ int mcac_box_grow(mcac_box *b, double dr, double *volume, double *surface, doubl
     double *g = b->d_grow;
     hipStream_t st = bstream(b);
     const unsigned blocks = static_cast<unsigned>((n * 64 + kBlock - 1) / kBlock);
+    int *left = reinterpret_cast<int *>(g + 5 * N);
+    HIP_TRY(hipMemsetAsync(left, 0, sizeof(double), st));
     hipLaunchKernelGGL(k_grow_update, dim3(blocks), dim3(kBlock), 0, st, b->d, b->d.alive, static_cast<int>(n), dr,
-                       b->p.box_length, static_cast<int>(b->p.n_div), g, g + N, g + 2 * N, g + 3 * N, g + 4 * N);
+                       b->p.box_length, static_cast<int>(b->p.n_div), g, g + N, g + 2 * N, g + 3 * N, g + 4 * N,
+                       left);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(b->h_grow, g, 5 * N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(b->h_grow, g, (5 * N + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    int n_left = 0;
+    std::memcpy(&n_left, b->h_grow + 5 * N, sizeof(int));
     for (size_t w = 0; w < N; ++w) {
         const int id = b->alive_ids[w];
         if (b->count[static_cast<size_t>(id)] <= 0) continue;
@@ -1729,7 +1735,7 @@ int mcac_box_grow(mcac_box *b, double dr, double *volume, double *surface, doubl
         rg[id] = b->h_grow[3 * N + w];
         rmax[id] = b->h_grow[4 * N + w];
     }
-    b->cells_stale = true;
+    if (n_left > 0) b->cells_stale = true;  // else the search's copies were refreshed in place
     b->mirrors_exact = false;
     return 0;
 }
