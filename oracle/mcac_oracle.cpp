@@ -340,10 +340,8 @@ struct Physics {
         // surface growth (flux >= 0) with the whole-box update every step and no
         // overlap correction; oxidation (flux < 0: sphere removal and split),
         // per-aggregate reactions and sparser full updates are not restated
-        if (with_surface_reactions && flux_surfgrowth < 0) unsupported = "flux_surfgrowth<0 (oxidation)";
-        if (with_surface_reactions && individual_surf_reactions) unsupported = "individual_surf_reactions";
-        if (with_surface_reactions && full_aggregate_update_frequency != 1)
-            unsupported = "full_aggregate_update_frequency!=1";
+        if (with_surface_reactions && full_aggregate_update_frequency < 1)
+            unsupported = "full_aggregate_update_frequency<1";
         if (with_surface_reactions && volsurf != 0) unsupported = "volsurf_method with surface reactions";
         if (with_nucleation) unsupported = "with_nucleation";
         if (with_potentials) unsupported = "with_potentials";
@@ -1292,12 +1290,23 @@ struct Sim {
         return static_cast<size_t>(it - aggs.begin());
     }
     // aggregat_list.cpp:367-410
+    int64_t merged_keeper = -1, merged_removed = -1;  // stable ids of the last merge
     bool merge(const Contact &ci) {
         if (ci.ms < 0 || ci.os < 0) return false;
         size_t ms = static_cast<size_t>(ci.ms), os = static_cast<size_t>(ci.os);
         if (!contact(spos(ms), sr[ms], spos(os), sr[os], pm.box_length)) return false;
+        // Aggregate::merge (aggregat.cpp:488-493) refuses when an aggregate of the
+        // contact is gone (split or vanished since), and AggregatList::merge throws
+        auto alive = [this](int64_t uid) {
+            auto it = std::lower_bound(aggs.begin(), aggs.end(), uid,
+                                       [](const Agg &a, int64_t u) { return a.uid < u; });
+            return it != aggs.end() && it->uid == uid;
+        };
+        if (!alive(ci.magg) || !alive(ci.oagg)) throw std::runtime_error("MergeError");
         size_t keep = static_cast<size_t>(std::min(slabel[ms], slabel[os]));
         size_t rem = static_cast<size_t>(std::max(slabel[ms], slabel[os]));
+        merged_keeper = aggs[keep].uid;
+        merged_removed = aggs[rem].uid;
         double newtime = aggs[keep].proper_time + aggs[rem].proper_time - pm.time;
         // Aggregate::merge aggregat.cpp:486-544
         {
@@ -1333,7 +1342,15 @@ struct Sim {
             k.sph.insert(k.sph.end(), aggs[rem].sph.begin(), aggs[rem].sph.end());
             update(keep);
         }
-        // AggregatList::remove aggregat_list_storage.cpp:37-44 (~Aggregate unset_verlet first)
+        remove_aggregate(rem);
+        aggs[keep].proper_time = newtime;
+        return true;
+    }
+
+    // AggregatList::remove (aggregat_list_storage.cpp:37-44; ~Aggregate unsets
+    // its Verlet entry first): later labels shift down by one, in the Verlet
+    // cells and in the spheres' labels
+    void remove_aggregate(size_t rem) {
         cell(aggs[rem].vidx).erase(rem);
         aggs.erase(aggs.begin() + long(rem));
         for (size_t i = rem; i < aggs.size(); i++) {
@@ -1342,8 +1359,140 @@ struct Sim {
             for (size_t s : aggs[i].sph) slabel[s]--;
             c.insert(i);
         }
-        aggs[keep].proper_time = newtime;
+    }
+
+    // ------------------------------------------------------------ surface reactions
+    // spheres of the pending contact, followed through sphere removals (-1: removed)
+    int64_t track_ms = -1, track_os = -1;
+
+    // The aggregate's relative positions re-based on its first sphere, the
+    // full update, and the spheres placed again from the new position
+    // (aggregat.cpp:639-652, 606-617)
+    void rebase_update_place(size_t label) {
+        const V3 ref0 = spos(aggs[label].sph[0]);
+        for (size_t x : aggs[label].sph) {
+            const V3 r = spos(x) - ref0;
+            srx[x] = r[0];
+            sry[x] = r[1];
+            srz[x] = r[2];
+        }
+        update(label);
+        const Agg &a = aggs[label];
+        const V3 refpos = a.pos - a.rcm;
+        for (size_t x : a.sph) set_spos(x, refpos + srel(x));
+    }
+
+    // Aggregate::remove_sphere (aggregat.cpp:629-653) with SphereList::remove:
+    // the sphere leaves its aggregate and the storage, where later sphere
+    // indices shift down by one
+    void remove_sphere(size_t label, size_t s) {
+        Agg &a = aggs[label];
+        a.sph.erase(std::find(a.sph.begin(), a.sph.end(), s));
+        for (auto *v : {&sx, &sy, &sz, &sr, &svol, &ssurf, &srx, &sry, &srz}) v->erase(v->begin() + long(s));
+        slabel.erase(slabel.begin() + long(s));
+        for (auto &b : aggs)
+            for (auto &x : b.sph)
+                if (x > s) --x;
+        for (int64_t *t : {&track_ms, &track_os}) {
+            if (*t == static_cast<int64_t>(s)) *t = -1;
+            else if (*t > static_cast<int64_t>(s)) --*t;
+        }
+        if (!aggs[label].sph.empty()) rebase_update_place(label);
+    }
+
+    // AggregatList::croissance_surface(dt, index) (aggregat_list.cpp:567-579)
+    // and Aggregate::croissance_surface (aggregat.cpp:230-243): the aggregate's
+    // spheres grow by u_sg dt (Sphere::croissance_surface, sphere.cpp:113-120;
+    // PhysicalModel::grow, physical_model.cpp:587-590), then each sphere at or
+    // below rp_min_oxid is removed in list order; true when none is left (the
+    // aggregate is removed)
+    bool croissance_surface(double dt, size_t l) {
+        for (size_t sp : aggs[l].sph) {
+            const double new_r = sr[sp] + pm.u_sg * dt;
+            const double new_r_2 = new_r * new_r;
+            const double new_r_3 = new_r_2 * new_r;
+            sr[sp] = new_r;
+            svol[sp] = kVolumeFactor * new_r_3;
+            ssurf[sp] = kSurfaceFactor * new_r_2;
+        }
+        for (size_t i = 0; i < aggs[l].sph.size();) {
+            const size_t sp = aggs[l].sph[i];
+            if (sr[sp] <= pm.rp_min_oxid) remove_sphere(l, sp);
+            else ++i;
+        }
+        if (aggs[l].sph.empty()) {
+            remove_aggregate(l);
+            return true;
+        }
+        return false;
+    }
+    // AggregatList::croissance_surface(dt) (aggregat_list.cpp:549-566)
+    bool croissance_surface_all(double dt) {
+        bool removed = false;
+        for (size_t l = 0; l < aggs.size();) {
+            if (croissance_surface(dt, l)) removed = true;  // the next aggregate took label l
+            else ++l;
+        }
+        return removed;
+    }
+
+    // Aggregate::split (aggregat.cpp:553-628): the connected components of the
+    // spheres' contact graph (sphere_distances.cpp:81-90), each grown from the
+    // first unvisited sphere by a stack of discovered spheres that scans the
+    // unvisited ones in order; with two or more, each becomes a copy of the
+    // aggregate holding its spheres, appended to the list with the next stable
+    // id, re-based, updated and placed; AggregatList::split (aggregat_list.cpp:
+    // 433-444) then removes the original
+    bool split(size_t l) {
+        const size_t n = aggs[l].sph.size();
+        std::vector<size_t> unvisited(n);
+        std::iota(unvisited.begin(), unvisited.end(), size_t(0));
+        std::vector<std::vector<size_t>> comps;
+        while (!unvisited.empty()) {
+            std::vector<size_t> comp{unvisited.front()};
+            unvisited.erase(unvisited.begin());
+            std::vector<size_t> discovered = comp;
+            while (!discovered.empty()) {
+                const size_t a1 = discovered.back();
+                discovered.pop_back();
+                const size_t s1 = aggs[l].sph[a1];
+                for (size_t idx = 0; idx < unvisited.size();) {
+                    const size_t s2 = aggs[l].sph[unvisited[idx]];
+                    if (contact(spos(s1), sr[s1], spos(s2), sr[s2], pm.box_length)) {
+                        comp.push_back(unvisited[idx]);
+                        discovered.push_back(unvisited[idx]);
+                        unvisited.erase(unvisited.begin() + long(idx));
+                    } else {
+                        ++idx;
+                    }
+                }
+            }
+            comps.push_back(std::move(comp));
+        }
+        if (comps.size() < 2) return false;
+        for (const auto &comp : comps) {
+            Agg copy = aggs[l];
+            copy.uid = next_uid++;
+            copy.sph.clear();
+            for (size_t i : comp) copy.sph.push_back(aggs[l].sph[i]);
+            const size_t label = aggs.size();
+            copy.vidx = compute_index_verlet(copy);  // set_verlet at the copied position
+            aggs.push_back(std::move(copy));
+            cell(aggs[label].vidx).insert(label);
+            for (size_t x : aggs[label].sph) slabel[x] = static_cast<long>(label);
+            rebase_update_place(label);
+        }
+        remove_aggregate(l);
         return true;
+    }
+    // AggregatList::split() (aggregat_list.cpp:413-432)
+    bool split_all() {
+        bool any = false;
+        for (size_t l = 0; l < aggs.size();) {
+            if (split(l)) any = true;  // its parts were appended; the next aggregate took label l
+            else ++l;
+        }
+        return any;
     }
 
     // physical_model.cpp:288-337 (STOPCODE / cpu limits are host-environment checks, not modelled)
@@ -1360,26 +1509,10 @@ struct Sim {
         return false;
     }
 
-    // AggregatList::croissance_surface(dt) (aggregat_list.cpp:549-566) with
-    // u_sg >= 0: every sphere of every aggregate grows (Aggregate::
-    // croissance_surface, aggregat.cpp:230-243, Sphere::croissance_surface,
-    // sphere.cpp:113-120, PhysicalModel::grow, physical_model.cpp:587-590); no
-    // radius falls to rp_min_oxid, so no sphere or aggregate is removed
-    void grow_all(double dt) {
-        for (const auto &a : aggs)
-            for (size_t sp : a.sph) {
-                const double new_r = sr[sp] + pm.u_sg * dt;
-                const double new_r_2 = new_r * new_r;
-                const double new_r_3 = new_r_2 * new_r;
-                sr[sp] = new_r;
-                svol[sp] = kVolumeFactor * new_r_3;
-                ssurf[sp] = kSurfaceFactor * new_r_2;
-            }
-    }
-
-    // calcul.cpp:66-281 (one iteration; surface growth as calcul.cpp:155-205
-    // with individual_surf_reactions off and full_aggregate_update_frequency 1;
-    // no nucleation or potentials)
+    // calcul.cpp:66-281 (one iteration; surface reactions as calcul.cpp:155-205:
+    // growth or oxidation with sphere removal and splits, for every aggregate
+    // or the moved one alone, whole or partial updates; no nucleation,
+    // potentials or flame coupling)
     int step() {
         // calcul.cpp:72-81
         if (event && pm.with_domain_duplication && aggs.size() <= dup_threshold && !(pm.u_sg < 0.0)) duplication();
@@ -1399,18 +1532,46 @@ struct Sim {
             if (contact_flag) move = nc.distance;
         }
         translate(num, dir * move);
+        double deltatemps_indiv = aggs[num].time_step;  // calcul.cpp:108
         deltatemps = deltatemps * (move / full + static_cast<double>(1 - 1));
+        deltatemps_indiv = deltatemps_indiv * (move / full + static_cast<double>(1 - 1));
         aggs[num].proper_time += deltatemps;
         pm.time = pm.time + deltatemps;
-        if (pm.with_surface_reactions) grow_all(deltatemps);  // calcul.cpp:164-172
-        bool merged = false;
-        if (contact_flag) merged = merge(nc);
+        // calcul.cpp:155-172
+        bool split_flag = false, disappear = false;
+        track_ms = nc.ms;
+        track_os = nc.os;
         if (pm.with_surface_reactions) {
-            // calcul.cpp:183-205: n_iter_without_event % 1 == 0, so every
-            // aggregate gets the full update (the merged one a second time)
-            for (size_t l = 0; l < aggs.size(); l++) update(l);
+            if (pm.individual_surf_reactions) {
+                disappear = croissance_surface(deltatemps_indiv, num);
+                if (pm.u_sg < 0.0 && !disappear) split_flag = split(num);
+            } else {
+                disappear = croissance_surface_all(deltatemps);
+                if (pm.u_sg < 0.0) split_flag = split_all();
+            }
         }
-        event = merged;
+        // calcul.cpp:175-181 (the contact's spheres as they are now)
+        bool merged = false;
+        if (contact_flag && (!pm.individual_surf_reactions || !(split_flag || disappear))) {
+            Contact now = nc;
+            now.ms = track_ms;
+            now.os = track_os;
+            merged = merge(now);
+        }
+        // calcul.cpp:183-205
+        if (pm.with_surface_reactions) {
+            const bool whole = pm.n_iter_without_event % pm.full_aggregate_update_frequency == 0;
+            if (pm.individual_surf_reactions && !merged && !split_flag && !disappear) {
+                if (whole) update(num);
+                else update_partial(num, true);
+            } else {
+                for (size_t l = 0; l < aggs.size(); l++) {
+                    if (whole) update(l);
+                    else update_partial(l, true);
+                }
+            }
+        }
+        event = merged || split_flag || disappear;  // calcul.cpp:222
         if (event) {
             pm.n_iter_without_event = 0;
             total_events++;
@@ -1438,14 +1599,11 @@ struct Sim {
             steps.push_back(r);
         }
         if (merged && trace_merges) {
-            size_t keep = static_cast<size_t>(std::min(nc.magg, nc.oagg));
-            // keeper is the one with the smaller label, i.e. the smaller stable id
-            size_t kl = label_of_uid(static_cast<int64_t>(keep));
-            const Agg &a = aggs[kl];
+            const Agg &a = aggs[label_of_uid(merged_keeper)];
             ora_merge_rec m{};
             m.step = step_count;
             m.keeper = a.uid;
-            m.removed = std::max(nc.magg, nc.oagg);
+            m.removed = merged_removed;
             m.n = static_cast<int64_t>(a.sph.size());
             m.volume = a.volume; m.surface = a.surface; m.rg = a.rg; m.rmax = a.rmax; m.dp = a.dp;
             m.f_agg = a.f_agg; m.lpm = a.lpm; m.time_step = a.time_step; m.d_m = a.d_m;
@@ -1544,6 +1702,7 @@ double ora_get_scalar(void *sim, const char *name) {
     if (n == "volume_fraction") return s->pm.volume_fraction;
     if (n == "total_volume") return s->get_total_volume();
     if (n == "steps") return static_cast<double>(s->step_count);
+    if (n == "n_iter_without_event") return static_cast<double>(s->pm.n_iter_without_event);
     return std::nan("");
 }
 int64_t ora_n_spheres(void *sim) { return static_cast<int64_t>(static_cast<Sim *>(sim)->sx.size()); }
