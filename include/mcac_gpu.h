@@ -7,7 +7,9 @@
  * stable id (their creation index); the reference's label of an aggregate is
  * its rank among the live ids, so ascending id order is label order and no
  * relabelling is ever needed.  Spheres are named by their index, which never
- * changes on this path.
+ * changes within a box (oxidation's sphere removals are carried out by the
+ * caller on the state read back, and a new box holds the remaining spheres in
+ * their order, as SphereList::remove leaves them).
  *
  * The entry points replace these reference operators (MCAC, C++):
  *   mcac_next_contact  AggregatList::distance_to_next_contact
@@ -238,6 +240,20 @@ int mcac_box_members(mcac_box *box, int64_t agg, int64_t *out, int64_t max, int6
  * physics.  No step may be in flight and no inline merge pending. */
 int mcac_box_grow(mcac_box *box, double dr, double *volume, double *surface, double *radius_sum, double *rg,
                   double *rmax);
+/* mcac_box_grow for the surface reactions in general (calcul.cpp:155-205):
+ * the aggregates of ids[0..n_ids) (ids NULL: every live aggregate), spheres
+ * grown by dr (negative: oxidation) when grow != 0, else the update alone
+ * (radii, volumes and surfaces as stored: the whole-box update after an event
+ * with individual_surf_reactions, calcul.cpp:186-191).  flags (by id, NULL to
+ * skip) gets bit 0 when a sphere of the aggregate is at or below rp_min after
+ * growing (Aggregate::croissance_surface, aggregat.cpp:230-243, removes it)
+ * and, with check_split, bit 1 when its spheres are no longer one connected
+ * set under the contact test (Aggregate::split, aggregat.cpp:553-584) or it
+ * has more than 64 spheres (not checked).  The caller carries out removals
+ * and splits on the state it reads back, and uploads the result to a new box. */
+int mcac_box_grow_ids(mcac_box *box, const int64_t *ids, int64_t n_ids, double dr, int grow, double rp_min,
+                      int check_split, double *volume, double *surface, double *radius_sum, double *rg, double *rmax,
+                      int32_t *flags);
 
 /* Persistent step engine: with blocks >= 1 (at most 255), queued one-launch
  * steps (mcac_move_enqueue of a mover the step's own finishing block moves,

@@ -58,6 +58,7 @@ GPU_SYMBOLS = (
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
     "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
     "mcac_swept_sphere_batch_noalloc", "mcac_box_set_engine", "mcac_box_engine_stats", "mcac_box_grow",
+    "mcac_box_grow_ids",
     "mcac_box_download_radii", "mcac_box_download_members",
 )
 SIM_SYMBOLS = (

@@ -128,6 +128,7 @@ void PhysicalModel::load(const IniFile &ini) {
     ini.extract("surface_growth", "flux_surfgrowth", flux_surfgrowth);
     ini.extract("surface_growth", "full_aggregate_update_frequency", full_aggregate_update_frequency);
     ini.extract("numerics", "individual_surf_reactions", individual_surf_reactions);
+    ini.extract("oxidation", "rp_min", rp_min_oxid);
     s.clear();
     if (ini.get("surface_growth", "volsurf_method", s) && !s.empty()) {  // constants.hpp:103-111
         if (s == "caps") volsurf = 1;
@@ -135,12 +136,10 @@ void PhysicalModel::load(const IniFile &ini) {
         else if (s == "arvo") volsurf = 3;
         else if (s != "none") unsupported = "volsurf_method=" + s;
     }
-    // surface growth (calcul.cpp:155-205) runs with a non-negative flux, the
-    // whole-box update every step and no overlap correction; oxidation (a
-    // negative flux: sphere removal and split), per-aggregate reactions and
-    // sparser full updates are not offered
-    if (with_surface_reactions && flux_surfgrowth < 0) unsupported = "flux_surfgrowth<0 (oxidation)";
-    if (with_surface_reactions && individual_surf_reactions) unsupported = "individual_surf_reactions";
+    // surface reactions (calcul.cpp:155-205): growth or oxidation (sphere
+    // removal, splits), for every aggregate or the moved one, with the whole
+    // update every step and no overlap correction; sparser whole updates are
+    // not offered
     if (with_surface_reactions && full_aggregate_update_frequency != 1)
         unsupported = "full_aggregate_update_frequency!=1";
     if (with_surface_reactions && volsurf != 0) unsupported = "volsurf_method with surface reactions";

@@ -23,6 +23,7 @@ constexpr double kSutherland = 110;
 constexpr double kPressureRef = 101300;
 constexpr double kViscosityRef = 18.203E-6;
 constexpr double kContactEpsilon = 1e-28;
+constexpr double kCoordinationEpsilon = 1e-10;  // constants.hpp:128
 
 using Vec3 = std::array<double, 3>;
 
@@ -89,6 +90,7 @@ struct PhysicalModel {
     bool with_surface_reactions = false;  // surface growth (physical_model.cpp:79, 91, 127-159)
     size_t full_aggregate_update_frequency = 1;
     bool individual_surf_reactions = false;
+    double rp_min_oxid = 0.166e-09;  // physical_model.cpp:59, [oxidation] rp_min
     std::string unsupported;  // first option this path does not implement
 
     void load(const IniFile &ini);

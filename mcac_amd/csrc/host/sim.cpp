@@ -297,8 +297,8 @@ struct Simulation {
         mcac_box_params bp{};
         bp.box_length = pm.box_length;
         bp.n_div = static_cast<int64_t>(pm.n_verlet_divisions);
-        bp.max_spheres = static_cast<int64_t>(N);
-        bp.max_aggregates = static_cast<int64_t>(n_ids);
+        bp.max_spheres = std::max<int64_t>(1, static_cast<int64_t>(N));
+        bp.max_aggregates = std::max<int64_t>(1, static_cast<int64_t>(n_ids));
         bp.device = device;
         bp.precision = precision;
         bp.volsurf = pm.volsurf;
@@ -795,40 +795,355 @@ struct Simulation {
         return 0;
     }
 
-    // AggregatList::croissance_surface(dt) and the update of every aggregate
-    // (calcul.cpp:164-205): the device grows the spheres and returns each
-    // aggregate's volume, surface, radius sum, Rg and rmax; the physics
-    // (update_partial, aggregat.cpp:254-281) runs here with the host's libm.
-    int grow(double dt) {
+    // Surface reactions of one step and the updates they call for
+    // (calcul.cpp:155-205).  The device grows (or, with a negative u_sg,
+    // shrinks) the spheres of every live aggregate, or of the moved one alone
+    // with individual_surf_reactions, and updates those aggregates; the
+    // physics (update_partial, aggregat.cpp:254-281) runs here with the host's
+    // libm.  Oxidation's removals and splits, which the device flags, are
+    // carried out on the state read back (remove_and_split).
+    std::vector<int32_t> g_flag;
+    int react(double dt, int only, bool *split, bool *disappear) {
         auto t0 = Clock::now();
+        *split = *disappear = false;
         const double dr = pm.u_sg * dt;  // PhysicalModel::grow, physical_model.cpp:587-590
-        int rc = mcac_box_grow(box, dr, g_vol.data(), g_surf.data(), g_rsum.data(), g_rg.data(), g_rmax.data());
+        const bool oxid = pm.u_sg < 0.0;
+        g_flag.assign(n_ids, 0);
+        const int64_t one = only;
+        int rc = mcac_box_grow_ids(box, only >= 0 ? &one : nullptr, only >= 0 ? 1 : 0, dr, 1, pm.rp_min_oxid,
+                                   oxid ? 1 : 0, g_vol.data(), g_surf.data(), g_rsum.data(), g_rg.data(),
+                                   g_rmax.data(), g_flag.data());
         if (rc) return rc;
-        const std::vector<int> &ids = live.ids();
+        std::vector<int> grown;
+        if (only >= 0) grown.push_back(only);
+        else grown = live.ids();
+        bool events = false;
+        for (int id : grown) events = events || g_flag[static_cast<size_t>(id)] != 0;
+        take_physics(grown, true);
+        if (events) {
+            rc = remove_and_split(grown, split, disappear);
+            if (rc) return rc;
+        }
+        pm_stale = true;
+        t_grow += ms_since(t0);
+        return 0;
+    }
+
+    // Aggregate::update of every live aggregate without growth: the whole-box
+    // update after an event (calcul.cpp:186-191, 197-202), which re-derives
+    // each position from its first sphere.
+    int update_all() {
+        auto t0 = Clock::now();
+        g_flag.assign(n_ids, 0);
+        int rc = mcac_box_grow_ids(box, nullptr, 0, 0.0, 0, -1.0, 0, g_vol.data(), g_surf.data(), g_rsum.data(),
+                                   g_rg.data(), g_rmax.data(), g_flag.data());
+        if (rc) return rc;
+        take_physics(live.ids(), false);
+        t_grow += ms_since(t0);
+        return 0;
+    }
+
+    // The device's update outputs of `ids` into the per-id arrays and the
+    // physics; aggregates flagged for removal or split are left to
+    // remove_and_split (their state before it was never updated in the
+    // reference, so it does not raise maxradius).
+    void take_physics(const std::vector<int> &ids, bool skip_flagged) {
         constexpr size_t kPer = 512;
         host_parallel_for((ids.size() + kPer - 1) / kPer, [&](size_t k) {
             const size_t e = std::min(ids.size(), (k + 1) * kPer);
             for (size_t l = k * kPer; l < e; ++l) {
                 const size_t id = static_cast<size_t>(ids[l]);
+                if (skip_flagged && g_flag[id]) continue;
                 vol[id] = g_vol[id];
                 surf[id] = g_surf[id];
                 rg[id] = g_rg[id];
                 rmax[id] = g_rmax[id];
-                // method none: the list-order sum of the spheres' volumes is the volume
-                const AggregatePhysics ph =
-                    aggregate_physics(pm, static_cast<size_t>(nsph[id]), g_vol[id], g_vol[id], g_rsum[id], g_rg[id]);
-                dp[id] = ph.dp;
-                f_agg[id] = ph.f_agg;
-                d_m[id] = ph.d_m;
-                ts[id] = ph.time_step;
-                lpm[id] = ph.lpm;
-                dg_over_dp[id] = ph.dg_over_dp;
+                set_physics(id, g_rsum[id]);
             }
         });
         for (int id : ids)
-            if (rmax[static_cast<size_t>(id)] > maxradius) maxradius = rmax[static_cast<size_t>(id)];
-        pm_stale = true;
-        t_grow += ms_since(t0);
+            if (!(skip_flagged && g_flag[static_cast<size_t>(id)]) && rmax[static_cast<size_t>(id)] > maxradius)
+                maxradius = rmax[static_cast<size_t>(id)];
+    }
+    // update_partial's physics of one aggregate (method none: the list-order
+    // sum of the spheres' volumes is the volume)
+    void set_physics(size_t id, double radius_sum) {
+        const AggregatePhysics ph =
+            aggregate_physics(pm, static_cast<size_t>(nsph[id]), vol[id], vol[id], radius_sum, rg[id]);
+        dp[id] = ph.dp;
+        f_agg[id] = ph.f_agg;
+        d_m[id] = ph.d_m;
+        ts[id] = ph.time_step;
+        lpm[id] = ph.lpm;
+        dg_over_dp[id] = ph.dg_over_dp;
+    }
+
+    // Host copy of the box for the removals and splits of a step.
+    struct HostState {
+        std::vector<double> x, y, z, rx, ry, rz, r, sv, ss;
+        std::vector<std::vector<int64_t>> mem;  // member lists by id
+    };
+    // The contact's spheres followed through removals (-1: removed), so that
+    // the merge sees them as they are after the reactions.
+    int64_t track_ms = -1, track_os = -1;
+    bool track_touch = false;  // they touch after the reactions (sphere_distances.cpp:81-90)
+
+    // Aggregate::update (aggregat.cpp:284-288, 321-483, method none) of one
+    // aggregate on the host state: the same list-order sums and operations as
+    // the device update, plus update_distances_and_overlapping (:719-764).
+    void host_update(HostState &h, size_t id) {
+        const std::vector<int64_t> &m = h.mem[id];
+        double V = 0.0, S = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, rs = 0.0, brg = 0.0;
+        for (int64_t s64 : m) {
+            const size_t s = static_cast<size_t>(s64);
+            const double v = h.sv[s], r = h.r[s];
+            V = V + v;
+            S = S + h.ss[s];
+            vx = vx + h.rx[s] * v;
+            vy = vy + h.ry[s] * v;
+            vz = vz + h.rz[s] * v;
+            rs = rs + r;
+            brg = brg + v * (r * r);
+        }
+        const double cx = vx / V, cy = vy / V, cz = vz / V;
+        double rm = 0.0, arg = 0.0;
+        for (int64_t s64 : m) {
+            const size_t s = static_cast<size_t>(s64);
+            const double dx = h.rx[s] - cx, dy = h.ry[s] - cy, dz = h.rz[s] - cz;
+            const double dc = std::sqrt(dx * dx + dy * dy + dz * dz);
+            rm = smax(rm, h.r[s] + dc);
+            arg = arg + h.sv[s] * (dc * dc);
+        }
+        // overlapping and coordination number (aggregat.cpp:719-764)
+        double ovl = 0.0;
+        int inter = 0;
+        for (size_t i = 0; i < m.size(); ++i)
+            for (size_t j = i + 1; j < m.size(); ++j) {
+                const size_t a = static_cast<size_t>(m[i]), b = static_cast<size_t>(m[j]);
+                const double ex = h.rx[a] - h.rx[b], ey = h.ry[a] - h.ry[b], ez = h.rz[a] - h.rz[b];
+                const double dist = std::sqrt(ex * ex + ey * ey + ez * ez);
+                if (dist <= (1. + kCoordinationEpsilon) * (h.r[a] + h.r[b])) {
+                    ovl += 2.0 * ((h.r[a] + h.r[b] - dist) / (h.r[a] + h.r[b]));
+                    inter += 2;
+                }
+            }
+        overlapping[id] = inter > 0 ? ovl / static_cast<double>(inter) : 0.0;
+        coord[id] = inter > 0 ? static_cast<double>(inter) / static_cast<double>(m.size()) : 0.0;
+        const size_t s0 = static_cast<size_t>(m[0]);
+        const double L = pm.box_length;
+        com[id] = {periodic_position(h.x[s0] + cx, L), periodic_position(h.y[s0] + cy, L),
+                   periodic_position(h.z[s0] + cz, L)};
+        rcm[id] = {cx, cy, cz};
+        vol[id] = V;
+        surf[id] = S;
+        rmax[id] = rm;
+        rg[id] = std::sqrt(std::fabs((arg + 3. / 5. * brg) / V));
+        nsph[id] = static_cast<int>(m.size());
+        set_physics(id, rs);
+        if (rmax[id] > maxradius) maxradius = rmax[id];  // aggregat.cpp:277-281
+    }
+    // Relative positions re-based on the first sphere, the update, and the
+    // spheres placed again from the new position (aggregat.cpp:639-652, 606-617).
+    void rebase_update_place(HostState &h, size_t id) {
+        const std::vector<int64_t> &m = h.mem[id];
+        const size_t s0 = static_cast<size_t>(m[0]);
+        const double r0x = h.x[s0], r0y = h.y[s0], r0z = h.z[s0];
+        for (int64_t s64 : m) {
+            const size_t s = static_cast<size_t>(s64);
+            h.rx[s] = h.x[s] - r0x;
+            h.ry[s] = h.y[s] - r0y;
+            h.rz[s] = h.z[s] - r0z;
+        }
+        host_update(h, id);
+        const double px = com[id][0] - rcm[id][0], py = com[id][1] - rcm[id][1], pz = com[id][2] - rcm[id][2];
+        for (int64_t s64 : m) {
+            const size_t s = static_cast<size_t>(s64);
+            h.x[s] = px + h.rx[s];
+            h.y[s] = py + h.ry[s];
+            h.z[s] = pz + h.rz[s];
+        }
+    }
+
+    // Oxidation's events of the step on the state read back: spheres at or
+    // below rp_min leave their aggregates in list order (Aggregate::
+    // croissance_surface and remove_sphere, aggregat.cpp:230-243, 629-653; an
+    // aggregate left empty is removed), then aggregates that fell apart split
+    // into their connected parts (Aggregate::split, aggregat.cpp:553-628,
+    // AggregatList::split, aggregat_list.cpp:413-444): each part, grown from
+    // the first unvisited sphere by a stack of discovered spheres scanning the
+    // unvisited ones in order, becomes a copy of the aggregate with the next
+    // stable id.  The spheres that remain keep their order (SphereList::remove)
+    // and the device box is rebuilt from the result.
+    int remove_and_split(const std::vector<int> &grown, bool *split, bool *disappear) {
+        const size_t S = N, A = n_ids;
+        HostState h;
+        for (auto *v : {&h.x, &h.y, &h.z, &h.rx, &h.ry, &h.rz, &h.r, &h.sv, &h.ss}) v->resize(S);
+        int rc = mcac_box_download_spheres(box, h.x.data(), h.y.data(), h.z.data(), h.rx.data(), h.ry.data(),
+                                           h.rz.data());
+        if (rc == 0) rc = mcac_box_download_radii(box, h.r.data(), h.sv.data(), h.ss.data());
+        std::vector<int64_t> cnt(A), flat(S);
+        if (rc == 0) rc = mcac_box_download_members(box, cnt.data(), flat.data());
+        std::vector<double> pos_d(3 * A), rcm_d(3 * A);
+        if (rc == 0)
+            rc = mcac_box_download_aggs(box, pos_d.data(), rcm_d.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        h.mem.resize(A);
+        size_t off = 0;
+        for (size_t a = 0; a < A; ++a) {
+            h.mem[a].assign(flat.begin() + static_cast<long>(off), flat.begin() + static_cast<long>(off + cnt[a]));
+            off += static_cast<size_t>(cnt[a]);
+            com[a] = {pos_d[3 * a], pos_d[3 * a + 1], pos_d[3 * a + 2]};
+            rcm[a] = {rcm_d[3 * a], rcm_d[3 * a + 1], rcm_d[3 * a + 2]};
+        }
+        std::vector<char> gone(S, 0);
+        std::vector<int> checked;
+        // removals, aggregate by aggregate in label order
+        for (int id : grown) {
+            const size_t a = static_cast<size_t>(id);
+            if (!(g_flag[a] & 1)) {
+                if (g_flag[a] & 2) checked.push_back(id);
+                continue;
+            }
+            std::vector<int64_t> &m = h.mem[a];
+            for (size_t i = 0; i < m.size();) {
+                const size_t s = static_cast<size_t>(m[i]);
+                if (h.r[s] <= pm.rp_min_oxid) {
+                    gone[s] = 1;
+                    for (int64_t *t : {&track_ms, &track_os})
+                        if (*t == static_cast<int64_t>(s)) *t = -1;
+                    m.erase(m.begin() + static_cast<long>(i));
+                    if (!m.empty()) rebase_update_place(h, a);
+                } else {
+                    ++i;
+                }
+            }
+            if (m.empty()) *disappear = true;  // AggregatList::croissance_surface removes it
+            else checked.push_back(id);
+        }
+        // splits (only with oxidation), in label order; the parts take the next ids
+        const double L = pm.box_length;
+        for (int id : (pm.u_sg < 0.0 ? checked : std::vector<int>{})) {
+            const size_t a = static_cast<size_t>(id);
+            const std::vector<int64_t> m = h.mem[a];
+            const size_t n = m.size();
+            std::vector<size_t> unvisited(n);
+            std::iota(unvisited.begin(), unvisited.end(), size_t(0));
+            std::vector<std::vector<size_t>> comps;
+            while (!unvisited.empty()) {
+                std::vector<size_t> comp{unvisited.front()};
+                unvisited.erase(unvisited.begin());
+                std::vector<size_t> discovered = comp;
+                while (!discovered.empty()) {
+                    const size_t s1 = static_cast<size_t>(m[discovered.back()]);
+                    discovered.pop_back();
+                    for (size_t idx = 0; idx < unvisited.size();) {
+                        const size_t s2 = static_cast<size_t>(m[unvisited[idx]]);
+                        if (contact({h.x[s1], h.y[s1], h.z[s1]}, h.r[s1], {h.x[s2], h.y[s2], h.z[s2]}, h.r[s2], L)) {
+                            comp.push_back(unvisited[idx]);
+                            discovered.push_back(unvisited[idx]);
+                            unvisited.erase(unvisited.begin() + static_cast<long>(idx));
+                        } else {
+                            ++idx;
+                        }
+                    }
+                }
+                comps.push_back(std::move(comp));
+            }
+            if (comps.size() < 2) {
+                if (g_flag[a] & 1) continue;  // updated after its removals above
+                host_update(h, a);            // the device update stands, read back for the physics
+                continue;
+            }
+            *split = true;
+            for (const auto &comp : comps) {
+                const size_t c = h.mem.size();
+                h.mem.emplace_back();
+                for (size_t i : comp) h.mem[c].push_back(m[i]);
+                grow_ids_arrays(c + 1);
+                copy_fields(a, c);  // Aggregate(const Aggregate &): the fields, then its own spheres
+                rebase_update_place(h, c);
+            }
+            h.mem[a].clear();
+        }
+        return rebuild_after_events(h, gone);
+    }
+
+    void grow_ids_arrays(size_t n) {
+        if (n <= n_ids) return;
+        for (auto *v : {&ts, &lpm, &f_agg, &d_m, &dp, &rg, &rmax, &vol, &surf, &ptime, &dg_over_dp, &overlapping, &coord,
+                        &g_vol, &g_surf, &g_rsum, &g_rg, &g_rmax})
+            v->resize(n, 0.0);
+        com.resize(n, {0, 0, 0});
+        rcm.resize(n, {0, 0, 0});
+        nsph.resize(n, 0);
+        g_flag.resize(n, 0);
+        n_ids = n;
+    }
+    void copy_fields(size_t from, size_t to) {
+        for (auto *v : {&ts, &lpm, &f_agg, &d_m, &dp, &rg, &rmax, &vol, &surf, &ptime, &dg_over_dp, &overlapping, &coord})
+            (*v)[to] = (*v)[from];
+        com[to] = com[from];
+        rcm[to] = rcm[from];
+        nsph[to] = nsph[from];
+    }
+
+    // The device box for the state after removals and splits: the remaining
+    // spheres in their order, every id's member list remapped.
+    int rebuild_after_events(HostState &h, const std::vector<char> &gone) {
+        const size_t S = gone.size();
+        std::vector<int64_t> remap(S, -1);
+        size_t k = 0;
+        for (size_t s = 0; s < S; ++s)
+            if (!gone[s]) remap[s] = static_cast<int64_t>(k++);
+        const size_t S2 = k;
+        HostState c;
+        for (auto *v : {&c.x, &c.y, &c.z, &c.rx, &c.ry, &c.rz, &c.r, &c.sv, &c.ss}) v->resize(S2);
+        for (size_t s = 0; s < S; ++s) {
+            if (gone[s]) continue;
+            const size_t t = static_cast<size_t>(remap[s]);
+            c.x[t] = h.x[s], c.y[t] = h.y[s], c.z[t] = h.z[s];
+            c.rx[t] = h.rx[s], c.ry[t] = h.ry[s], c.rz[t] = h.rz[s];
+            c.r[t] = h.r[s], c.sv[t] = h.sv[s], c.ss[t] = h.ss[s];
+        }
+        for (int64_t *t : {&track_ms, &track_os})
+            if (*t >= 0) *t = remap[static_cast<size_t>(*t)];
+        track_touch = track_ms >= 0 && track_os >= 0 &&
+                      contact({c.x[static_cast<size_t>(track_ms)], c.y[static_cast<size_t>(track_ms)],
+                               c.z[static_cast<size_t>(track_ms)]},
+                              c.r[static_cast<size_t>(track_ms)],
+                              {c.x[static_cast<size_t>(track_os)], c.y[static_cast<size_t>(track_os)],
+                               c.z[static_cast<size_t>(track_os)]},
+                              c.r[static_cast<size_t>(track_os)], pm.box_length);
+        const size_t A2 = h.mem.size();
+        std::vector<int64_t> cnt(A2, 0), flat;
+        flat.reserve(S2);
+        for (size_t a = 0; a < A2; ++a) {
+            for (int64_t s : h.mem[a]) flat.push_back(remap[static_cast<size_t>(s)]);
+            cnt[a] = static_cast<int64_t>(h.mem[a].size());
+            nsph[a] = static_cast<int>(cnt[a]);
+        }
+        mcac_box_destroy(box);
+        box = nullptr;
+        N = S2;
+        n_ids = A2;
+        int rc = create_box();
+        if (rc) return rc;
+        std::vector<double> pos3(3 * A2), rcm3(3 * A2);
+        for (size_t a = 0; a < A2; ++a)
+            for (size_t l = 0; l < 3; ++l) {
+                pos3[3 * a + l] = com[a][l];
+                rcm3[3 * a + l] = rcm[a][l];
+            }
+        rc = mcac_box_upload(box, static_cast<int64_t>(N), c.x.data(), c.y.data(), c.z.data(), c.r.data(),
+                             c.sv.data(), c.ss.data(), c.rx.data(), c.ry.data(), c.rz.data(), static_cast<int64_t>(A2),
+                             pos3.data(), rcm3.data(), rmax.data(), vol.data(), surf.data(), cnt.data(), flat.data());
+        if (rc) return rc;
+        rc = box_modes();
+        if (rc) return rc;
+        live.reset(A2);
+        for (size_t a = 0; a < A2; ++a)
+            if (cnt[a] == 0) live.erase(a);
+        if (!sphere_r.empty()) sphere_r = c.r;
         return 0;
     }
 
@@ -899,19 +1214,42 @@ struct Simulation {
         deltatemps = deltatemps * (move / full + static_cast<double>(1 - 1));
         ptime[U] += deltatemps;
         pm.time = pm.time + deltatemps;
+        // calcul.cpp:155-172: the reactions of every aggregate, or of the moved
+        // one over its own time step scaled like the global one (calcul.cpp:108, 148)
+        bool split_ev = false, disappear = false;
+        const bool indiv = pm.individual_surf_reactions;
         if (pm.with_surface_reactions) {
-            rc = grow(deltatemps);
+            const double dt_indiv = ts[U] * (move / full + static_cast<double>(1 - 1));
+            track_ms = c.moving_sphere;
+            track_os = c.other_sphere;
+            track_touch = true;
+            rc = react(indiv ? dt_indiv : deltatemps, indiv ? num : -1, &split_ev, &disappear);
             if (rc) return rc;
         }
         bool merged = false;
-        if (contact_flag) {
+        mcac_contact cm = c;  // the contact as the merge sees it after the reactions
+        bool try_merge = contact_flag && (!indiv || !(split_ev || disappear));  // calcul.cpp:175-181
+        if (try_merge && (split_ev || disappear)) {
+            cm.moving_sphere = track_ms;
+            cm.other_sphere = track_os;
+            const bool both = live.alive(static_cast<size_t>(c.moving_agg)) && live.alive(static_cast<size_t>(c.other_agg));
+            if (track_ms < 0 || track_os < 0 || !track_touch) {
+                try_merge = false;  // AggregatList::merge returns false (aggregat_list.cpp:369-378)
+            } else if (!both) {
+                // Aggregate::merge refuses a contact whose aggregate is gone and
+                // AggregatList::merge throws (aggregat_list.cpp:390-393)
+                std::fprintf(stderr, "mcac_sim: MergeError at step %lld\n", static_cast<long long>(step_count));
+                return MCAC_MERGE_ERROR;
+            }
+        }
+        if (try_merge) {
             auto tdr0 = Clock::now();
             rc = drain_skipped();
             t_drain += ms_since(tdr0);
             if (rc) return rc;
             mcac_merge_result mr{};
             auto tm0 = Clock::now();
-            rc = mcac_merge(box, &c, &mr);
+            rc = mcac_merge(box, &cm, &mr);
             t_device += ms_since(tm0);
             t_merge_call += ms_since(tm0);
             if (rc) return rc;
@@ -974,10 +1312,21 @@ struct Simulation {
             rc = mcac_release(box);
             if (rc) return rc;
         }
-        event = merged;
+        // calcul.cpp:183-205: after an event every aggregate gets the whole update
+        // (with the reactions of every aggregate, those not touched by the event
+        // were updated with their growth)
+        if (pm.with_surface_reactions && (split_ev || disappear || (indiv && merged)) && live.size() > 0) {
+            rc = update_all();
+            if (rc) return rc;
+        }
+        event = merged || split_ev || disappear;  // calcul.cpp:222
         if (event) {
             pm.n_iter_without_event = 0;
             total_events++;
+        } else {
+            pm.n_iter_without_event++;
+        }
+        if (event && live.size() > 0) {
             // refresh (aggregat_list.cpp:100-108): the largest time step heads the pick table
             if (pm.with_surface_reactions) {
                 const std::vector<int> &ids = live.ids();
@@ -988,10 +1337,8 @@ struct Simulation {
             }
             avg_npp = static_cast<double>(N) / static_cast<double>(live.size());
             pm_stale = true;
-        } else {
-            pm.n_iter_without_event++;
         }
-        if (pm.with_surface_reactions) {
+        if (pm.with_surface_reactions && live.size() > 0) {
             // sort_time_steps(max) at the top of every step (calcul.cpp:99-101),
             // with max refreshed only at events
             auto tp0 = Clock::now();

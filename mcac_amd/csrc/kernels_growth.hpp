@@ -25,9 +25,13 @@ constexpr double kSurfaceFactorD = 4 * kPi;     // constants.hpp:115
 // cell it was built in; *o_left counts the aggregates whose growth took them
 // out of their built cell (their entries are stale: the cell list must be
 // rebuilt before the next search).
-__global__ __launch_bounds__(kBlock) void k_grow_update(DevBox b, const int *ids, int n_ids, double dr, double L,
-                                                       int n_div, double *o_vol, double *o_surf, double *o_rsum,
-                                                       double *o_rg, double *o_rmax, int *o_left) {
+// With grow = 0 the spheres keep their radii, volumes and surfaces (the update
+// alone); o_flag[wave] gets 1 when a sphere of the aggregate is at or below
+// rp_min after growing (oxidation removes it).
+__global__ __launch_bounds__(kBlock) void k_grow_update(DevBox b, const int *ids, int n_ids, double dr, int grow,
+                                                       double rp_min, double L, int n_div, double *o_vol,
+                                                       double *o_surf, double *o_rsum, double *o_rg, double *o_rmax,
+                                                       int *o_left, int *o_flag) {
     const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = static_cast<int>(threadIdx.x & 63);
     if (wave >= n_ids) return;
@@ -38,19 +42,27 @@ __global__ __launch_bounds__(kBlock) void k_grow_update(DevBox b, const int *ids
     // grow, then volume, surface, sum of vol * relative position, radius sum
     // and sum of vol * r^2 in list order
     double V = 0.0, S = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, rs = 0.0, brg = 0.0;
+    int small = 0;
     for (int c0 = 0; c0 < n; c0 += 64) {
         const int i = c0 + lane;
         double r = 0.0, vol = 0.0, surf = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
         if (i < n) {
             const int s = b.members[st + i];
-            r = b.sr[s] + dr;  // PhysicalModel::grow: r + u_sg * dt, dr = u_sg * dt
-            const double r2 = r * r;
-            const double r3 = r2 * r;
-            vol = kVolumeFactorD * r3;
-            surf = kSurfaceFactorD * r2;
-            b.sr[s] = r;
-            b.svol[s] = vol;
-            b.ssurf[s] = surf;
+            if (grow) {
+                r = b.sr[s] + dr;  // PhysicalModel::grow: r + u_sg * dt, dr = u_sg * dt
+                const double r2 = r * r;
+                const double r3 = r2 * r;
+                vol = kVolumeFactorD * r3;
+                surf = kSurfaceFactorD * r2;
+                b.sr[s] = r;
+                b.svol[s] = vol;
+                b.ssurf[s] = surf;
+            } else {
+                r = b.sr[s];
+                vol = b.svol[s];
+                surf = b.ssurf[s];
+            }
+            small |= (r <= rp_min) ? 1 : 0;
             rx = b.srx[s];
             ry = b.sry[s];
             rz = b.srz[s];
@@ -133,6 +145,55 @@ __global__ __launch_bounds__(kBlock) void k_grow_update(DevBox b, const int *ids
         o_rg[wave] = rg;
         o_rmax[wave] = rmax;
     }
+    const bool any_small = __any(small);
+    if (lane == 0) o_flag[wave] = any_small ? 1 : 0;
+}
+
+// Aggregate::split's question (aggregat.cpp:553-584) for each aggregate of
+// ids: are its spheres one connected set under the contact test
+// (sphere_distances.cpp:81-90)?  One wave per aggregate of at most 64
+// spheres: lane i builds sphere i's contact bits, and the set reached from
+// sphere 0 grows by wave-wide ORs until it stops.  o_flag[wave] |= 2 when the
+// aggregate falls apart, or when it has more than 64 spheres (left to the host).
+__global__ __launch_bounds__(kBlock) void k_connected(DevBox b, const int *ids, int n_ids, double L, int *o_flag) {
+    const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    if (wave >= n_ids) return;
+    const int id = ids[wave];
+    const int n = b.acount[id];
+    if (n <= 1) return;
+    if (n > 64) {
+        if (lane == 0) o_flag[wave] |= 2;
+        return;
+    }
+    const int st = b.astart[id];
+    double x = 0.0, y = 0.0, z = 0.0, r = 0.0;
+    if (lane < n) {
+        const int s = b.members[st + lane];
+        x = b.sx[s];
+        y = b.sy[s];
+        z = b.sz[s];
+        r = b.sr[s];
+    }
+    unsigned long long adj = 0ull;
+    for (int j = 0; j < n; ++j) {
+        const double xj = __shfl(x, j, 64), yj = __shfl(y, j, 64), zj = __shfl(z, j, 64), rj = __shfl(r, j, 64);
+        if (lane < n && contact(x, y, z, r, xj, yj, zj, rj, L)) adj |= 1ull << j;
+    }
+    const unsigned long long all = (n == 64) ? ~0ull : ((1ull << n) - 1ull);
+    unsigned long long reached = 1ull;
+    for (int it = 0; it < n; ++it) {
+        unsigned long long mine = ((reached >> lane) & 1ull) ? adj : 0ull;
+        unsigned lo = static_cast<unsigned>(mine), hi = static_cast<unsigned>(mine >> 32);
+        for (int m = 1; m < 64; m <<= 1) {
+            lo |= __shfl_xor(lo, m, 64);
+            hi |= __shfl_xor(hi, m, 64);
+        }
+        const unsigned long long next = reached | (static_cast<unsigned long long>(hi) << 32) | lo;
+        if (next == reached) break;
+        reached = next;
+    }
+    if (lane == 0 && (reached & all) != all) o_flag[wave] |= 2;
 }
 
 }  // namespace mcac_dev

@@ -1693,51 +1693,76 @@ int mcac_box_sphere_owners(mcac_box *b, int64_t *owner) {
     return 0;
 }
 
-int mcac_box_grow(mcac_box *b, double dr, double *volume, double *surface, double *radius_sum, double *rg,
-                  double *rmax) {
+int mcac_box_grow_ids(mcac_box *b, const int64_t *ids, int64_t n_ids, double dr, int grow, double rp_min,
+                      int check_split, double *volume, double *surface, double *radius_sum, double *rg, double *rmax,
+                      int32_t *flags) {
     if (!b || !volume || !surface || !radius_sum || !rg || !rmax || b->inflight || b->inline_pending ||
-        !(dr >= 0.0))
+        !(dr == dr) || (ids && n_ids < 0))
         return MCAC_E_ARGUMENT;
-    const int64_t n = static_cast<int64_t>(b->alive_ids.size());
+    const int64_t n = ids ? n_ids : static_cast<int64_t>(b->alive_ids.size());
     if (n == 0) return 0;
+    if (ids)
+        for (int64_t i = 0; i < n; ++i)
+            if (ids[i] < 0 || ids[i] >= b->n_aggs || b->count[static_cast<size_t>(ids[i])] <= 0) return MCAC_E_ARGUMENT;
+    // layout: five outputs per aggregate (doubles), its flags, the ids (ints), the
+    // count of aggregates that left their cell
+    const size_t N = static_cast<size_t>(n);
+    auto bytes_for = [](size_t m) { return 5 * m * sizeof(double) + 2 * m * sizeof(int) + sizeof(double); };
     if (b->grow_cap < n) {
         if (b->h_grow) HIP_TRY(hipHostFree(b->h_grow));
         if (b->d_grow) HIP_TRY(hipFree(b->d_grow));
         b->h_grow = nullptr;
         b->d_grow = nullptr;
         b->grow_cap = 0;
-        // five outputs per aggregate and the count of aggregates that left their cell
-        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&b->d_grow), (5 * static_cast<size_t>(n) + 1) * sizeof(double)));
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&b->h_grow), (5 * static_cast<size_t>(n) + 1) * sizeof(double),
-                              hipHostMallocDefault));
+        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&b->d_grow), bytes_for(N)));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&b->h_grow), bytes_for(N), hipHostMallocDefault));
         b->grow_cap = n;
     }
-    const size_t N = static_cast<size_t>(n);
     double *g = b->d_grow;
+    int *d_flag = reinterpret_cast<int *>(g + 5 * N);
+    int *d_ids = d_flag + N;
+    int *left = d_ids + N;
     hipStream_t st = bstream(b);
+    const int *idp = b->d.alive;
+    if (ids) {
+        int *h_ids = reinterpret_cast<int *>(b->h_grow + 5 * N) + N;
+        for (size_t i = 0; i < N; ++i) h_ids[i] = static_cast<int>(ids[i]);
+        HIP_TRY(hipMemcpyAsync(d_ids, h_ids, N * sizeof(int), hipMemcpyHostToDevice, st));
+        idp = d_ids;
+    }
+    HIP_TRY(hipMemsetAsync(left, 0, sizeof(int), st));
     const unsigned blocks = static_cast<unsigned>((n * 64 + kBlock - 1) / kBlock);
-    int *left = reinterpret_cast<int *>(g + 5 * N);
-    HIP_TRY(hipMemsetAsync(left, 0, sizeof(double), st));
-    hipLaunchKernelGGL(k_grow_update, dim3(blocks), dim3(kBlock), 0, st, b->d, b->d.alive, static_cast<int>(n), dr,
-                       b->p.box_length, static_cast<int>(b->p.n_div), g, g + N, g + 2 * N, g + 3 * N, g + 4 * N,
-                       left);
+    hipLaunchKernelGGL(k_grow_update, dim3(blocks), dim3(kBlock), 0, st, b->d, idp, static_cast<int>(n), dr,
+                       grow ? 1 : 0, rp_min, b->p.box_length, static_cast<int>(b->p.n_div), g, g + N, g + 2 * N,
+                       g + 3 * N, g + 4 * N, left, d_flag);
+    if (check_split)
+        hipLaunchKernelGGL(k_connected, dim3(blocks), dim3(kBlock), 0, st, b->d, idp, static_cast<int>(n),
+                           b->p.box_length, d_flag);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(b->h_grow, g, (5 * N + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpyAsync(b->h_grow, g, 5 * N * sizeof(double) + N * sizeof(int), hipMemcpyDeviceToHost, st));
     int n_left = 0;
-    std::memcpy(&n_left, b->h_grow + 5 * N, sizeof(int));
+    HIP_TRY(hipMemcpyAsync(&n_left, left, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const int *h_flag = reinterpret_cast<const int *>(b->h_grow + 5 * N);
     for (size_t w = 0; w < N; ++w) {
-        const int id = b->alive_ids[w];
+        const int id = ids ? static_cast<int>(ids[w]) : b->alive_ids[w];
         if (b->count[static_cast<size_t>(id)] <= 0) continue;
         volume[id] = b->h_grow[w];
         surface[id] = b->h_grow[N + w];
         radius_sum[id] = b->h_grow[2 * N + w];
         rg[id] = b->h_grow[3 * N + w];
         rmax[id] = b->h_grow[4 * N + w];
+        if (flags) flags[id] = h_flag[w];
     }
     if (n_left > 0) b->cells_stale = true;  // else the search's copies were refreshed in place
     b->mirrors_exact = false;
     return 0;
+}
+
+int mcac_box_grow(mcac_box *b, double dr, double *volume, double *surface, double *radius_sum, double *rg,
+                  double *rmax) {
+    if (!(dr >= 0.0)) return MCAC_E_ARGUMENT;
+    return mcac_box_grow_ids(b, nullptr, 0, dr, 1, -1.0, 0, volume, surface, radius_sum, rg, rmax, nullptr);
 }
 
 int mcac_box_set_engine(mcac_box *b, int blocks) {
