@@ -42,6 +42,14 @@ typedef struct mcac_merge_rec {
 
 /* ini_text: content of an MCAC parameter file.  output_dir: directory for
  * advancement.dat and params.ini (NULL or "" disables file output).
+ * Options the path runs: collisions and merges with every volsurf_method but
+ * sbl; surface reactions with volsurf_method none (growth, or oxidation with
+ * sphere removal and splits; for every aggregate or the moved one with
+ * individual_surf_reactions; full_aggregate_update_frequency 1); domain
+ * duplication with volsurf_method none.  Others (nucleation, potentials,
+ * flame coupling, domain reduction, pick_method last, a clock seed, sparser
+ * whole updates) make mcac_sim_create fail with MCAC_INPUT_ERROR and a
+ * message naming the option.
  * Environment: MCAC_PRECISION (0, 1, 2) selects the box's contact-search
  * precision (mcac_box_params.precision; results are identical);
  * MCAC_PIPELINE_DEPTH (1..6, default 3) the MC steps kept in flight;

@@ -555,6 +555,7 @@ struct Simulation {
         if (rc) return rc;
         if (pm.with_surface_reactions) {  // the radii have grown since the placement
             rc = mcac_box_download_radii(box, sphere_r.data(), nullptr, nullptr);
+            if (rc == 0) rc = refresh_overlaps();
             if (rc) return rc;
         }
         std::vector<double> pos3(3 * S);
@@ -888,6 +889,46 @@ struct Simulation {
     int64_t track_ms = -1, track_os = -1;
     bool track_touch = false;  // they touch after the reactions (sphere_distances.cpp:81-90)
 
+    // update_distances_and_overlapping (aggregat.cpp:719-764) of one aggregate:
+    // mean overlap and coordination number of its sphere pairs in contact
+    void overlaps(const HostState &h, size_t id) {
+        const std::vector<int64_t> &m = h.mem[id];
+        double ovl = 0.0;
+        int inter = 0;
+        for (size_t i = 0; i < m.size(); ++i)
+            for (size_t j = i + 1; j < m.size(); ++j) {
+                const size_t a = static_cast<size_t>(m[i]), b = static_cast<size_t>(m[j]);
+                const double ex = h.rx[a] - h.rx[b], ey = h.ry[a] - h.ry[b], ez = h.rz[a] - h.rz[b];
+                const double dist = std::sqrt(ex * ex + ey * ey + ez * ez);
+                if (dist <= (1. + kCoordinationEpsilon) * (h.r[a] + h.r[b])) {
+                    ovl += 2.0 * ((h.r[a] + h.r[b] - dist) / (h.r[a] + h.r[b]));
+                    inter += 2;
+                }
+            }
+        overlapping[id] = inter > 0 ? ovl / static_cast<double>(inter) : 0.0;
+        coord[id] = inter > 0 ? static_cast<double>(inter) / static_cast<double>(m.size()) : 0.0;
+    }
+    // With surface reactions every step's whole update changes the overlaps,
+    // which only the snapshots read: they are computed when one is written.
+    int refresh_overlaps() {
+        const size_t S = N, A = n_ids;
+        HostState h;
+        for (auto *v : {&h.rx, &h.ry, &h.rz, &h.r}) v->resize(S);
+        int rc = mcac_box_download_spheres(box, nullptr, nullptr, nullptr, h.rx.data(), h.ry.data(), h.rz.data());
+        if (rc == 0) rc = mcac_box_download_radii(box, h.r.data(), nullptr, nullptr);
+        std::vector<int64_t> cnt(A), flat(S);
+        if (rc == 0) rc = mcac_box_download_members(box, cnt.data(), flat.data());
+        if (rc) return rc;
+        h.mem.resize(A);
+        size_t off = 0;
+        for (size_t a = 0; a < A; ++a) {
+            h.mem[a].assign(flat.begin() + static_cast<long>(off), flat.begin() + static_cast<long>(off + cnt[a]));
+            off += static_cast<size_t>(cnt[a]);
+        }
+        for (int id : live.ids()) overlaps(h, static_cast<size_t>(id));
+        return 0;
+    }
+
     // Aggregate::update (aggregat.cpp:284-288, 321-483, method none) of one
     // aggregate on the host state: the same list-order sums and operations as
     // the device update, plus update_distances_and_overlapping (:719-764).
@@ -914,21 +955,7 @@ struct Simulation {
             rm = smax(rm, h.r[s] + dc);
             arg = arg + h.sv[s] * (dc * dc);
         }
-        // overlapping and coordination number (aggregat.cpp:719-764)
-        double ovl = 0.0;
-        int inter = 0;
-        for (size_t i = 0; i < m.size(); ++i)
-            for (size_t j = i + 1; j < m.size(); ++j) {
-                const size_t a = static_cast<size_t>(m[i]), b = static_cast<size_t>(m[j]);
-                const double ex = h.rx[a] - h.rx[b], ey = h.ry[a] - h.ry[b], ez = h.rz[a] - h.rz[b];
-                const double dist = std::sqrt(ex * ex + ey * ey + ez * ez);
-                if (dist <= (1. + kCoordinationEpsilon) * (h.r[a] + h.r[b])) {
-                    ovl += 2.0 * ((h.r[a] + h.r[b] - dist) / (h.r[a] + h.r[b]));
-                    inter += 2;
-                }
-            }
-        overlapping[id] = inter > 0 ? ovl / static_cast<double>(inter) : 0.0;
-        coord[id] = inter > 0 ? static_cast<double>(inter) / static_cast<double>(m.size()) : 0.0;
+        overlaps(h, id);
         const size_t s0 = static_cast<size_t>(m[0]);
         const double L = pm.box_length;
         com[id] = {periodic_position(h.x[s0] + cx, L), periodic_position(h.y[s0] + cy, L),
