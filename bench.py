@@ -270,11 +270,11 @@ def main() -> int:
                     help="boxes of the 512-box ensemble list run to completion (0: skip)")
     ap.add_argument("--plain-stride", type=int, default=32,
                     help="every n-th box of the ensemble list is also run one at a time on the plain path")
-    ap.add_argument("--compare-stride", type=int, default=16,
+    ap.add_argument("--compare-stride", type=int, default=4,
                     help="every n-th box of the ensemble list (with N <= --compare-nmax) is run on the CPU too")
     ap.add_argument("--no-plain-ensemble", dest="plain_ensemble", action="store_false",
                     help="skip running the ensemble sample again one box at a time")
-    ap.add_argument("--compare-nmax", type=float, default=4000,
+    ap.add_argument("--compare-nmax", type=float, default=8000,
                     help="largest box of the ensemble sample also run on the CPU for the runs/hour comparison")
     ap.add_argument("--serving-requests", type=int, default=16, help="serving requests (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=4.0, help="serving arrival rate, requests/s")
