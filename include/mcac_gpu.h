@@ -316,6 +316,9 @@ int mcac_swept_sphere_batch_noalloc(int64_t n, const double *p1, const double *r
 
 /* Library self-description for loaders and tests. */
 const char *mcac_version(void);
+/* Free and total device memory of a GPU (hipMemGetInfo; no reference
+ * counterpart), for drivers that size HBM budgets and for tests. */
+int mcac_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes);
 
 #ifdef __cplusplus
 }

@@ -58,7 +58,7 @@ GPU_SYMBOLS = (
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
     "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
     "mcac_swept_sphere_batch_noalloc", "mcac_box_set_engine", "mcac_box_engine_stats", "mcac_box_grow",
-    "mcac_box_grow_ids",
+    "mcac_box_grow_ids", "mcac_device_memory",
     "mcac_box_download_radii", "mcac_box_download_members",
 )
 SIM_SYMBOLS = (
@@ -130,6 +130,7 @@ def load_gpu():
         L.mcac_box_prune_stats.argtypes = [_P, _I64P]
         L.mcac_swept_sphere_batch.argtypes = [_I64, _DP, _DP, _DP, _DP, _DP, _DP, _D, _DP, _I]
         L.mcac_swept_sphere_batch_noalloc.argtypes = [_I64, _DP, _DP, _DP, _DP, _DP, _DP, _D, _DP, _I]
+        L.mcac_device_memory.argtypes = [_I, _I64P, _I64P]
         L.mcac_group_create.argtypes = [_I, _I, ctypes.POINTER(_P)]
         L.mcac_group_destroy.argtypes = [_P]
         L.mcac_group_add.argtypes = [_P, _P]
@@ -224,6 +225,13 @@ def arvo_part(balls) -> tuple:
     out = np.zeros(2)
     _check(load_gpu().mcac_arvo_part(len(b) - 1, *(_dp(c) for c in cols), _dp(out)), "mcac_arvo_part")
     return float(out[0]), float(out[1])
+
+
+def device_memory(device: int = 0) -> tuple:
+    """(free, total) bytes of device memory (mcac_device_memory)."""
+    f, t = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    _check(load_gpu().mcac_device_memory(device, _ip(f), _ip(t)), "mcac_device_memory")
+    return int(f[0]), int(t[0])
 
 
 def swept_sphere_batch(p1, r1, p2, r2, direction, dist, box: float, device: int = 0, noalloc: bool = False):

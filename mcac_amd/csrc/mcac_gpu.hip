@@ -870,6 +870,16 @@ int mcac_swept_sphere_batch_noalloc(int64_t n, const double *p1, const double *r
 
 const char *mcac_version(void) { return "mcac_gpu 0.2 (gfx950, FP64 collision path)"; }
 
+int mcac_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return MCAC_E_ARGUMENT;
+    HIP_TRY(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = static_cast<int64_t>(f);
+    *total_bytes = static_cast<int64_t>(t);
+    return 0;
+}
+
 int64_t mcac_box_device_bytes(const mcac_box_params *params) {
     if (!params || params->max_spheres < 1 || params->max_aggregates < 1 || params->n_div < 1 ||
         params->n_div > 1290 || params->volsurf < 0 || params->volsurf > 3)

@@ -52,15 +52,11 @@ def test_sim_bytes_cover_the_duplications_the_mean_size_limit_allows():
 
 @pytest.mark.gpu
 def test_box_allocation_matches_the_estimate():
-    import torch
     n = 400_000
-    torch.cuda.init()
     warm = mcac_amd.Box(1e-6, 10, 1000, 1000)  # runtime and module set-up are not the box's
-    torch.cuda.synchronize()
-    free0, _ = torch.cuda.mem_get_info()
+    free0, _ = mcac_amd.device_memory()
     box = mcac_amd.Box(1e-6, 10, n, n)
-    torch.cuda.synchronize()
-    free1, _ = torch.cuda.mem_get_info()
+    free1, _ = mcac_amd.device_memory()
     est = box_bytes(n, n)
     used = free0 - free1
     print(f"case: box of {n} spheres: estimate {est} B, measured {used} B", flush=True)
