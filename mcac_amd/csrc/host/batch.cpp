@@ -5,6 +5,7 @@
 // which runs one box per process (src/main.cpp).
 #include <algorithm>
 #include <atomic>
+#include <numeric>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -49,6 +50,56 @@ class HbmBudget {
     int64_t cap_;
     std::vector<int64_t> bytes_;
     std::atomic<int64_t> used_{0}, peak_{0};
+};
+
+// The order in which workers take the jobs of a grouped batch.  Jobs that
+// arrive over time are taken in list order.  A batch that is all there at the
+// start goes largest first; when the workers' box slots hold it all at once,
+// it is dealt out ahead by the longest-processing-time rule: each job, largest
+// first, to the worker with the least estimated work so far (work ~ N^2: N
+// monomers take ~12 N steps whose windows grow with the box), so that the
+// largest boxes, whose steps run one after another, spread over the workers
+// instead of gathering on whichever took them first.  Sizes come from
+// mcac_sim_device_bytes, which grows linearly with N.
+class JobQueue {
+  public:
+    JobQueue(const char *const *ini_texts, int64_t n, const double *arrival_s, int workers, int64_t slots)
+        : pos_(static_cast<size_t>(workers), 0) {
+        order_.resize(static_cast<size_t>(n));
+        std::iota(order_.begin(), order_.end(), int64_t(0));
+        if (arrival_s || n == 0) return;
+        std::vector<double> size(static_cast<size_t>(n));
+        for (int64_t j = 0; j < n; ++j)
+            size[static_cast<size_t>(j)] = static_cast<double>(std::max<int64_t>(1, mcac_sim_device_bytes(ini_texts[j])));
+        std::stable_sort(order_.begin(), order_.end(), [&size](int64_t a, int64_t b) {
+            return size[static_cast<size_t>(a)] > size[static_cast<size_t>(b)];
+        });
+        if (slots < n) return;  // largest first from the shared counter
+        dealt_.resize(static_cast<size_t>(workers));
+        std::vector<double> load(static_cast<size_t>(workers), 0.0);
+        for (int64_t j : order_) {
+            const size_t w = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+            dealt_[w].push_back(j);
+            const double x = size[static_cast<size_t>(j)];
+            load[w] += x * x;
+        }
+    }
+    // The next job of worker w, or -1 when none is left for it.
+    int64_t next(int w) {
+        if (!dealt_.empty()) {
+            size_t &p = pos_[static_cast<size_t>(w)];
+            const auto &mine = dealt_[static_cast<size_t>(w)];
+            return p < mine.size() ? mine[p++] : -1;
+        }
+        const int64_t k = next_.fetch_add(1);
+        return k < static_cast<int64_t>(order_.size()) ? order_[static_cast<size_t>(k)] : -1;
+    }
+
+  private:
+    std::vector<int64_t> order_;
+    std::vector<std::vector<int64_t>> dealt_;
+    std::vector<size_t> pos_;  // each touched by its own worker only
+    std::atomic<int64_t> next_{0};
 };
 
 // With MCAC_BATCH_PROGRESS=k (k >= 1) every k-th finished job prints a line
@@ -134,9 +185,9 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
     auto now_s = [t0] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
-    std::atomic<int64_t> next{0};
+    JobQueue queue(ini_texts, n_jobs, arrival_s, workers, static_cast<int64_t>(workers) * boxes_per_worker);
     std::atomic<int> worker_rc{0};
-    auto worker = [&] {
+    auto worker = [&](int w) {
         const int G = boxes_per_worker;
         mcac_group *group = nullptr;
         int grc = mcac_group_create(device, G, &group);
@@ -173,11 +224,8 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
             // admit the jobs that have arrived while there is room
             while (static_cast<int>(act.size()) < G) {
                 if (pending < 0 && !exhausted) {
-                    pending = next.fetch_add(1);
-                    if (pending >= n_jobs) {
-                        pending = -1;
-                        exhausted = true;
-                    }
+                    pending = queue.next(w);
+                    if (pending < 0) exhausted = true;
                 }
                 if (pending < 0) break;
                 const double at = arrival_s ? arrival_s[pending] : 0.0;
@@ -260,7 +308,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
         mcac_group_destroy(group);
     };
     std::vector<std::thread> pool;
-    for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
+    for (int w = 0; w < workers; ++w) pool.emplace_back(worker, w);
     for (auto &t : pool) t.join();
     if (peak_bytes) *peak_bytes = budget.peak();
     if (worker_rc.load()) return worker_rc.load();
