@@ -102,11 +102,28 @@ class JobQueue {
     std::atomic<int64_t> next_{0};
 };
 
+// Where a grouped worker's time went: picking the moves of a round, writing
+// and launching them, reading the results (which includes waiting for the
+// device: wait, summed from the finished runs' own counters).
+struct WorkerTimes {
+    double pick = 0, enqueue = 0, consume = 0, wait = 0;
+    int64_t rounds = 0, moves = 0;
+};
+
 // With MCAC_BATCH_PROGRESS=k (k >= 1) every k-th finished job prints a line
 // to stderr (jobs finished, seconds since the call), so that a long batch shows
-// its progress; off by default.
+// its progress, and each grouped worker prints its WorkerTimes at its end;
+// off by default.
 class Progress {
   public:
+    void worker_times(int w, const WorkerTimes &t) {
+        if (every_ > 0)
+            std::fprintf(stderr,
+                         "batch: worker %d: %lld rounds, %lld moves, pick %.2f s, enqueue %.2f s, consume %.2f s "
+                         "(waiting for the device %.2f s)\n",
+                         w, static_cast<long long>(t.rounds), static_cast<long long>(t.moves), t.pick, t.enqueue,
+                         t.consume, t.wait);
+    }
     Progress(int64_t n_jobs) : n_(n_jobs) {
         if (const char *e = std::getenv("MCAC_BATCH_PROGRESS")) every_ = std::max(0, std::atoi(e));
     }
@@ -189,6 +206,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
     std::atomic<int> worker_rc{0};
     auto worker = [&](int w) {
         const int G = boxes_per_worker;
+        WorkerTimes prof;
         mcac_group *group = nullptr;
         int grc = mcac_group_create(device, G, &group);
         if (grc) {
@@ -261,6 +279,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                 continue;
             }
             // one step of every box that has one due, in one launch
+            const double r0 = now_s();
             boxes.clear();
             aggs.clear();
             dirs.clear();
@@ -279,6 +298,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                 }
             }
             tickets.assign(boxes.size(), 0);
+            const double r1 = now_s();
             int rc = boxes.empty() ? 0
                                    : mcac_move_enqueue_group(group, boxes.data(), aggs.data(), dirs.data(), lpms.data(),
                                                              maxrs.data(), static_cast<int>(boxes.size()),
@@ -289,6 +309,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                 for (size_t k = act.size(); k-- > 0;) end_job(k, rc);
                 continue;
             }
+            const double r2 = now_s();
             std::vector<int> step_rc(act.size(), 0);
             for (size_t q = 0; q < who.size(); ++q) mcac::sim_issued(act[who[q]].sim, tickets[q]);
             for (size_t k = 0; k < act.size(); ++k) {
@@ -299,13 +320,23 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                     if (stepped) steps[act[k].job] += 1;
                 }
             }
+            const double r3 = now_s();
+            prof.pick += r1 - r0;
+            prof.enqueue += r2 - r1;
+            prof.consume += r3 - r2;
+            prof.rounds += 1;
+            prof.moves += static_cast<int64_t>(boxes.size());
             // jobs that ended: finished, step budget spent, or an error
             for (size_t k = act.size(); k-- > 0;) {
                 const bool over = steps[act[k].job] >= max_steps || mcac_sim_finished(act[k].sim);
-                if (step_rc[k] || over) end_job(k, step_rc[k]);
+                if (step_rc[k] || over) {
+                    prof.wait += mcac_sim_scalar(act[k].sim, "t_wait_ms") * 1e-3;
+                    end_job(k, step_rc[k]);
+                }
             }
         }
         mcac_group_destroy(group);
+        progress.worker_times(w, prof);
     };
     std::vector<std::thread> pool;
     for (int w = 0; w < workers; ++w) pool.emplace_back(worker, w);
