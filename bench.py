@@ -301,7 +301,14 @@ def main() -> int:
                     help="contact search: 0 FP64, 1 FP32 broad phase + FP64 solve, 2 as 1 with miss counting")
     ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="HIP hardware queues of this process (GPU_MAX_HW_QUEUES, 1..32; 0: leave as set)")
     args = ap.parse_args()
+    if 1 <= args.hw_queues <= 32:
+        # the host workers' streams each on a queue of their own: HIP's default
+        # of four puts two workers' group launches on one queue, one behind the
+        # other (set before anything starts the HIP runtime in this process)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.tuned:
         from mcac_amd.autotune import best_shape
         shape = best_shape(args.tuned)
@@ -393,7 +400,8 @@ def main() -> int:
         "config": {"workload": "N=1e6 spheres lognormal 30nm/1.25, FV=3e-3, 1700 K, "
                                + ("plain FP64 path, " if args.precision == 0 else "FP32 broad phase + FP64 solve, ")
                                + f"MC steps {warmup}..{warmup + steps} from t=0",
-                   "n_spheres": 1_000_000, "parallelism": "single box, 1 GPU", "precision": args.precision},
+                   "n_spheres": 1_000_000, "parallelism": "single box, 1 GPU", "precision": args.precision,
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "HIP default")},
         "roofline": {"bound": "hbm", "kernel": "k_step" if one_launch else "k_filter", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": filt_ms,

@@ -199,6 +199,17 @@ int mcac_move_enqueue_group(mcac_group *group, mcac_box *const *boxes, const int
                             const double *lpm, const double *maxradius, int n, uint64_t *tickets);
 /* [group launches, moves in group launches, moves launched on their own] */
 int mcac_group_stats(mcac_group *group, int64_t out[3]);
+/* Host seconds spent in mcac_move_enqueue_group on this group: [in all, in
+ * the cell-list rebuilds it ran (with the waits they need), in kernel
+ * launches, waiting for the launch that read a job half to have read it].  No
+ * counterpart in the reference; for tuning a driver loop. */
+int mcac_group_times(mcac_group *group, double out[4]);
+/* Moves the box can take in its next group launch without waiting for the
+ * device: its free in-flight slots and, while steps are in flight, the moves
+ * left before a cell-list rebuild falls due (a rebuild with steps in flight
+ * drains the stream first).  No counterpart in the reference; lets a driver
+ * keep two group launches in flight. */
+int mcac_box_spare_moves(mcac_box *box);
 /* 1 when a move of the box whose sweep reaches `reach` (the mover's rmax +
  * maxradius + lpm) around its centre would currently run in one workgroup of
  * a group launch (its cell window holds few enough aggregates), else 0.  No

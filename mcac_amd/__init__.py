@@ -56,7 +56,7 @@ GPU_SYMBOLS = (
     "mcac_box_download_spheres", "mcac_box_sphere_owners", "mcac_box_download_aggs", "mcac_box_members",
     "mcac_box_set_timing", "mcac_box_timing", "mcac_box_stats", "mcac_box_prune_stats", "mcac_swept_sphere_batch",
     "mcac_version", "mcac_group_create", "mcac_group_destroy", "mcac_group_add", "mcac_move_enqueue_group",
-    "mcac_group_stats", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
+    "mcac_group_stats", "mcac_group_times", "mcac_box_spare_moves", "mcac_box_block_steps", "mcac_neighbour_order", "mcac_arvo_part", "mcac_box_device_bytes",
     "mcac_swept_sphere_batch_noalloc", "mcac_box_set_engine", "mcac_box_engine_stats", "mcac_box_grow",
     "mcac_box_grow_ids", "mcac_device_memory",
     "mcac_box_download_radii", "mcac_box_download_members",
@@ -137,6 +137,8 @@ def load_gpu():
         L.mcac_move_enqueue_group.argtypes = [_P, ctypes.POINTER(_P), _I64P, _DP, _DP, _DP, _I,
                                               ctypes.POINTER(ctypes.c_uint64)]
         L.mcac_group_stats.argtypes = [_P, _I64P]
+        L.mcac_group_times.argtypes = [_P, ctypes.POINTER(ctypes.c_double)]
+        L.mcac_box_spare_moves.argtypes = [_P]
         L.mcac_box_block_steps.argtypes = [_P, _D]
         L.mcac_neighbour_order.argtypes = [_I, _I64P, _I, _I64P]
         L.mcac_arvo_part.argtypes = [_I, _DP, _DP, _DP, _DP, _DP]

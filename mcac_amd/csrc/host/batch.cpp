@@ -107,6 +107,7 @@ class JobQueue {
 // device: wait, summed from the finished runs' own counters).
 struct WorkerTimes {
     double pick = 0, enqueue = 0, consume = 0, wait = 0;
+    double g[4] = {0, 0, 0, 0};  // mcac_group_times: enqueue, rebuilds, launches, half waits
     int64_t rounds = 0, moves = 0;
 };
 
@@ -119,10 +120,10 @@ class Progress {
     void worker_times(int w, const WorkerTimes &t) {
         if (every_ > 0)
             std::fprintf(stderr,
-                         "batch: worker %d: %lld rounds, %lld moves, pick %.2f s, enqueue %.2f s, consume %.2f s "
-                         "(waiting for the device %.2f s)\n",
+                         "batch: worker %d: %lld rounds, %lld moves, pick %.2f s, enqueue %.2f s (rebuilds %.2f s, "
+                         "launches %.2f s, half waits %.2f s), consume %.2f s (waiting for the device %.2f s)\n",
                          w, static_cast<long long>(t.rounds), static_cast<long long>(t.moves), t.pick, t.enqueue,
-                         t.consume, t.wait);
+                         t.g[1], t.g[2], t.g[3], t.consume, t.wait);
     }
     Progress(int64_t n_jobs) : n_(n_jobs) {
         if (const char *e = std::getenv("MCAC_BATCH_PROGRESS")) every_ = std::max(0, std::atoi(e));
@@ -189,7 +190,12 @@ extern "C" int mcac_batch_run(const char *const *ini_texts, const double *arriva
 // and runs rounds: pick up to `depth` steps ahead for every box, enqueue them
 // together (one launch; one block runs a box's steps in order, those behind a
 // contact do nothing), read every box's results.  A box whose job ends leaves
-// the round and the worker admits the next job that has arrived.
+// the round and the worker admits the next job that has arrived.  With two
+// launches in flight (the default; MCAC_GROUP_OVERLAP=0 keeps one) a round
+// enqueues the next launch before it reads the results of the one before, so
+// the host's picks and reads overlap the device's steps: a box's new steps
+// queue behind its steps still in flight and do nothing if one of those finds
+// a contact, exactly as steps behind a contact in one launch.
 extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double *arrival_s, int64_t n_jobs,
                                      int workers, int boxes_per_worker, int depth, int device, int64_t max_steps,
                                      int64_t hbm_budget, double *start_s, double *end_s, int64_t *steps, int *status,
@@ -204,6 +210,8 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
     auto now_s = [t0] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
     JobQueue queue(ini_texts, n_jobs, arrival_s, workers, static_cast<int64_t>(workers) * boxes_per_worker);
     std::atomic<int> worker_rc{0};
+    int launches = 2;
+    if (const char *e = std::getenv("MCAC_GROUP_OVERLAP")) launches = std::atoi(e) != 0 ? 2 : 1;
     auto worker = [&](int w) {
         const int G = boxes_per_worker;
         WorkerTimes prof;
@@ -216,6 +224,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
         struct Active {
             mcac_sim *sim;
             int64_t job;
+            size_t fresh;  // its moves in the newest launch
         };
         std::vector<Active> act;
         auto end_job = [&](size_t k, int rc) {
@@ -238,6 +247,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
         std::vector<double> dirs, lpms, maxrs;
         std::vector<uint64_t> tickets;
         std::vector<size_t> who;
+        std::vector<int> step_rc;
         while (true) {
             // admit the jobs that have arrived while there is room
             while (static_cast<int>(act.size()) < G) {
@@ -259,7 +269,7 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                 char err[256] = {0};
                 mcac_sim *sim = nullptr;
                 int rc = mcac_sim_create(ini_texts[j], nullptr, device, &sim, err, static_cast<int>(sizeof err));
-                if (rc == 0) rc = mcac::sim_join_group(sim, group, depth);
+                if (rc == 0) rc = mcac::sim_join_group(sim, group, depth, launches);
                 if (rc == 0) rc = mcac_sim_init(sim);
                 if (rc) {
                     if (sim) mcac_sim_destroy(sim);
@@ -272,13 +282,15 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                                  rc, err);
                     continue;
                 }
-                act.push_back({sim, j});
+                act.push_back({sim, j, 0});
             }
             if (act.empty()) {
                 if (exhausted && pending < 0) break;
                 continue;
             }
-            // one step of every box that has one due, in one launch
+            // the next launch: up to `depth` steps of every box that has one
+            // due (fewer while its earlier steps are in flight and a cell-list
+            // rebuild is near, which would wait for them)
             const double r0 = now_s();
             boxes.clear();
             aggs.clear();
@@ -288,14 +300,20 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
             who.clear();
             for (size_t k = 0; k < act.size(); ++k) {
                 mcac::GroupMove m{};
-                while (mcac::sim_next_move(act[k].sim, steps[act[k].job], max_steps, &m)) {
+                size_t limit = static_cast<size_t>(depth);
+                if (mcac::sim_in_flight(act[k].sim) > 0)
+                    limit = std::min<size_t>(limit, static_cast<size_t>(mcac_box_spare_moves(mcac::sim_box(act[k].sim))));
+                size_t n = 0;
+                while (n < limit && mcac::sim_next_move(act[k].sim, steps[act[k].job], max_steps, &m)) {
                     boxes.push_back(mcac::sim_box(act[k].sim));
                     aggs.push_back(m.agg);
                     dirs.insert(dirs.end(), m.dir, m.dir + 3);
                     lpms.push_back(m.lpm);
                     maxrs.push_back(m.maxradius);
                     who.push_back(k);
+                    ++n;
                 }
+                act[k].fresh = n;
             }
             tickets.assign(boxes.size(), 0);
             const double r1 = now_s();
@@ -310,11 +328,14 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
                 continue;
             }
             const double r2 = now_s();
-            std::vector<int> step_rc(act.size(), 0);
+            step_rc.assign(act.size(), 0);
             for (size_t q = 0; q < who.size(); ++q) mcac::sim_issued(act[who[q]].sim, tickets[q]);
             for (size_t k = 0; k < act.size(); ++k) {
-                // every step of the box in this launch: run, or skipped behind its contact
-                while (step_rc[k] == 0 && mcac::sim_in_flight(act[k].sim) > 0) {
+                // the box's steps of the launch before (with one launch in
+                // flight: of this one): run, or skipped behind a contact (then
+                // its steps in this launch are skipped too, and read with them)
+                const size_t keep = launches > 1 ? act[k].fresh : 0;
+                while (step_rc[k] == 0 && mcac::sim_in_flight(act[k].sim) > keep) {
                     bool stepped = false;
                     step_rc[k] = mcac::sim_consume(act[k].sim, &stepped);
                     if (stepped) steps[act[k].job] += 1;
@@ -326,15 +347,19 @@ extern "C" int mcac_batch_run_groups(const char *const *ini_texts, const double 
             prof.consume += r3 - r2;
             prof.rounds += 1;
             prof.moves += static_cast<int64_t>(boxes.size());
-            // jobs that ended: finished, step budget spent, or an error
+            // jobs that ended: finished, step budget spent, or an error (a
+            // finished run has nothing in flight: a run ends at a merge, which
+            // skips the steps behind it, or at its step budget, past which no
+            // step is issued)
             for (size_t k = act.size(); k-- > 0;) {
                 const bool over = steps[act[k].job] >= max_steps || mcac_sim_finished(act[k].sim);
-                if (step_rc[k] || over) {
+                if (step_rc[k] || (over && mcac::sim_in_flight(act[k].sim) == 0)) {
                     prof.wait += mcac_sim_scalar(act[k].sim, "t_wait_ms") * 1e-3;
                     end_job(k, step_rc[k]);
                 }
             }
         }
+        mcac_group_times(group, prof.g);
         mcac_group_destroy(group);
         progress.worker_times(w, prof);
     };

@@ -480,7 +480,7 @@ struct Simulation {
         rc = create_box();
         if (rc) return rc;
         if (group) {
-            depth = group_depth;  // steps picked ahead per box and group launch
+            depth = group_depth;  // steps picked ahead per box over its group launches
         } else if (rounds && out_dir.empty() && depth > 1) {
             rc = mcac_group_create(device, 1, &own_group);
             if (rc == 0) rc = mcac_group_add(own_group, box);
@@ -679,7 +679,7 @@ struct Simulation {
     std::deque<Draw> draws;      // taken from the stream, not yet consumed by a step
     std::deque<Pending> flight;  // enqueued on the device, in order
     size_t unticketed = 0;       // newest entries of `flight` picked for a group launch, not enqueued yet
-    size_t group_depth = 1;      // steps per box and group launch
+    size_t group_depth = 1;      // steps of a box in flight over its group launches
     size_t depth = 3;
 
     // the k-th draw after the last consumed step (calcul.cpp:102, tools.cpp:83-84)
@@ -1426,10 +1426,11 @@ struct mcac_sim {
 
 namespace mcac {
 
-int sim_join_group(mcac_sim *sim, mcac_group *group, int depth) {
-    if (!sim || !group || sim->s.initialized || depth < 1 || depth > 8) return MCAC_E_ARGUMENT;
+int sim_join_group(mcac_sim *sim, mcac_group *group, int depth, int launches) {
+    if (!sim || !group || sim->s.initialized || depth < 1 || depth > 8 || launches < 1 || launches > 2)
+        return MCAC_E_ARGUMENT;
     sim->s.group = group;
-    sim->s.group_depth = static_cast<size_t>(depth);
+    sim->s.group_depth = static_cast<size_t>(depth) * static_cast<size_t>(launches);  // steps in flight
     return 0;
 }
 

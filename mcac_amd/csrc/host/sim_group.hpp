@@ -1,7 +1,9 @@
 // Lock-step driving of simulations whose boxes share a group (mcac_group): the
 // ensemble runner picks up to `depth` steps ahead for every box (as the
 // pipelined loop does), enqueues them together (one launch, one block per box)
-// and then reads every box's results.  Internal to the host library.
+// and reads every box's results, up to `launches` launches behind (with 2, it
+// reads a launch's results while the next one runs).  Internal to the host
+// library.
 #pragma once
 #include <cstdint>
 
@@ -17,7 +19,8 @@ struct GroupMove {
     double maxradius;
 };
 
-int sim_join_group(mcac_sim *sim, mcac_group *group, int depth);  // before mcac_sim_init; depth 1..8
+// before mcac_sim_init; depth 1..8 steps per launch, launches 1..2 in flight
+int sim_join_group(mcac_sim *sim, mcac_group *group, int depth, int launches = 1);
 // The next step's move when one is due (not finished, fewer than max_steps
 // done or in flight, fewer than depth in flight); it joins the flight queue.
 bool sim_next_move(mcac_sim *sim, int64_t steps_done, int64_t max_steps, GroupMove *move);

@@ -72,4 +72,67 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(DevBox b, int n_alive) 
     }
 }
 
+// The four kernels above in one workgroup, for grids of at most kSmallCells
+// cells: counts, scan and cursors in LDS, one launch instead of four.  Same
+// result: cstart, the CSR entries (their order inside a cell is the order of
+// the cursor atomics, as above) and every live id's slot and dirty flag.
+constexpr int kSmallCells = 8192;
+constexpr int kBuildThreads = 1024;
+__global__ __launch_bounds__(kBuildThreads) void k_cell_build_small(DevBox b, int n_alive, int n_cells) {
+    __shared__ int cnt[kSmallCells];
+    __shared__ int part[kBuildThreads];
+    const int tid = threadIdx.x;
+    for (int c = tid; c < n_cells; c += kBuildThreads) cnt[c] = 0;
+    __syncthreads();
+    for (int t = tid; t < n_alive; t += kBuildThreads) {
+        const int c = b.acell[b.alive[t]];
+        if (c >= 0) atomicAdd(&cnt[c], 1);
+    }
+    __syncthreads();
+    // exclusive scan: each thread sums a contiguous run of cells, the run
+    // totals are scanned in LDS (Hillis-Steele), then each run is written
+    const int per = (n_cells + kBuildThreads - 1) / kBuildThreads;
+    const int lo = tid * per < n_cells ? tid * per : n_cells;
+    const int hi = lo + per < n_cells ? lo + per : n_cells;
+    int s = 0;
+    for (int i = lo; i < hi; ++i) s += cnt[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < kBuildThreads; off <<= 1) {
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int acc = part[tid] - s;
+    for (int i = lo; i < hi; ++i) {
+        const int v = cnt[i];
+        cnt[i] = acc;
+        b.cstart[i] = acc;
+        acc += v;
+    }
+    if (tid == kBuildThreads - 1) {
+        b.cstart[n_cells] = part[kBuildThreads - 1];
+        b.ss->dirty_count = 0;
+    }
+    __syncthreads();
+    for (int t = tid; t < n_alive; t += kBuildThreads) {
+        const int id = b.alive[t];
+        const int c = b.acell[id];
+        b.dirty[id] = (c >= 0) ? 0 : 1;
+        int pos = -1;
+        if (c >= 0) {
+            pos = atomicAdd(&cnt[c], 1);
+            b.citems[pos] = id;
+            b.cpx[pos] = b.ax[id];
+            b.cpy[pos] = b.ay[id];
+            b.cpz[pos] = b.az[id];
+            b.cpr[pos] = b.armax[id];
+            b.cf[pos] = make_float4(static_cast<float>(b.ax[id]), static_cast<float>(b.ay[id]),
+                                    static_cast<float>(b.az[id]), static_cast<float>(b.armax[id]));
+        }
+        b.cslot[id] = pos;
+    }
+}
+
 }  // namespace mcac_dev

@@ -44,6 +44,11 @@ struct mcac_box {
     int n_cells = 0, n_built = 0;
     int *h_cstart = nullptr;         // pinned copy of the cell starts of the current build (small grids)
     bool cstart_ok = false;
+    bool verify_cells = false;       // MCAC_VERIFY_CELLS=1: host-counted cell starts checked against the device's
+    int *h_alive = nullptr;          // pinned staging of the live ids uploaded by a rebuild
+    int64_t alive_cap = 0;
+    hipEvent_t alive_read = nullptr; // recorded after the upload that reads h_alive
+    std::vector<int> cell_fill;      // host cell counts of a rebuild
     // host mirrors for allocation, launch sizing and the dirty list
     std::vector<int> count, start, cap;
     std::vector<int> alive_ids;  // ascending, may hold dead ids until rebuilt
@@ -70,7 +75,7 @@ struct mcac_box {
     int64_t dead_in_list = 0;
     int64_t pool_used = 0;
     // pinned records (host pointer, device pointer)
-    static constexpr int kRing = 8;  // steps in flight
+    static constexpr int kRing = 16;  // steps in flight (two group launches of 8 per box)
     StepRecord *h_step = nullptr, *d_step = nullptr;  // kRing records
     int inflight = 0;
     unsigned long long oldest = 1;   // oldest ticket not yet consumed
@@ -123,20 +128,25 @@ struct mcac_box {
 
 // Boxes that share one stream; their queued steps go into k_step_batch
 // launches, one block per box.  Jobs and block segments are written to pinned
-// memory, alternating between two halves.
+// memory, in turn into kBuffers job lists, so that a list is written while the
+// launches that read the others are still queued.
 struct mcac_group {
-    static constexpr int kMovesPerBox = 8;  // = mcac_box::kRing, steps a box can have in flight
+    static constexpr int kMovesPerBox = 8;  // = kMaxBlockJobs, steps of a box in one launch
+    static constexpr int kBuffers = 3;      // job lists
     int device = 0;
     hipStream_t stream = nullptr;
     int cap = 0;                           // boxes
-    int job_cap = 0;                       // jobs per half
-    int parity = 0;
-    StepJob *h_jobs = nullptr, *d_jobs = nullptr;  // 2 * job_cap jobs
-    int *h_seg = nullptr, *d_seg = nullptr;        // 2 * (job_cap + 1) segment starts
-    hipEvent_t read[2]{};                           // recorded after the launch reading each half
-    bool pending[2] = {false, false};
+    int job_cap = 0;                       // jobs per list
+    int parity = 0;                        // the list written next
+    StepJob *h_jobs = nullptr, *d_jobs = nullptr;  // kBuffers * job_cap jobs
+    int *h_seg = nullptr, *d_seg = nullptr;        // kBuffers * (job_cap + 1) segment starts
+    hipEvent_t read[kBuffers]{};                    // recorded after the launch reading each list
+    bool pending[kBuffers] = {};
     int64_t calls = 0;
     int64_t launches = 0, batched_steps = 0, single_steps = 0;
+    // host seconds in mcac_move_enqueue_group: in all, in cell-list rebuilds
+    // (with the waits they need), in launches, in waits for a list's reader
+    double t_enqueue = 0, t_rebuild = 0, t_launch = 0, t_half = 0;
 };
 
 namespace {
@@ -281,11 +291,40 @@ int rebuild_alive(mcac_box *b) {
         if (b->count[static_cast<size_t>(id)] > 0) ids.push_back(id);
     b->alive_ids.swap(ids);
     b->dead_in_list = 0;
-    if (!b->alive_ids.empty())
-        HIP_TRY(hipMemcpyAsync(b->d.alive, b->alive_ids.data(), b->alive_ids.size() * sizeof(int),
-                               hipMemcpyHostToDevice, bstream(b)));
+    const size_t n = b->alive_ids.size();
+    if (n == 0) return 0;
+    if (b->h_alive && static_cast<int64_t>(n) <= b->alive_cap) {
+        // through pinned staging, in stream order: no wait unless the last
+        // upload has not read the staging yet
+        HIP_TRY(hipEventSynchronize(b->alive_read));
+        std::memcpy(b->h_alive, b->alive_ids.data(), n * sizeof(int));
+        HIP_TRY(hipMemcpyAsync(b->d.alive, b->h_alive, n * sizeof(int), hipMemcpyHostToDevice, bstream(b)));
+        HIP_TRY(hipEventRecord(b->alive_read, bstream(b)));
+        return 0;
+    }
+    HIP_TRY(hipMemcpyAsync(b->d.alive, b->alive_ids.data(), n * sizeof(int), hipMemcpyHostToDevice, bstream(b)));
     HIP_TRY(hipStreamSynchronize(bstream(b)));
     return 0;
+}
+
+// Cell starts of a build counted on the host from its position mirrors (the
+// device files each live aggregate under ref_cell of its CoM, the same
+// function on the same doubles).  Only when the mirrors hold every aggregate
+// and no step is in flight.
+void count_cells_on_host(mcac_box *b) {
+    const int n = static_cast<int>(b->p.n_div);
+    b->cell_fill.assign(static_cast<size_t>(b->n_cells), 0);
+    for (int id : b->alive_ids) {
+        const size_t a = static_cast<size_t>(id);
+        if (b->count[a] <= 0) continue;
+        b->cell_fill[static_cast<size_t>(ref_cell(b->h_x[a], b->h_y[a], b->h_z[a], n, b->p.box_length))] += 1;
+    }
+    int acc = 0;
+    for (int c = 0; c < b->n_cells; ++c) {
+        b->h_cstart[c] = acc;
+        acc += b->cell_fill[static_cast<size_t>(c)];
+    }
+    b->h_cstart[b->n_cells] = acc;
 }
 
 // CSR cell list from the live ids; clears the dirty list (host and device).
@@ -295,12 +334,17 @@ int rebuild_cells(mcac_box *b) {
         if (rc) return rc;
     }
     const int n_alive = static_cast<int>(b->alive_ids.size());
-    const unsigned g = static_cast<unsigned>(std::min<int64_t>(2048, std::max<int64_t>(1, (n_alive + kBlock - 1) / kBlock)));
-    const unsigned gc = static_cast<unsigned>(std::min<int64_t>(2048, (b->n_cells + kBlock) / kBlock));
-    hipLaunchKernelGGL(k_cell_clear, dim3(gc), dim3(kBlock), 0, bstream(b), b->d, b->n_cells);
-    hipLaunchKernelGGL(k_cell_count, dim3(g), dim3(kBlock), 0, bstream(b), b->d, n_alive);
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, bstream(b), b->d, b->n_cells);
-    hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(kBlock), 0, bstream(b), b->d, n_alive);
+    if (b->n_cells <= kSmallCells && n_alive <= 65536) {
+        hipLaunchKernelGGL(k_cell_build_small, dim3(1), dim3(kBuildThreads), 0, bstream(b), b->d, n_alive, b->n_cells);
+    } else {
+        const unsigned g =
+            static_cast<unsigned>(std::min<int64_t>(2048, std::max<int64_t>(1, (n_alive + kBlock - 1) / kBlock)));
+        const unsigned gc = static_cast<unsigned>(std::min<int64_t>(2048, (b->n_cells + kBlock) / kBlock));
+        hipLaunchKernelGGL(k_cell_clear, dim3(gc), dim3(kBlock), 0, bstream(b), b->d, b->n_cells);
+        hipLaunchKernelGGL(k_cell_count, dim3(g), dim3(kBlock), 0, bstream(b), b->d, n_alive);
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, bstream(b), b->d, b->n_cells);
+        hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(kBlock), 0, bstream(b), b->d, n_alive);
+    }
     HIP_TRY(hipGetLastError());
     b->dirty_count = 0;
     std::fill(b->h_dirty.begin(), b->h_dirty.end(), 0);
@@ -310,10 +354,20 @@ int rebuild_cells(mcac_box *b) {
     b->cells_stale = false;
     b->cstart_ok = false;
     if (b->h_cstart) {
-        // the host passes the visited cells' bounds with each step
-        HIP_TRY(hipMemcpyAsync(b->h_cstart, b->d.cstart, (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
-                               hipMemcpyDeviceToHost, bstream(b)));
-        HIP_TRY(hipStreamSynchronize(bstream(b)));
+        // the host passes the visited cells' bounds with each step: counted
+        // here when the mirrors allow, else read back from the build
+        const bool on_host = b->mirrors_exact && b->inflight == 0;
+        if (on_host) count_cells_on_host(b);
+        if (!on_host || b->verify_cells) {
+            std::vector<int> host(b->h_cstart, b->h_cstart + b->n_cells + 1);
+            HIP_TRY(hipMemcpyAsync(b->h_cstart, b->d.cstart, (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
+                                   hipMemcpyDeviceToHost, bstream(b)));
+            HIP_TRY(hipStreamSynchronize(bstream(b)));
+            if (on_host && !std::equal(host.begin(), host.end(), b->h_cstart)) {
+                std::fprintf(stderr, "mcac_gpu: cell starts counted on the host differ from the device's\n");
+                return MCAC_UNKNOWN_ERROR;
+            }
+        }
         b->cstart_ok = true;
     }
     return 0;
@@ -903,6 +957,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (!b) return MCAC_E_CAPACITY;
     b->p = *params;
     if (const char *e = std::getenv("MCAC_FUSED_MERGE")) b->fused_merge = std::atoi(e) != 0;  // 0: the kernel sequence
+    if (const char *e = std::getenv("MCAC_VERIFY_CELLS")) b->verify_cells = std::atoi(e) != 0;
     int rc = 0;
     auto fail = [&](int code) {
         mcac_box_destroy(b);
@@ -920,6 +975,11 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (b->n_cells + 1 <= (1 << 20) &&
         hipHostMalloc(reinterpret_cast<void **>(&b->h_cstart), (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
                       hipHostMallocDefault) != hipSuccess)
+        return fail(MCAC_E_DEVICE);
+    b->alive_cap = z.Na;
+    if (hipHostMalloc(reinterpret_cast<void **>(&b->h_alive), static_cast<size_t>(z.Na) * sizeof(int),
+                      hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&b->alive_read, hipEventDisableTiming) != hipSuccess)
         return fail(MCAC_E_DEVICE);
     if (hipHostMalloc(reinterpret_cast<void **>(&b->h_step), mcac_box::kRing * sizeof(StepRecord),
                       hipHostMallocCoherent) != hipSuccess ||
@@ -948,9 +1008,6 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     }
     for (auto &e : b->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(MCAC_E_DEVICE);
-    for (auto &row : b->sev)
-        for (auto &e : row)
-            if (hipEventCreate(&e) != hipSuccess) return fail(MCAC_E_DEVICE);
     *out = b;
     return 0;
 }
@@ -963,6 +1020,8 @@ int mcac_box_destroy(mcac_box *b) {
     if (b->h_merge) (void)hipHostFree(b->h_merge);
     if (b->h_alpha) (void)hipHostFree(b->h_alpha);
     if (b->h_cstart) (void)hipHostFree(b->h_cstart);
+    if (b->h_alive) (void)hipHostFree(b->h_alive);
+    if (b->alive_read) (void)hipEventDestroy(b->alive_read);
     if (b->h_ring) (void)hipHostFree(b->h_ring);
     if (b->h_grow) (void)hipHostFree(b->h_grow);
     if (b->d_grow) (void)hipFree(b->d_grow);
@@ -1395,15 +1454,16 @@ int mcac_group_create(int device, int max_boxes, mcac_group **out) {
     g->device = device;
     g->cap = max_boxes;
     g->job_cap = max_boxes * mcac_group::kMovesPerBox;
-    const size_t bytes = 2 * static_cast<size_t>(g->job_cap) * sizeof(StepJob);
-    const size_t seg_bytes = 2 * (static_cast<size_t>(g->job_cap) + 1) * sizeof(int);
+    const size_t bytes = mcac_group::kBuffers * static_cast<size_t>(g->job_cap) * sizeof(StepJob);
+    const size_t seg_bytes = mcac_group::kBuffers * (static_cast<size_t>(g->job_cap) + 1) * sizeof(int);
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void **>(&g->h_jobs), bytes, hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void **>(&g->d_jobs), g->h_jobs, 0) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void **>(&g->h_seg), seg_bytes, hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void **>(&g->d_seg), g->h_seg, 0) != hipSuccess ||
         hipEventCreateWithFlags(&g->read[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&g->read[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&g->read[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->read[2], hipEventDisableTiming) != hipSuccess) {
         mcac_group_destroy(g);
         return MCAC_E_DEVICE;
     }
@@ -1438,41 +1498,50 @@ int mcac_group_add(mcac_group *g, mcac_box *b) {
 
 namespace {
 
-// Job list of the current half: jobs[0..n), block segments seg[0..blocks].
+// Job list being written: jobs[0..n), block segments seg[0..blocks]; opened
+// when its first job is written (open = false: none yet).
 struct JobList {
-    StepJob *jobs;
-    int *seg;
+    StepJob *jobs = nullptr;
+    int *seg = nullptr;
     int n = 0, blocks = 0;
+    bool open = false;
     const mcac_box *last = nullptr;
 };
 
 int group_open(mcac_group *g, JobList &l) {
-    const int half = g->parity;
-    if (g->pending[half]) {  // the launch that read this half has copied its jobs
-        HIP_TRY(hipEventSynchronize(g->read[half]));
-        g->pending[half] = false;
+    const int q = g->parity;
+    if (g->pending[q]) {  // the launch that read this list has copied its jobs
+        const auto t0 = std::chrono::steady_clock::now();
+        HIP_TRY(hipEventSynchronize(g->read[q]));
+        g->t_half += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        g->pending[q] = false;
     }
-    l.jobs = g->h_jobs + static_cast<size_t>(half) * static_cast<size_t>(g->job_cap);
-    l.seg = g->h_seg + static_cast<size_t>(half) * (static_cast<size_t>(g->job_cap) + 1);
+    l.jobs = g->h_jobs + static_cast<size_t>(q) * static_cast<size_t>(g->job_cap);
+    l.seg = g->h_seg + static_cast<size_t>(q) * (static_cast<size_t>(g->job_cap) + 1);
     l.n = l.blocks = 0;
+    l.open = true;
     l.last = nullptr;
     return 0;
 }
 
 int group_flush(mcac_group *g, JobList &l) {
-    if (l.n == 0) return 0;
-    const int half = g->parity;
+    if (!l.open || l.n == 0) return 0;
+    const int q = g->parity;
     l.seg[l.blocks] = l.n;
+    const auto t0 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(k_step_batch, dim3(static_cast<unsigned>(l.blocks)), dim3(kBlock), 0, g->stream,
-                       g->d_jobs + static_cast<size_t>(half) * static_cast<size_t>(g->job_cap),
-                       g->d_seg + static_cast<size_t>(half) * (static_cast<size_t>(g->job_cap) + 1));
+                       g->d_jobs + static_cast<size_t>(q) * static_cast<size_t>(g->job_cap),
+                       g->d_seg + static_cast<size_t>(q) * (static_cast<size_t>(g->job_cap) + 1));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(g->read[half], g->stream));
-    g->pending[half] = true;
-    g->parity ^= 1;
+    HIP_TRY(hipEventRecord(g->read[q], g->stream));
+    g->t_launch += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    g->pending[q] = true;
+    g->parity = (q + 1) % mcac_group::kBuffers;
     g->launches += 1;
     g->batched_steps += l.n;
-    return group_open(g, l);
+    l.open = false;
+    l.last = nullptr;
+    return 0;
 }
 
 }  // namespace
@@ -1482,6 +1551,7 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
     if (!g || n < 0 || n > g->job_cap || (n > 0 && (!boxes || !aggs || !dirs || !lpm || !maxradius || !tickets)))
         return MCAC_E_ARGUMENT;
     // the moves of one box are contiguous and in the order they are to run
+    const auto t_in = std::chrono::steady_clock::now();
     const int64_t call = ++g->calls;
     for (int i = 0; i < n; ++i) {
         mcac_box *b = boxes[i];
@@ -1491,8 +1561,7 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
         b->group_call = call;
     }
     JobList l;
-    int rc = group_open(g, l);
-    if (rc) return rc;
+    int rc = 0;
     for (int i = 0; i < n; ++i) {
         mcac_box *b = boxes[i];
         const int64_t agg = aggs[i];
@@ -1504,12 +1573,26 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
         // enqueue launches, runs after every job already listed; a block
         // runs at most kMaxBlockJobs jobs (a box has at most that many steps
         // in flight, so this never splits a box's moves in practice)
+        if ((i == 0 || boxes[i - 1] != b) && b->inflight == 0 && !b->inline_pending &&
+            (b->cells_stale || b->dirty_count + mcac_group::kMovesPerBox + 3 >= rebuild_threshold(b))) {
+            // a rebuild that this call's moves would make due, run before the
+            // box's first move: nothing of the box is in flight, so it needs no
+            // drain, and its kernels run ahead of the box's jobs on the stream
+            const auto t_r = std::chrono::steady_clock::now();
+            rc = rebuild_cells(b);
+            g->t_rebuild += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_r).count();
+            if (rc) break;
+        }
         const bool full = batch && l.last == b && l.n - l.seg[l.blocks - 1] >= kMaxBlockJobs;
-        if ((!batch || full || (rebuild_due(b) && l.last == b)) && (rc = group_flush(g, l)) != 0) break;
+        const bool rebuild = rebuild_due(b) || b->cells_stale;
+        if ((!batch || full || (rebuild && l.last == b)) && (rc = group_flush(g, l)) != 0) break;
+        if (batch && !l.open && (rc = group_open(g, l)) != 0) break;
         unsigned long long t = 0;
         bool batched = false;
+        const auto t_s = std::chrono::steady_clock::now();
         rc = enqueue_search(b, agg, dirs + 3 * i, lpm[i], maxradius[i], 1, 1, &t, batch ? l.jobs + l.n : nullptr,
                             batch ? &batched : nullptr);
+        if (rebuild) g->t_rebuild += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_s).count();
         if (rc) break;
         if (batched != batch) {
             rc = MCAC_UNKNOWN_ERROR;  // the two decisions must agree
@@ -1527,7 +1610,30 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
     }
     // the steps enqueued before a failing one still run: their results are read as usual
     const int frc = group_flush(g, l);
+    g->t_enqueue += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_in).count();
     return rc ? rc : frc;
+}
+
+int mcac_box_spare_moves(mcac_box *b) {
+    if (!b) return 0;
+    int spare = mcac_box::kRing - b->inflight;
+    if (b->inflight > 0) {
+        // with steps in flight a due rebuild would drain the stream first:
+        // stop short of it (a box with nothing in flight rebuilds early instead)
+        if (b->cells_stale) return 0;
+        const int64_t room = rebuild_threshold(b) - 4 - b->dirty_count;
+        spare = static_cast<int>(std::min<int64_t>(spare, std::max<int64_t>(0, room)));
+    }
+    return std::max(0, spare);
+}
+
+int mcac_group_times(mcac_group *g, double out[4]) {
+    if (!g || !out) return MCAC_E_ARGUMENT;
+    out[0] = g->t_enqueue;
+    out[1] = g->t_rebuild;
+    out[2] = g->t_launch;
+    out[3] = g->t_half;
+    return 0;
 }
 
 int mcac_box_block_steps(mcac_box *b, double reach) {
@@ -1800,6 +1906,10 @@ int mcac_box_engine_stats(mcac_box *b, double out[8]) {
 
 int mcac_box_set_timing(mcac_box *b, int enable) {
     if (!b) return MCAC_E_ARGUMENT;
+    if (enable > 0)  // the per-slot events of timed steps, made on first use
+        for (auto &row : b->sev)
+            for (auto &e : row)
+                if (!e) HIP_TRY(hipEventCreate(&e));
     b->timing = enable < 0 ? 0 : enable;
     b->timing_base = b->seq + 1;  // the next search is the first timed one
     for (int k = 0; k < mcac_box::kRing; ++k) b->slot_timed[k] = 0;

@@ -1,3 +1,4 @@
+import os
 import pathlib
 import sys
 
@@ -5,6 +6,10 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 for p in (str(ROOT), str(ROOT / "tests")):
     if p not in sys.path:
         sys.path.insert(0, p)
+
+# every cell-list build whose starts the host counts from its mirrors is also
+# read back from the device and compared (a mismatch is a status error)
+os.environ.setdefault("MCAC_VERIFY_CELLS", "1")
 
 
 def pytest_configure(config):

@@ -1,7 +1,8 @@
 """Boxes stepped in lock step, one launch per round for all of a worker's boxes
 and up to `depth` steps of each (mcac_batch_run_groups), give the runs each box
 gives alone: the same number of MC steps and the same final physical time, bit
-for bit, as the oracle."""
+for bit, as the oracle; with two launches in flight (the default) and with
+one."""
 import pathlib
 import sys
 
@@ -15,10 +16,11 @@ pytestmark = pytest.mark.gpu
 CFG = pathlib.Path(__file__).parent / "configs"
 
 
-@pytest.mark.parametrize("depth", [1, 6, 8])
-def test_group_runs_match_oracle_runs(depth):
-    print(f"case: 8 boxes (one dense, hand-over path), 2 workers x 3 boxes, depth {depth}, staggered arrivals",
-          flush=True)
+@pytest.mark.parametrize("depth,overlap", [(1, 1), (6, 1), (8, 1), (8, 0)])
+def test_group_runs_match_oracle_runs(depth, overlap, monkeypatch):
+    monkeypatch.setenv("MCAC_GROUP_OVERLAP", str(overlap))
+    print(f"case: 8 boxes (one dense, hand-over path), 2 workers x 3 boxes, depth {depth}, "
+          f"{1 + overlap} launches in flight, staggered arrivals", flush=True)
     inis = [ensemble.box_ini(n, fv, t, seed) for n, fv, t, seed in
             [(600, 3e-3, 1500.0, 21), (900, 1e-3, 1200.0, 22), (1500, 4e-3, 1900.0, 23),
              (700, 2e-3, 1000.0, 24), (2500, 5e-3, 1700.0, 25), (1200, 1e-4, 1600.0, 26),
