@@ -257,6 +257,22 @@ def run_serving(args) -> dict:
                   f"T U[1200,2000] K, until N_agg <= N/10"}
 
 
+LEGS = {"ensemble": run_ensemble, "serving": run_serving, "compare": run_ensemble_compare}
+
+
+def run_leg(name: str) -> dict:
+    """One of the batch legs in a process of its own (this command with --leg):
+    each starts from a fresh HIP runtime, as a separate ensemble or serving job
+    would; run in the process that has just run the N=1e6 box, the ensemble leg
+    took 80-87 s instead of 63 s."""
+    import subprocess
+    cmd = [sys.executable, "-u", str(pathlib.Path(__file__).resolve())] + sys.argv[1:] + ["--leg", name]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"bench leg {name} ended with status {p.returncode}")
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +317,8 @@ def main() -> int:
                     help="contact search: 0 FP64, 1 FP32 broad phase + FP64 solve, 2 as 1 with miss counting")
     ap.add_argument("--timing-stride", type=int, default=64,
                     help="HIP events around every n-th contact search of the timed region")
+    ap.add_argument("--leg", choices=sorted(LEGS), default="",
+                    help="run only this batch leg and print its JSON (the bench runs each leg so, in a child process)")
     ap.add_argument("--hw-queues", type=int, default=8,
                     help="HIP hardware queues of this process (GPU_MAX_HW_QUEUES, 1..32; 0: leave as set)")
     args = ap.parse_args()
@@ -318,8 +336,12 @@ def main() -> int:
               "(independent boxes shard across GPUs with no exchange); use --gpus 1", file=sys.stderr)
         return 2
     os.environ["MCAC_PRECISION"] = str(args.precision)
+    if args.leg:
+        print(json.dumps(LEGS[args.leg](args)), flush=True)
+        return 0
     print("bench: importing torch", file=sys.stderr, flush=True)
     import torch
+    torch.cuda.synchronize()  # torch's HIP runtime starts before the library's
     if args.copy_calibration:
         n = 256 << 20
         src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
@@ -421,12 +443,15 @@ def main() -> int:
                       "avg_candidates_filtered": d["filtered"] / steps,
                       "timed_searches": tm["n_search"], "launch_stats": st, "fp32_prune": prune},
     }
+    # the N=1e6 box gives its device memory and stream back before the other legs
+    del box
+    sim.close()
     if args.ensemble_boxes > 0:
-        out["ensemble"] = run_ensemble(args)
+        out["ensemble"] = run_leg("ensemble")
     if args.serving_requests > 0:
-        out["serving"] = run_serving(args)
+        out["serving"] = run_leg("serving")
     if args.ensemble_boxes > 0 and not args.no_cpu:
-        out["ensemble_vs_cpu"] = run_ensemble_compare(args)
+        out["ensemble_vs_cpu"] = run_leg("compare")
     if not args.no_cpu:
         cb = cpu_baseline(text, args.cpu_steps)
         cb["ratio"] = steps_per_s / cb["value"]
