@@ -974,12 +974,19 @@ __global__ __launch_bounds__(kBlock) void k_step_batch(const StepJob *jobs, cons
     __syncthreads();
     if (nj == 0 || behind_gate(box, args[0])) return;  // every job of the block is behind the same gate
     int ran = 0;
+    // the mover's entry slot and spheres of the next job load while a step
+    // runs: no queued step changes them unless it closes the gate (the slot
+    // holds until the next cell-list build, a move keeps the relative
+    // positions, a merge is a contact)
+    StepPrefetch next = prefetch_step(box, args[0]);
     for (int q = 0; q < nj; ++q) {
         // the previous step's writes (positions, lists) and its use of the
         // shared state are complete before the next step starts
         if (q > 0) __syncthreads();
         if (s_closed) break;
-        const bool closed = step_body(box, args[q], 0u, 1u, true, &cells[q], false);
+        const StepPrefetch cur = next;
+        if (q + 1 < nj) next = prefetch_step(box, args[q + 1]);
+        const bool closed = step_body(box, args[q], 0u, 1u, true, &cells[q], false, nullptr, &cur);
         ran = q + 1;
         if (threadIdx.x == 0 && closed) s_closed = 1;
     }
