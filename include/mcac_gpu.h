@@ -296,11 +296,13 @@ int mcac_box_engine_stats(mcac_box *box, double out[8]);
  * n_merge, first-launch ms (k_step or k_filter), second-launch ms (k_solve)]. */
 int mcac_box_set_timing(mcac_box *box, int enable);
 int mcac_box_timing(mcac_box *box, double out[8]);
-/* Launch statistics: [searches in one launch (k_step), searches in two
- * launches (k_filter + k_solve), one-launch searches finished by a second
- * launch, cell-list rebuilds, member-pool compactions, member-pool slots in
- * use, merges done inside a queued move's launch, merges in their own
- * launch].  No reference counterpart (diagnostics). */
+/* Launch statistics: [searches of small movers (k_step, one launch),
+ * searches of large movers (k_step_grid, one launch with a grid-wide barrier
+ * between filter and pair solve; with MCAC_GRID_SOLVE=0 two launches, k_filter
+ * + k_solve), small-mover searches finished by a second launch, cell-list
+ * rebuilds, member-pool compactions, member-pool slots in use, merges done
+ * inside a queued move's launch, merges in their own launch].  No reference
+ * counterpart (diagnostics). */
 int mcac_box_stats(mcac_box *box, int64_t out[8]);
 /* FP32 broad phase, counted with precision 2: [cell entries tested in FP32,
  * entries passed on to the FP64 filter, entries dropped in FP32 that the FP64

@@ -10,7 +10,8 @@ namespace mcac_dev {
 constexpr int kBlock = 256;            // threads per block (4 waves of 64)
 constexpr int kFilterBlocks = 128;     // contact-search filter grid
 constexpr int kSolveBlocks = 256;      // pair-solve grid (one block per CU)
-constexpr long long kChunk = 4096;     // minimum sphere pairs per solve chunk
+constexpr long long kChunk = 4096;     // sphere pairs per block the host sizes the grid form's grid by
+constexpr long long kMinChunk = 256;   // minimum sphere pairs per solve chunk (one per thread)
 constexpr int kMaxVisit = 2048;        // visited cells handled with an LDS prefix
 constexpr int kMaxPlan = 4096;         // filtered candidates planned in LDS
 constexpr int kOverlapSlots = 16;      // contacts kept per sphere row (multi-kernel merge)
@@ -42,6 +43,9 @@ struct SearchState {
     unsigned long long eng_busy;      // clock ticks of completed steps (finishing block: done wait to done)
     unsigned long long eng_steps;     // steps completed by the engine
     unsigned long long eng_idle;      // clock ticks block 0 waited for posted jobs
+    // k_step_grid: blocks past the filter (grid-wide barrier), and a flag set
+    // when a block gave up waiting at it
+    unsigned int grid_bar, grid_fail;
 };
 
 // Record published to pinned host memory by the last block of a search.

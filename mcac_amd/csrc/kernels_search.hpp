@@ -555,6 +555,7 @@ __device__ __forceinline__ bool finish_step(const DevBox &b, const SearchArgs &a
         s->overflow = 0;
         s->done_solve = 0;
         s->tick = 0ull;
+        s->grid_bar = 0;
     }
     lds_barrier();  // s_ref
     if (a.apply && a.translate_in_finish && m.loaded) {
@@ -641,8 +642,57 @@ __device__ __forceinline__ bool finish_step(const DevBox &b, const SearchArgs &a
     return closed;
 }
 
-__global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
-    if (behind_gate(b, a)) return;  // the host knows it is skipped: no record
+// Lexicographic (d, pair) minimum of the swept-sphere distances over pairs
+// p = first, first + step, ... below end of the mover (spheres members[ssrc +
+// i]) against a candidate of nk spheres (members[sk + j]), p = i nk + j
+// (aggregat_distance.cpp:24-44).  kPairsInFlight pairs have their index and
+// position loads issued together, instead of one dependent chain per pair; the
+// minimum does not depend on the order pairs are taken in.
+constexpr int kPairsInFlight = 4;
+__device__ __forceinline__ void pair_scan(const DevBox &b, const SearchArgs &a, int ssrc, int sk, long long nk,
+                                          long long first, long long end, long long step, double &bd,
+                                          long long &bp) {
+    for (long long p = first; p < end; p += kPairsInFlight * step) {
+        int si[kPairsInFlight], sj[kPairsInFlight];
+#pragma unroll
+        for (int u = 0; u < kPairsInFlight; ++u) {
+            const long long q = p + u * step;
+            si[u] = sj[u] = -1;
+            if (q < end) {
+                const long long i = q / nk;
+                si[u] = b.members[ssrc + i];
+                sj[u] = b.members[sk + (q - i * nk)];
+            }
+        }
+        double xi[kPairsInFlight], yi[kPairsInFlight], zi[kPairsInFlight], ri[kPairsInFlight];
+        double xj[kPairsInFlight], yj[kPairsInFlight], zj[kPairsInFlight], rj[kPairsInFlight];
+#pragma unroll
+        for (int u = 0; u < kPairsInFlight; ++u) {
+            const int i = si[u] >= 0 ? si[u] : 0, j = si[u] >= 0 ? sj[u] : 0;
+            xi[u] = b.sx[i];
+            yi[u] = b.sy[i];
+            zi[u] = b.sz[i];
+            ri[u] = b.sr[i];
+            xj[u] = b.sx[j];
+            yj[u] = b.sy[j];
+            zj[u] = b.sz[j];
+            rj[u] = b.sr[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kPairsInFlight; ++u) {
+            if (si[u] < 0) continue;
+            const double d = swept_sphere(xi[u], yi[u], zi[u], ri[u], xj[u], yj[u], zj[u], rj[u], a.dir[0], a.dir[1],
+                                          a.dir[2], a.lpm, a.L);
+            lexmin(bd, bp, d, p + u * step);
+        }
+    }
+}
+
+// Pair solve of the survivors filtered for step a (aggregat_distance.cpp:24-44):
+// chunks of each candidate's sphere-pair space over the blocks of the grid,
+// per-chunk lexicographic (d, pair) minima, and the last block to finish
+// reduces them per candidate and finishes the step.
+__device__ __forceinline__ void solve_phase(const DevBox &b, const SearchArgs &a) {
     __shared__ long long cpre[kMaxPlan + 1];
     __shared__ long long s_chunk;
     __shared__ bool last;
@@ -655,7 +705,10 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) tpairs += nsrc * b.acount[b.cand_id[c]];
     tpairs = block_sum(tpairs);
     if (threadIdx.x == 0) {
-        long long chunk = kChunk;
+        // chunks sized to spread the pairs over the grid (at least kMinChunk
+        // pairs each), fewer when the partials would not fit
+        long long chunk = (tpairs + gridDim.x - 1) / gridDim.x;
+        if (chunk < kMinChunk) chunk = kMinChunk;
         if (tpairs / chunk + F > a.part_cap) chunk = tpairs / (a.part_cap - F) + 1;
         s_chunk = chunk;
     }
@@ -709,15 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
         const int sk = b.astart[k];
         double bd = kInf;
         long long bp = 0x7fffffffffffffffLL;
-        for (long long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-            long long i = p / nk;
-            long long j = p - i * nk;
-            int si = b.members[ssrc + i];
-            int sj = b.members[sk + j];
-            double d = swept_sphere(b.sx[si], b.sy[si], b.sz[si], b.sr[si], b.sx[sj], b.sy[sj], b.sz[sj], b.sr[sj],
-                                    a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
-            lexmin(bd, bp, d, p);
-        }
+        pair_scan(b, a, ssrc, sk, nk, p0 + threadIdx.x, p1, blockDim.x, bd, bp);
         block_lexmin(bd, bp);
         if (threadIdx.x == 0) {
             b.part_d[g] = bd;
@@ -732,12 +777,15 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     __syncthreads();
     if (!last) return;
     __threadfence();
-    // per-candidate reduction of the chunk partials
+    // per-candidate reduction of the chunk partials: a candidate of a few
+    // chunks by one thread, one of many (planned) by the whole block
+    constexpr long long kThreadChunks = 8;
     for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
         long long g0, g1;
         if (planned) {
             g0 = cpre[c];
             g1 = cpre[c + 1];
+            if (g1 - g0 > kThreadChunks) continue;
         } else {
             g0 = 0;
             for (unsigned int e = 0; e < c; ++e) g0 += chunk_count(nsrc * b.acount[b.cand_id[e]], chunk);
@@ -749,6 +797,21 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
         b.cand_exact[c] = bd;
         b.cand_pair[c] = (bd < kInf) ? bp : -1;
     }
+    if (planned) {
+        for (unsigned int c = 0; c < F; ++c) {
+            const long long g0 = cpre[c], g1 = cpre[c + 1];
+            if (g1 - g0 <= kThreadChunks) continue;  // uniform over the block
+            double bd = kInf;
+            long long bp = 0x7fffffffffffffffLL;
+            for (long long g = g0 + threadIdx.x; g < g1; g += blockDim.x) lexmin(bd, bp, b.part_d[g], b.part_p[g]);
+            block_lexmin(bd, bp);
+            if (threadIdx.x == 0) {
+                b.cand_exact[c] = bd;
+                b.cand_pair[c] = (bd < kInf) ? bp : -1;
+            }
+        }
+    }
+    __syncthreads();  // every candidate's minimum before the ordered scan reads them
     StepTotals t;
     t.nquery = b.ss->nquery;
     t.nfilt = b.ss->nfilt;
@@ -756,6 +819,46 @@ __global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
     t.src_slot = b.cslot[a.src];
     t.tpairs = tpairs;
     finish_step(b, a, F, nsrc, ssrc, t, MoverSphere{0, -1, 0., 0., 0.}, nullptr, true);
+}
+
+__global__ __launch_bounds__(kBlock) void k_solve(DevBox b, SearchArgs a) {
+    if (behind_gate(b, a)) return;  // the host knows it is skipped: no record
+    solve_phase(b, a);
+}
+
+// One launch for a large mover: k_filter's work, a grid-wide barrier in place
+// of the boundary between k_filter and k_solve, then k_solve's work by the same
+// blocks; the same arithmetic in the same order, so the same step.  The grid
+// is small enough to be resident at once (at most kFilterBlocks blocks, half
+// the CUs; one box stream, launches in order); a block that waits past kGridWaitTicks at the
+// barrier gives up and sets grid_fail, the step then publishes no record and
+// the host reports a device error instead of waiting without end.
+constexpr unsigned long long kGridWaitTicks = 2000000;  // 20 ms of the 100 MHz constant clock
+__global__ __launch_bounds__(kBlock) void k_step_grid(DevBox b, SearchArgs a) {
+    if (behind_gate(b, a)) return;  // every block reads the same gate: the host knows it is skipped
+    const unsigned long long tot =
+        block_sum(filter_phase(b, a, blockIdx.x, gridDim.x, CellBounds{0, 0, 0}, false)) & 0xffffffffull;
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) {
+        if (tot) atomicAdd(&b.ss->nquery, tot);
+        __threadfence();  // this block's candidates and counts before its arrival
+        atomicAdd(&b.ss->grid_bar, 1u);
+        const unsigned long long t0 = wall_clock64();
+        int ok = 0;
+        while (wall_clock64() - t0 <= kGridWaitTicks) {
+            if (__hip_atomic_load(&b.ss->grid_bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= gridDim.x) {
+                ok = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!ok) atomicExch(&b.ss->grid_fail, 1u);
+        s_ok = ok;
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    __threadfence();  // every block's candidates are visible to this one
+    solve_phase(b, a);
 }
 
 // One MC step for a small mover, worked on by vg blocks (vb is this one):
@@ -903,15 +1006,7 @@ __device__ __forceinline__ bool step_body(const DevBox &b, const SearchArgs &a, 
             const int sk = sc.sk[c];
             double bd = kInf;
             long long bp = 0x7fffffffffffffffLL;
-            for (long long p = threadIdx.x; p < nsrc * nk; p += blockDim.x) {
-                long long i = p / nk;
-                long long j = p - i * nk;
-                int si = b.members[ssrc + i];
-                int sj = b.members[sk + j];
-                double d = swept_sphere(b.sx[si], b.sy[si], b.sz[si], b.sr[si], b.sx[sj], b.sy[sj], b.sz[sj],
-                                        b.sr[sj], a.dir[0], a.dir[1], a.dir[2], a.lpm, a.L);
-                lexmin(bd, bp, d, p);
-            }
+            pair_scan(b, a, ssrc, sk, nk, threadIdx.x, nsrc * nk, blockDim.x, bd, bp);
             block_lexmin(bd, bp);
             if (threadIdx.x == 0) {
                 sc.ex[c] = bd;

@@ -131,14 +131,19 @@ def test_lognormal_run_bit_exact():
     compare_runs((CFG / "lognormal_small.ini").read_text(), 10 ** 7)
 
 
-@pytest.mark.parametrize("rounds", ["0", "1"])
-def test_dense_run_bit_exact(rounds, monkeypatch):
+@pytest.mark.parametrize("rounds,grid", [("0", "1"), ("1", "1"), ("0", "0")])
+def test_dense_run_bit_exact(rounds, grid, monkeypatch):
     """N=3000 at FV=0.05 to 60 aggregates: every step and merge identical (large candidate sets),
-    with one launch per step and with rounds of up to eight steps per group launch."""
+    with one launch per step and with rounds of up to eight steps per group launch; the large
+    movers' searches in one launch with a grid-wide barrier (k_step_grid) and in two
+    (k_filter + k_solve)."""
     monkeypatch.setenv("MCAC_ROUNDS", rounds)
-    say(f"case: dense N=3000 FV=0.05 -> 60 aggregates, MCAC_ROUNDS={rounds}")
+    monkeypatch.setenv("MCAC_GRID_SOLVE", grid)
+    say(f"case: dense N=3000 FV=0.05 -> 60 aggregates, MCAC_ROUNDS={rounds}, MCAC_GRID_SOLVE={grid}")
     ora, sim = compare_runs((CFG / "dense_small.ini").read_text(), 10 ** 7)
-    say("stats", sim.box().stats())
+    st = sim.box().stats()
+    say("stats", st)
+    assert st["two_launch"] > 0  # large movers took the form under test
 
 
 def test_c2_prefix_bit_exact():
