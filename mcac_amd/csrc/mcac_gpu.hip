@@ -46,6 +46,7 @@ struct mcac_box {
     bool cstart_ok = false;
     bool verify_cells = false;       // MCAC_VERIFY_CELLS=1: host-counted cell starts checked against the device's
     bool grid_solve = true;          // large movers in one launch (k_step_grid); MCAC_GRID_SOLVE=0: k_filter + k_solve
+    int grid_blocks = 32;            // its largest grid (MCAC_GRID_BLOCKS, 1..kFilterBlocks)
     int *h_alive = nullptr;          // pinned staging of the live ids uploaded by a rebuild
     int64_t alive_cap = 0;
     hipEvent_t alive_read = nullptr; // recorded after the upload that reads h_alive
@@ -624,11 +625,12 @@ int enqueue_search(mcac_box *b, int64_t agg, const double dir[3], double lpm, do
         else hipLaunchKernelGGL(k_step, dim3(grid), dim3(kBlock), 0, bstream(b), b->d, a);
     } else if (b->grid_solve) {
         // one launch: the filter grid, widened to the solve chunks that a
-        // survivor as large as the largest aggregate would bring, at most half
-        // the CUs (a block of it holds 81 KB of LDS, one per CU, and all of
-        // them must be resident together for its barrier)
+        // survivor as large as the largest aggregate would bring, at most
+        // grid_blocks (a block of it holds 81 KB of LDS, one per CU, all of
+        // them resident together for its barrier, whose arrivals are atomics
+        // on one counter)
         const double chunks = std::ceil(static_cast<double>(n_mov) * static_cast<double>(b->max_count) / kChunk);
-        const int g2 = std::min(kFilterBlocks, std::max(grid, static_cast<int>(std::min<double>(kFilterBlocks, chunks))));
+        const int g2 = std::min(b->grid_blocks, std::max(grid, static_cast<int>(std::min<double>(kFilterBlocks, chunks))));
         if (timed) hipExtLaunchKernelGGL(k_step_grid, dim3(g2), dim3(kBlock), 0, bstream(b), ev[0], ev[1], 0, b->d, a);
         else hipLaunchKernelGGL(k_step_grid, dim3(g2), dim3(kBlock), 0, bstream(b), b->d, a);
     } else if (timed) {
@@ -969,6 +971,7 @@ int mcac_box_create(const mcac_box_params *params, mcac_box **out) {
     if (const char *e = std::getenv("MCAC_FUSED_MERGE")) b->fused_merge = std::atoi(e) != 0;  // 0: the kernel sequence
     if (const char *e = std::getenv("MCAC_VERIFY_CELLS")) b->verify_cells = std::atoi(e) != 0;
     if (const char *e = std::getenv("MCAC_GRID_SOLVE")) b->grid_solve = std::atoi(e) != 0;
+    if (const char *e = std::getenv("MCAC_GRID_BLOCKS")) b->grid_blocks = std::min(kFilterBlocks, std::max(1, std::atoi(e)));
     int rc = 0;
     auto fail = [&](int code) {
         mcac_box_destroy(b);
