@@ -73,10 +73,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_scatter(DevBox b, int n_alive) 
 }
 
 // The four kernels above in one workgroup, for grids of at most kSmallCells
-// cells: counts, scan and cursors in LDS, one launch instead of four.  Same
-// result: cstart, the CSR entries (their order inside a cell is the order of
-// the cursor atomics, as above) and every live id's slot and dirty flag.
+// cells and at most kSmallBuild live aggregates: counts, scan and cursors in
+// LDS, one launch instead of four.  Same result: cstart, the CSR entries (their
+// order inside a cell is the order of the cursor atomics, as above) and every
+// live id's slot and dirty flag.  Its time grows with the live aggregates
+// (108 us for 13,000 on one workgroup), past kSmallBuild the grid form is
+// faster.
 constexpr int kSmallCells = 8192;
+constexpr int kSmallBuild = 4096;
 constexpr int kBuildThreads = 1024;
 __global__ __launch_bounds__(kBuildThreads) void k_cell_build_small(DevBox b, int n_alive, int n_cells) {
     __shared__ int cnt[kSmallCells];

@@ -336,7 +336,7 @@ int rebuild_cells(mcac_box *b) {
         if (rc) return rc;
     }
     const int n_alive = static_cast<int>(b->alive_ids.size());
-    if (b->n_cells <= kSmallCells && n_alive <= 65536) {
+    if (b->n_cells <= kSmallCells && n_alive <= kSmallBuild) {
         hipLaunchKernelGGL(k_cell_build_small, dim3(1), dim3(kBuildThreads), 0, bstream(b), b->d, n_alive, b->n_cells);
     } else {
         const unsigned g =
