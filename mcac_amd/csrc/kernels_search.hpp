@@ -700,10 +700,24 @@ __device__ __forceinline__ void solve_phase(const DevBox &b, const SearchArgs &a
     if (F > static_cast<unsigned int>(a.cap)) F = static_cast<unsigned int>(a.cap);
     const long long nsrc = a.n_mov;
     const int ssrc = a.start;
+    // a few candidates' labels, sizes and member segments are loaded once,
+    // into LDS, for the plan and the chunks
+    constexpr unsigned int kStaged = 256;
+    __shared__ int s_cid[kStaged], s_cnk[kStaged], s_csk[kStaged];
+    const bool staged = F <= kStaged;
     // plan: chunks of the pair space of each candidate
     long long tpairs = 0;
-    for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) tpairs += nsrc * b.acount[b.cand_id[c]];
-    tpairs = block_sum(tpairs);
+    for (unsigned int c = threadIdx.x; c < F; c += blockDim.x) {
+        const int k = b.cand_id[c];
+        const int nk = b.acount[k];
+        if (staged) {
+            s_cid[c] = k;
+            s_cnk[c] = nk;
+            s_csk[c] = b.astart[k];
+        }
+        tpairs += nsrc * nk;
+    }
+    tpairs = block_sum(tpairs);  // its barrier also publishes the staged candidates
     if (threadIdx.x == 0) {
         // chunks sized to spread the pairs over the grid (at least kMinChunk
         // pairs each), fewer when the partials would not fit
@@ -718,7 +732,7 @@ __device__ __forceinline__ void solve_phase(const DevBox &b, const SearchArgs &a
     long long nchunks = 0;
     if (planned) {
         for (unsigned int c = threadIdx.x; c < F; c += blockDim.x)
-            cpre[c] = chunk_count(nsrc * b.acount[b.cand_id[c]], chunk);
+            cpre[c] = chunk_count(nsrc * (staged ? s_cnk[c] : b.acount[b.cand_id[c]]), chunk);
         __syncthreads();
         block_exclusive_scan(cpre, static_cast<int>(F));
         nchunks = cpre[F];
@@ -754,12 +768,12 @@ __device__ __forceinline__ void solve_phase(const DevBox &b, const SearchArgs &a
             }
             c0 = acc;
         }
-        const int k = b.cand_id[c];
-        const long long nk = b.acount[k];
+        const int k = staged ? s_cid[c] : b.cand_id[c];
+        const long long nk = staged ? s_cnk[c] : b.acount[k];
         const long long Ptot = nsrc * nk;
         const long long p0 = (g - c0) * chunk;
         const long long p1 = (p0 + chunk < Ptot) ? p0 + chunk : Ptot;
-        const int sk = b.astart[k];
+        const int sk = staged ? s_csk[c] : b.astart[k];
         double bd = kInf;
         long long bp = 0x7fffffffffffffffLL;
         pair_scan(b, a, ssrc, sk, nk, p0 + threadIdx.x, p1, blockDim.x, bd, bp);
