@@ -189,7 +189,10 @@ int wait_seq(mcac_box *b, const unsigned long long *seqp, unsigned long long seq
             }
             if (e == hipSuccess) {
                 if (__atomic_load_n(seqp, __ATOMIC_ACQUIRE) == seq) break;
-                std::fprintf(stderr, "mcac_gpu: stream idle without a published record\n");
+                unsigned int grid_fail = 0;
+                (void)hipMemcpy(&grid_fail, &b->d.ss->grid_fail, sizeof grid_fail, hipMemcpyDeviceToHost);
+                std::fprintf(stderr, "mcac_gpu: stream idle without a published record%s\n",
+                             grid_fail ? " (a large-mover search gave up waiting at its grid-wide barrier)" : "");
                 return MCAC_E_DEVICE;
             }
             if (e != hipErrorNotReady) {
