@@ -305,7 +305,7 @@ def main() -> int:
     ap.add_argument("--tuned", default="",
                     help="shape file of python3 -m mcac_amd.autotune: its workers, boxes per worker and depth")
     ap.add_argument("--workers", type=int, default=8, help="host threads sharing the GPU (ensemble, serving)")
-    ap.add_argument("--group-boxes", type=int, default=64,
+    ap.add_argument("--group-boxes", type=int, default=32,
                     help="boxes per worker stepped in lock step, one launch per round (0: one box per worker)")
     ap.add_argument("--group-depth", type=int, default=8, help="steps of each box per grouped launch (1..8)")
     ap.add_argument("--max-job-steps", type=int, default=3_000_000, help="MC step cap per ensemble/serving job")
