@@ -107,7 +107,7 @@ class JobQueue {
 // device: wait, summed from the finished runs' own counters).
 struct WorkerTimes {
     double pick = 0, enqueue = 0, consume = 0, wait = 0;
-    double g[4] = {0, 0, 0, 0};  // mcac_group_times: enqueue, rebuilds, launches, half waits
+    double g[4] = {0, 0, 0, 0};  // mcac_group_times: enqueue, rebuilds, launches, job-list waits
     int64_t rounds = 0, moves = 0;
 };
 
@@ -121,7 +121,7 @@ class Progress {
         if (every_ > 0)
             std::fprintf(stderr,
                          "batch: worker %d: %lld rounds, %lld moves, pick %.2f s, enqueue %.2f s (rebuilds %.2f s, "
-                         "launches %.2f s, half waits %.2f s), consume %.2f s (waiting for the device %.2f s)\n",
+                         "launches %.2f s, job-list waits %.2f s), consume %.2f s (waiting for the device %.2f s)\n",
                          w, static_cast<long long>(t.rounds), static_cast<long long>(t.moves), t.pick, t.enqueue,
                          t.g[1], t.g[2], t.g[3], t.consume, t.wait);
     }

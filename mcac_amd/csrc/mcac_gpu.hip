@@ -148,7 +148,7 @@ struct mcac_group {
     int64_t launches = 0, batched_steps = 0, single_steps = 0;
     // host seconds in mcac_move_enqueue_group: in all, in cell-list rebuilds
     // (with the waits they need), in launches, in waits for a list's reader
-    double t_enqueue = 0, t_rebuild = 0, t_launch = 0, t_half = 0;
+    double t_enqueue = 0, t_rebuild = 0, t_launch = 0, t_list = 0;
 };
 
 namespace {
@@ -1530,7 +1530,7 @@ int group_open(mcac_group *g, JobList &l) {
     if (g->pending[q]) {  // the launch that read this list has copied its jobs
         const auto t0 = std::chrono::steady_clock::now();
         HIP_TRY(hipEventSynchronize(g->read[q]));
-        g->t_half += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        g->t_list += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         g->pending[q] = false;
     }
     l.jobs = g->h_jobs + static_cast<size_t>(q) * static_cast<size_t>(g->job_cap);
@@ -1649,7 +1649,7 @@ int mcac_group_times(mcac_group *g, double out[4]) {
     out[0] = g->t_enqueue;
     out[1] = g->t_rebuild;
     out[2] = g->t_launch;
-    out[3] = g->t_half;
+    out[3] = g->t_list;
     return 0;
 }
 
