@@ -364,7 +364,8 @@ int rebuild_cells(mcac_box *b) {
         const bool on_host = b->mirrors_exact && b->inflight == 0;
         if (on_host) count_cells_on_host(b);
         if (!on_host || b->verify_cells) {
-            std::vector<int> host(b->h_cstart, b->h_cstart + b->n_cells + 1);
+            std::vector<int> host;
+            if (on_host) host.assign(b->h_cstart, b->h_cstart + b->n_cells + 1);
             HIP_TRY(hipMemcpyAsync(b->h_cstart, b->d.cstart, (static_cast<size_t>(b->n_cells) + 1) * sizeof(int),
                                    hipMemcpyDeviceToHost, bstream(b)));
             HIP_TRY(hipStreamSynchronize(bstream(b)));
