@@ -16,11 +16,14 @@ pytestmark = pytest.mark.gpu
 CFG = pathlib.Path(__file__).parent / "configs"
 
 
-@pytest.mark.parametrize("depth,overlap", [(1, 1), (6, 1), (8, 1), (8, 0)])
-def test_group_runs_match_oracle_runs(depth, overlap, monkeypatch):
+@pytest.mark.parametrize("depth,overlap,verify", [(1, 1, 1), (6, 1, 1), (8, 1, 1), (8, 0, 1), (8, 1, 0)])
+def test_group_runs_match_oracle_runs(depth, overlap, verify, monkeypatch):
+    """verify 0: the cell starts the host counts are used without the read-back
+    check the other tests make, as in production runs."""
     monkeypatch.setenv("MCAC_GROUP_OVERLAP", str(overlap))
+    monkeypatch.setenv("MCAC_VERIFY_CELLS", str(verify))
     print(f"case: 8 boxes (one dense, hand-over path), 2 workers x 3 boxes, depth {depth}, "
-          f"{1 + overlap} launches in flight, staggered arrivals", flush=True)
+          f"{1 + overlap} launches in flight, cell starts checked {verify}, staggered arrivals", flush=True)
     inis = [ensemble.box_ini(n, fv, t, seed) for n, fv, t, seed in
             [(600, 3e-3, 1500.0, 21), (900, 1e-3, 1200.0, 22), (1500, 4e-3, 1900.0, 23),
              (700, 2e-3, 1000.0, 24), (2500, 5e-3, 1700.0, 25), (1200, 1e-4, 1600.0, 26),
