@@ -119,6 +119,36 @@ inline bool contact(const V3 &p1, double r1, const V3 &p2, double r2, double box
     return (distance - dist_contact <= kContactEpsilon);
 }
 
+// Aggregate::split's component search (aggregat.cpp:553-577): each component
+// starts with the first unvisited sphere; a stack of discovered spheres is
+// popped from its back, and each popped sphere scans the unvisited ones in
+// order, appending every one it touches to the component and to the stack.
+template <class Touch> std::vector<std::vector<size_t>> contact_components(size_t n, Touch touch) {
+    std::vector<size_t> unvisited(n);
+    std::iota(unvisited.begin(), unvisited.end(), size_t(0));
+    std::vector<std::vector<size_t>> comps;
+    while (!unvisited.empty()) {
+        std::vector<size_t> comp{unvisited.front()};
+        unvisited.erase(unvisited.begin());
+        std::vector<size_t> discovered = comp;
+        while (!discovered.empty()) {
+            const size_t a1 = discovered.back();
+            discovered.pop_back();
+            for (size_t idx = 0; idx < unvisited.size();) {
+                if (touch(a1, unvisited[idx])) {
+                    comp.push_back(unvisited[idx]);
+                    discovered.push_back(unvisited[idx]);
+                    unvisited.erase(unvisited.begin() + long(idx));
+                } else {
+                    ++idx;
+                }
+            }
+        }
+        comps.push_back(std::move(comp));
+    }
+    return comps;
+}
+
 // sphere_contact.cpp:47-125 — swept-sphere distance to contact
 double swept_sphere(const V3 &pos1, double r1, V3 pos2, double r2, const V3 &dir, double dist,
                     double box) {
@@ -1444,31 +1474,10 @@ struct Sim {
     // id, re-based, updated and placed; AggregatList::split (aggregat_list.cpp:
     // 433-444) then removes the original
     bool split(size_t l) {
-        const size_t n = aggs[l].sph.size();
-        std::vector<size_t> unvisited(n);
-        std::iota(unvisited.begin(), unvisited.end(), size_t(0));
-        std::vector<std::vector<size_t>> comps;
-        while (!unvisited.empty()) {
-            std::vector<size_t> comp{unvisited.front()};
-            unvisited.erase(unvisited.begin());
-            std::vector<size_t> discovered = comp;
-            while (!discovered.empty()) {
-                const size_t a1 = discovered.back();
-                discovered.pop_back();
-                const size_t s1 = aggs[l].sph[a1];
-                for (size_t idx = 0; idx < unvisited.size();) {
-                    const size_t s2 = aggs[l].sph[unvisited[idx]];
-                    if (contact(spos(s1), sr[s1], spos(s2), sr[s2], pm.box_length)) {
-                        comp.push_back(unvisited[idx]);
-                        discovered.push_back(unvisited[idx]);
-                        unvisited.erase(unvisited.begin() + long(idx));
-                    } else {
-                        ++idx;
-                    }
-                }
-            }
-            comps.push_back(std::move(comp));
-        }
+        const auto comps = contact_components(aggs[l].sph.size(), [&](size_t a1, size_t a2) {
+            const size_t s1 = aggs[l].sph[a1], s2 = aggs[l].sph[a2];
+            return contact(spos(s1), sr[s1], spos(s2), sr[s2], pm.box_length);
+        });
         if (comps.size() < 2) return false;
         for (const auto &comp : comps) {
             Agg copy = aggs[l];
@@ -1770,6 +1779,22 @@ double ora_swept_sphere(const double p1[3], double r1, const double p2[3], doubl
 }
 int ora_contact(const double p1[3], double r1, const double p2[3], double r2, double box) {
     return contact({p1[0], p1[1], p1[2]}, r1, {p2[0], p2[1], p2[2]}, r2, box) ? 1 : 0;
+}
+// The split's components of n spheres: `order` receives them one after the
+// other, `starts` each one's first position (n + 1 entries at most, the last
+// one n); returns the number of components.
+int64_t ora_contact_components(int64_t n, const double *x, const double *y, const double *z, const double *r,
+                               double box, int64_t *order, int64_t *starts) {
+    const auto comps = contact_components(static_cast<size_t>(n), [&](size_t a, size_t b) {
+        return contact({x[a], y[a], z[a]}, r[a], {x[b], y[b], z[b]}, r[b], box);
+    });
+    int64_t k = 0;
+    for (size_t c = 0; c < comps.size(); ++c) {
+        starts[c] = k;
+        for (size_t i : comps[c]) order[k++] = static_cast<int64_t>(i);
+    }
+    starts[comps.size()] = k;
+    return static_cast<int64_t>(comps.size());
 }
 void *ora_rng_create(int seed) { return new Rng(static_cast<unsigned>(seed)); }
 void ora_rng_destroy(void *rng) { delete static_cast<Rng *>(rng); }

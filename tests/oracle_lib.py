@@ -66,6 +66,8 @@ def lib():
         L.ora_swept_sphere.restype = _D
         L.ora_swept_sphere.argtypes = [_DP, _D, _DP, _D, _DP, _D, _D]
         L.ora_contact.argtypes = [_DP, _D, _DP, _D, _D]
+        L.ora_contact_components.restype = _I64
+        L.ora_contact_components.argtypes = [_I64] + [_DP] * 4 + [_D, _I64P, _I64P]
         L.ora_rng_create.restype = _P
         L.ora_rng_create.argtypes = [ctypes.c_int]
         L.ora_rng_destroy.argtypes = [_P]
@@ -182,6 +184,18 @@ def swept_sphere(p1, r1, p2, r2, direction, dist, box):
 
 def contact(p1, r1, p2, r2, box):
     return bool(lib().ora_contact(dp(v3(p1)), r1, dp(v3(p2)), r2, box))
+
+
+def contact_components(pos, r, box):
+    """The oracle's split components (aggregat.cpp:553-577) of spheres at pos (n x 3) with radii r."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    r = np.ascontiguousarray(r, np.float64)
+    n = len(r)
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    order = np.zeros(max(n, 1), np.int64)
+    starts = np.zeros(n + 1, np.int64)
+    k = lib().ora_contact_components(n, dp(x), dp(y), dp(z), dp(r), box, ip(order), ip(starts))
+    return [order[starts[c]:starts[c + 1]].tolist() for c in range(int(k))]
 
 
 def caps_intersection(r1, vol1, surf1, r2, vol2, surf2, dist):

@@ -193,3 +193,56 @@ def test_partial_updates_keep_the_volume_until_the_next_whole_update():
             assert not np.array_equal(agg1["volume"], sums)
             partial += 1
     assert whole > 50 and partial > 100
+
+
+def test_split_components_follow_the_reference_stack():
+    """Aggregate::split (aggregat.cpp:553-577) grows each part from its first
+    unvisited sphere with a stack: the sphere popped last scans the unvisited
+    ones in index order.  On this tree (0-1, 0-3, 3-2, 1-4, and 5 alone) that
+    order is 0, 1, 3, 2, 4, worked by hand: 0 finds 1 and 3; 3, popped first,
+    finds 2; 2 finds nothing; 1 finds 4.  A queue would give 0, 1, 3, 4, 2 and a
+    recursive search 0, 1, 4, 3, 2."""
+    pos = np.array([[0, 0, 0], [1.9, 0, 0], [0, 3.8, 0], [0, 1.9, 0], [3.8, 0, 0], [50, 50, 50]], float)
+    r = np.ones(6)
+    assert O.contact_components(pos, r, 1000.0) == [[0, 1, 3, 2, 4], [5]]
+    # a part need not hold sphere 0's neighbours first: 0 alone, then 1-2
+    pos2 = np.array([[0, 0, 0], [10, 0, 0], [11.9, 0, 0]], float)
+    assert O.contact_components(pos2, np.ones(3), 1000.0) == [[0], [1, 2]]
+
+
+def test_split_components_are_the_contact_graph_components():
+    """On random clusters: the parts are the connected components of the
+    contact graph (union-find over the periodic contact test,
+    sphere_distances.cpp:81-90), ordered by their first sphere, each starting
+    with its smallest index, and every later sphere touches an earlier one of
+    its part."""
+    rng = np.random.default_rng(7)
+    L = 9.0
+    for trial in range(300):
+        n = int(rng.integers(2, 13))
+        pos = rng.uniform(0, L, (n, 3))
+        r = rng.uniform(0.6, 1.4, n)
+        d = pos[:, None, :] - pos[None, :, :]
+        d = d - L * np.round(d / L)
+        touch = (d ** 2).sum(axis=2) - (r[:, None] + r[None, :]) ** 2 <= CONTACT_EPS
+        parent = list(range(n))
+
+        def find(i):
+            while parent[i] != i:
+                parent[i] = parent[parent[i]]
+                i = parent[i]
+            return i
+        for i in range(n):
+            for j in range(i + 1, n):
+                if touch[i, j]:
+                    parent[find(i)] = find(j)
+        want = {}
+        for i in range(n):
+            want.setdefault(find(i), set()).add(i)
+        got = O.contact_components(pos, r, L)
+        assert sorted(map(frozenset, got), key=min) == sorted(map(frozenset, want.values()), key=min), trial
+        assert [c[0] for c in got] == sorted(min(c) for c in want.values())
+        for c in got:
+            assert c[0] == min(c)
+            for k in range(1, len(c)):
+                assert touch[c[k], c[:k]].any(), (trial, c)
