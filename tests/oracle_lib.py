@@ -1,6 +1,6 @@
 """ctypes binding of the oracle (oracle/libmcac_oracle.so) — test infrastructure.
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+Only tests/, __graft_entry__.smoke() and bench.py's CPU legs (cpu_baseline, ensemble_vs_cpu) use this.
 """
 import ctypes
 import pathlib
