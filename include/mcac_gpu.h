@@ -144,7 +144,7 @@ int mcac_merge(mcac_box *box, const mcac_contact *contact, mcac_merge_result *ou
  * so the caller can prepare and enqueue the next step while this one runs.  A
  * step that finds a contact closes a gate: steps enqueued after it do nothing
  * and report skipped = 1 from mcac_move_result, until mcac_release is called
- * (after the caller has merged).  At most 8 steps may be in flight. */
+ * (after the caller has merged).  At most 16 steps may be in flight. */
 int mcac_move_enqueue(mcac_box *box, int64_t agg, const double dir[3], double lpm, double maxradius,
                       uint64_t *ticket);
 /* Waits for an enqueued step, in ticket order.  *skipped is 0 when the step
@@ -188,7 +188,8 @@ int mcac_group_create(int device, int max_boxes, mcac_group **out);
 int mcac_group_destroy(mcac_group *group);
 int mcac_group_add(mcac_group *group, mcac_box *box);
 /* mcac_move_enqueue for n moves of boxes of the group: the moves of one box
- * are contiguous, in the order they are to run (at most 8 in flight per box),
+ * are contiguous, in the order they are to run (at most 16 in flight per
+ * box; past 8 of a box in one call the list goes to a further launch),
  * n <= 8 * max_boxes.  One launch takes every move that fits one workgroup,
  * one workgroup running each box's moves in order; the others take their
  * usual launches on the group stream, in order with the rest.  dirs holds 3

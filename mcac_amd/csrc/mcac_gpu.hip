@@ -1589,8 +1589,8 @@ int mcac_move_enqueue_group(mcac_group *g, mcac_box *const *boxes, const int64_t
                            window_items(b, reach) <= kBlockItems;
         // a step launched on its own, or the cell-list rebuild that the
         // enqueue launches, runs after every job already listed; a block
-        // runs at most kMaxBlockJobs jobs (a box has at most that many steps
-        // in flight, so this never splits a box's moves in practice)
+        // runs at most kMaxBlockJobs jobs (the batch runner gives a box at
+        // most that many moves per call, so this never splits them there)
         if ((i == 0 || boxes[i - 1] != b) && b->inflight == 0 && !b->inline_pending &&
             (b->cells_stale || b->dirty_count + mcac_group::kMovesPerBox + 3 >= rebuild_threshold(b))) {
             // a rebuild that this call's moves would make due, run before the
